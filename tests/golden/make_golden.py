@@ -1,0 +1,93 @@
+#!/usr/bin/env python3
+"""Write the golden vectors under tests/golden/ from the reference's own build.
+
+Run in the development container only (it needs the reference sources):
+
+    make -C oracle ref          # compiles the reference's sources into oracle/_ref/
+    python tests/golden/make_golden.py
+
+``oracle/_ref/ref_harness`` (source: ``oracle/ref_harness.c``) calls the
+reference's public API -- lb_collide, lb_propagation, field_grad_compute,
+phi_force_calculation, phi_cahn_hilliard, field_phi_init_spinodal -- on
+seeded inputs and writes raw float64 arrays.  This script packs each case
+into ``<case>.npz``: inputs, outputs and the parameters.  Large arrays are
+stored as a SHA-256 of their bytes plus values at sampled sites, which is
+enough to check bit-exactness and a relative tolerance.
+
+The fixtures are data (inputs and expected outputs); no reference source
+is copied.
+"""
+
+import hashlib
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+HARNESS = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+
+# C2 parameters (symmetric free energy, Cahn-Hilliard)
+A, B, K, M = -0.0625, 0.0625, 0.04, 0.45
+
+# name: (harness args, shape, arrays stored in full, arrays stored by hash+samples)
+CASES = {
+    "collide_m10": (["collide", 8, 0, 1, 0.8], 8, ["f_in", "force", "f_out", "rho", "u"], []),
+    "collide_bgk": (["collide", 8, 1, 2, 0.8], 8, ["f_in", "force", "f_out", "rho", "u"], []),
+    "collide_trt": (["collide", 8, 2, 3, 0.8], 8, ["f_in", "force", "f_out", "rho", "u"], []),
+    "collide_m10_tau1": (["collide", 8, 0, 4, 1.0], 8, ["f_in", "force", "f_out", "rho", "u"], []),
+    "prop": (["prop", 8, 5], 8, ["f_in", "f_out"], []),
+    "force_stress": (["force", 8, 6, 0, A, B, K], 8, ["phi", "grad", "delsq", "mu", "force"], []),
+    "force_gradmu": (["force", 8, 7, 1, A, B, K], 8, ["phi", "grad", "delsq", "mu", "force"], []),
+    "ch_order1": (["ch", 8, 8, 1, A, B, K, M], 8, ["phi_in", "u", "phi_out"], []),
+    "ch_order2": (["ch", 8, 9, 2, A, B, K, M], 8, ["phi_in", "u", "phi_out"], []),
+    "ch_order3": (["ch", 8, 10, 3, A, B, K, M], 8, ["phi_in", "u", "phi_out"], []),
+    # full binary step, C2 physics (tau = 1), spinodal start
+    "binary16_o1_s20": (["binary", 16, 20, 8361235, 1, A, B, K, M, 1.0, 0.0, 0.1, 10], 16,
+                        ["phi_init", "phi_10", "phi", "rho", "u"], ["f"]),
+    "binary16_o3_s10": (["binary", 16, 10, 4242, 3, A, B, K, M, 1.0, 0.0, 0.1], 16,
+                        ["phi", "rho", "u"], ["f"]),
+    "binary16_o2_s5": (["binary", 16, 5, 77, 2, A, B, K, M, 0.8, 0.1, 0.2], 16,
+                       ["phi", "rho", "u"], ["f"]),
+    "binary32_o1_s100": (["binary", 32, 100, 2024, 1, A, B, K, M, 1.0, 0.0, 0.1], 32,
+                         [], ["phi", "rho", "u", "f"]),
+    # C1: plain D3Q19 MRT, 32^3, tau 0.8, 100 steps, random-velocity start
+    "single32_s100": (["single", 32, 100, 12345, 0.8], 32, [], ["u_init", "f_init", "f", "rho", "u"]),
+}
+
+NSAMPLE = 256
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("build the reference harness first: make -C oracle ref")
+    rng = np.random.default_rng(20261202)
+    for name, (args, n, full, hashed) in CASES.items():
+        with tempfile.TemporaryDirectory() as tmp:
+            cmd = [HARNESS, tmp] + [str(a) for a in args]
+            subprocess.run(cmd, check=True, env=dict(os.environ, OMP_NUM_THREADS="8"))
+            out = {"args": np.array([str(a) for a in args]), "n": np.array(n)}
+            nsite = n ** 3
+            samples = np.sort(rng.choice(nsite, size=min(NSAMPLE, nsite), replace=False))
+            out["sample_sites"] = samples
+            for key in full:
+                out[key] = np.fromfile(os.path.join(tmp, key + ".f64"))
+            for key in hashed:
+                a = np.fromfile(os.path.join(tmp, key + ".f64"))
+                ncomp = a.size // nsite
+                out[key + "_sha256"] = np.array(sha(a))
+                out[key + "_samples"] = a.reshape(ncomp, nsite)[:, samples]
+                out[key + "_sum"] = np.array(a.reshape(ncomp, nsite).sum(axis=1))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,125 @@
+"""The CPU oracle against golden vectors written by the reference's own build.
+
+The fixtures under tests/golden/ come from oracle/_ref/ref_harness, which
+drives the reference's lb_collide (collision.c:143), lb_propagation
+(propagation.c:43), field_grad_compute with grad_3d_7pt_fluid_d2
+(gradient_3d_7pt_fluid.c:73), phi_force_calculation (phi_force.c:74),
+phi_cahn_hilliard (phi_cahn_hilliard.c:213) and field_phi_init_spinodal
+(field_phi_init.c:484) on seeded inputs.  The bar is bit-exact equality.
+"""
+
+import hashlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def assert_exact(got, want, what):
+    got = np.asarray(got)
+    want = np.asarray(want)
+    assert got.shape == want.shape, what
+    bad = np.flatnonzero(got != want)
+    if bad.size:
+        i = bad[0]
+        rel = np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-300))
+        pytest.fail(f"{what}: {bad.size} differ, first at {i}: {got[i]!r} vs {want[i]!r}, max rel {rel:.3e}")
+
+
+@pytest.mark.parametrize("case,relax,tau", [
+    ("collide_m10", O.M10, 0.8), ("collide_bgk", O.BGK, 0.8),
+    ("collide_trt", O.TRT, 0.8), ("collide_m10_tau1", O.M10, 1.0)])
+def test_collide(golden, case, relax, tau):
+    g = golden(case)
+    n = int(g["n"])
+    p = O.lb_param(tau=tau, relax=relax)
+    f, rho, u = O.collide((n, n, n), p, g["f_in"], g["force"])
+    assert_exact(f, g["f_out"], "f")
+    assert_exact(rho, g["rho"], "rho")
+    assert_exact(u, g["u"], "u")
+
+
+def test_propagate(golden):
+    g = golden("prop")
+    n = int(g["n"])
+    assert_exact(O.propagate((n, n, n), g["f_in"]), g["f_out"], "f")
+
+
+@pytest.mark.parametrize("case,method", [("force_stress", 0), ("force_gradmu", 1)])
+def test_gradient_mu_force(golden, case, method):
+    g = golden(case)
+    n = int(g["n"])
+    s = (n, n, n)
+    fe = O.fe_param(force_method=method)
+    grad, delsq = O.grad7(s, g["phi"])
+    assert_exact(grad, g["grad"], "grad")
+    assert_exact(delsq, g["delsq"], "delsq")
+    assert_exact(O.mu(fe, g["phi"], delsq), g["mu"], "mu")
+    assert_exact(O.force(s, fe, g["phi"], grad, delsq), g["force"], "force")
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_cahn_hilliard(golden, order):
+    g = golden(f"ch_order{order}")
+    n = int(g["n"])
+    s = (n, n, n)
+    fe = O.fe_param(order=order)
+    _, delsq = O.grad7(s, g["phi_in"])
+    assert_exact(O.cahn_hilliard(s, fe, g["phi_in"], delsq, g["u"]), g["phi_out"], "phi")
+
+
+def _binary(g):
+    args = [str(a) for a in g["args"]]
+    n, nsteps, seed, order = int(args[1]), int(args[2]), int(args[3]), int(args[4])
+    a, b, k, m, tau, phi0, amp = map(float, args[5:12])
+    s = (n, n, n)
+    lb = O.lb_param(tau=tau)
+    fe = O.fe_param(a=a, b=b, kappa=k, mobility=m, order=order)
+    phi = O.spinodal(s, seed, phi0, amp)
+    f = O.equilibrium(s, rho_uniform=1.0)
+    return s, nsteps, lb, fe, phi, f
+
+
+@pytest.mark.parametrize("case", ["binary16_o1_s20", "binary16_o3_s10", "binary16_o2_s5"])
+def test_binary_steps_full_fields(golden, case):
+    g = golden(case)
+    s, nsteps, lb, fe, phi, f = _binary(g)
+    if "phi_init" in g:
+        assert_exact(phi, g["phi_init"], "phi_init")
+    if "phi_10" in g:
+        _, p10, _, _ = O.binary_steps(s, 10, lb, fe, f, phi)
+        assert_exact(p10, g["phi_10"], "phi at step 10")
+    f1, phi1, rho1, u1 = O.binary_steps(s, nsteps, lb, fe, f, phi)
+    assert_exact(phi1, g["phi"], "phi")
+    assert_exact(rho1, g["rho"], "rho")
+    assert_exact(u1, g["u"], "u")
+    assert sha(f1) == str(g["f_sha256"])
+
+
+def test_binary32_100_steps(golden):
+    g = golden("binary32_o1_s100")
+    s, nsteps, lb, fe, phi, f = _binary(g)
+    f1, phi1, rho1, u1 = O.binary_steps(s, nsteps, lb, fe, f, phi)
+    for key, a in (("phi", phi1), ("rho", rho1), ("u", u1), ("f", f1)):
+        assert sha(a) == str(g[key + "_sha256"]), key
+
+
+def test_single_fluid_c1(golden):
+    """C1: plain D3Q19 MRT 32^3, tau 0.8, 100 steps from a random-velocity
+    equilibrium; bit-exact f, rho, u after 100 steps."""
+    g = golden("single32_s100")
+    args = [str(a) for a in g["args"]]
+    n, nsteps, seed, tau = int(args[1]), int(args[2]), int(args[3]), float(args[4])
+    s = (n, n, n)
+    u0 = O.random_velocity(s, seed)
+    assert sha(u0) == str(g["u_init_sha256"])
+    f0 = O.equilibrium(s, u=u0, rho_uniform=1.0)
+    assert sha(f0) == str(g["f_init_sha256"])
+    f, rho, u = O.single_steps(s, nsteps, O.lb_param(tau=tau), f0)
+    for key, a in (("f", f), ("rho", rho), ("u", u)):
+        assert sha(a) == str(g[key + "_sha256"]), key
