@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the D3Q19 binary-fluid step on one MI355X.
+
+Workload (a step = one full time step of the path over the whole lattice):
+symmetric free energy + finite-difference Cahn-Hilliard + D3Q19 MRT, the
+plain binary-fluid configuration: 128^3 periodic, FP64, A = -0.0625,
+B = 0.0625, kappa = 0.04, M = 0.45, tau = 1, phi0 = 0 with +-0.05 spinodal
+noise, rho = 1, u = 0.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line:
+  value        MLUPS = sites * steps / wall time of the timed region
+  roofline     for the dominant kernel (collide, K1): algorithmic bytes per
+               launch (360 B/site in FP64) / average kernel duration from HIP
+               events on the library's stream; peak 8000 GB/s
+  cpu_baseline the reference's own serial/OpenMP build (oracle/_ref/Ludwig.exe,
+               compiled from the reference sources) on the host cores, or the
+               CPU oracle when that binary is absent
+  extra        256^3 FP64 and FP32-shifted throughput, per-kernel times
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
+"""
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_SITE = {64: {"collide": 360, "phi_force_mu": 40, "cahn_hilliard": 48, "step": 448},
+              32: {"collide": 180, "phi_force_mu": 20, "cahn_hilliard": 24, "step": 224}}
+C2 = dict(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.45, tau=1.0, order=1)
+SEED = 8361235
+
+
+def run_gpu(n, steps, warmup, precision, timing_steps):
+    import ludwig_amd as L
+    b = L.BinaryFluid((n, n, n), precision=precision, fatal=True, **C2)
+    b.init_spinodal(SEED, 0.0, 0.1)
+    lib = L.lib()
+    b.step(warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    # kernel durations: HIP events around each launch, on the library stream
+    lib.ldw_timer_reset()
+    lib.ldw_timer_enable(1)
+    b.step(timing_steps)
+    b.synchronize()
+    lib.ldw_timer_enable(0)
+    kt = {}
+    import ctypes
+    for name in ("collide", "phi_force_mu", "cahn_hilliard", "mu", "propagate"):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int()
+        lib.ldw_timer_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+        if cnt.value:
+            kt[name] = ms.value / cnt.value
+    r, p = b.conserved_sums()
+    ok = bool(abs(r - n ** 3) / n ** 3 < 1e-6)
+    b.close()
+    nsite = n ** 3
+    mlups = nsite * steps / wall / 1e6
+    return {"n": n, "precision": precision, "steps": steps, "wall_s": wall,
+            "ms_per_step": 1e3 * wall / steps, "mlups": mlups,
+            "hbm_frac_step": mlups * 1e6 * BYTES_SITE[precision]["step"] / (HBM_PEAK_GBS * 1e9),
+            "kernel_ms": kt, "mass_ok": ok}
+
+
+def cpu_baseline(n=128, steps=10):
+    """Time the reference's CPU build on this host, bounded to ~10-30 s."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    exe = os.path.join(HERE, "oracle", "_ref", "Ludwig.exe")
+    if os.path.exists(exe):
+        inp = f"""N_cycles {steps}
+size {n}_{n}_{n}
+grid 1_1_1
+viscosity 0.16666666666666667
+free_energy symmetric
+symmetric_a -0.0625
+symmetric_b 0.0625
+symmetric_kappa 0.04
+mobility 0.45
+phi0 0.0
+phi_initialisation spinodal
+noise 0.1
+random_seed {SEED}
+fd_gradient_calculation 3d_7pt_fluid
+fd_advection_scheme_order 1
+colloid_init no_colloids
+periodicity 1_1_1
+freq_statistics {steps}
+config_at_end no
+"""
+        with tempfile.TemporaryDirectory() as tmp:
+            with open(os.path.join(tmp, "input"), "w") as fp:
+                fp.write(inp)
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+                       OMP_PLACES="cores")
+            out = subprocess.run([exe], cwd=tmp, env=env, capture_output=True, text=True,
+                                 timeout=600).stdout
+        m = re.search(r"Time step loop:\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)\s+\((\d+) calls\)", out)
+        if m:
+            total, calls = float(m.group(3)), int(m.group(5))
+            return {"value": n ** 3 * calls / total / 1e6, "unit": "MLUPS", "cores": threads,
+                    "kind": "reference",
+                    "sample": f"reference Ludwig.exe (gcc -O2 -fopenmp -DNDEBUG), {n}^3 binary "
+                              f"fluid, {calls} steps, 'Time step loop' timer, {threads} OpenMP threads"}
+    # fall back to the C restatement
+    import numpy as np
+    import oracle as O
+    shape = (n, n, n)
+    f0 = O.equilibrium(shape, rho_uniform=1.0)
+    phi0 = O.spinodal(shape, SEED, 0.0, 0.1)
+    t0 = time.perf_counter()
+    O.binary_steps(shape, steps, O.lb_param(tau=1.0), O.fe_param(), f0, phi0)
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"C oracle (gcc -O2 -fopenmp), {n}^3 binary fluid, {steps} steps"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    main_run = run_gpu(args.size, args.steps, args.warmup, 64, min(args.steps, 50))
+    print(f"# main: {main_run}", file=sys.stderr, flush=True)
+    extra = {}
+    if not args.no_extra:
+        for n, prec in ((256, 64), (256, 32), (128, 32)):
+            r = run_gpu(n, max(20, args.steps // 4), 5, prec, 20)
+            extra[f"{n}^3_fp{prec}"] = {k: r[k] for k in ("mlups", "ms_per_step", "hbm_frac_step",
+                                                          "kernel_ms", "mass_ok")}
+            print(f"# extra {n} fp{prec}: {r}", file=sys.stderr, flush=True)
+
+    nsite = args.size ** 3
+    coll_ms = main_run["kernel_ms"].get("collide")
+    roof = None
+    if coll_ms:
+        achieved = BYTES_SITE[64]["collide"] * nsite / (coll_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "collide (K1, fused MRT collision + pull propagation)",
+                "algorithmic_bytes_per_site": BYTES_SITE[64]["collide"],
+                "avg_kernel_ms": coll_ms}
+    cpu = None if args.no_cpu else cpu_baseline()
+
+    line = {
+        "metric": "D3Q19 binary-fluid MLUPS & %HBM peak; surrogate speedup at <=2% L(t) error",
+        "value": main_run["mlups"],
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_run["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded spinodal initial condition)",
+        "config": {"workload": f"binary fluid D3Q19 MRT + symmetric FE + Cahn-Hilliard, "
+                               f"{args.size}^3 periodic, FP64, tau=1, A=-0.0625, B=0.0625, "
+                               f"kappa=0.04, M=0.45, advection order 1",
+                   "lattice": [args.size] * 3, "precision": "fp64"},
+        "hbm_frac_step": main_run["hbm_frac_step"],
+        "kernel_ms": main_run["kernel_ms"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,382 @@
+/*****************************************************************************
+ *
+ *  ludwig_hip.h -- C ABI of libludwig_hip.so
+ *
+ *  The D3Q19 binary-fluid time step of Ludwig (lattice Boltzmann MRT
+ *  collision + propagation, symmetric free energy mu and force, finite
+ *  difference Cahn-Hilliard update) on one MI355X GPU, behind the
+ *  reference's own C object API.  Each declaration cites the reference
+ *  interface it replaces (file:line in the reference's src/ tree).
+ *
+ *  Conventions kept from the reference:
+ *    - every function returns int, 0 on success;
+ *    - errors are fatal by default (message with file:line, then abort),
+ *      as pe_fatal()/tdpAssert() are (pe.c:240, target/target.h:136-138);
+ *      ldw_error_mode_set(0) makes them return -1 instead, with the message
+ *      in ldw_last_error();
+ *    - host arrays keep the reference's SoA layout with halos
+ *      (memory.h:184-230 "reverse" order, -DADDR_SOA), indexed by
+ *      cs_index() (coords.c:617-631);
+ *    - state lives on the device between *_memcpy() calls
+ *      (ludwig.c:501-509).
+ *
+ *  Device side (not visible through this header): a periodic lattice
+ *  without halos, SoA, site = (x*ny + y)*nz + z; *_halo() calls are
+ *  therefore no-ops and propagation is folded into the next collision.
+ *
+ *****************************************************************************/
+
+#ifndef LUDWIG_HIP_H
+#define LUDWIG_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Library control (no counterpart in the reference)                       */
+/* ------------------------------------------------------------------------ */
+
+/* 1: errors abort (default, as the reference); 0: functions return -1 */
+int ldw_error_mode_set(int fatal);
+const char * ldw_last_error(void);
+
+/* Storage and arithmetic precision of objects created afterwards:
+ * 64 = FP64 (default; bit-compatible with the reference),
+ * 32 = FP32 with distributions stored as f_i - w_i rho0.
+ * The environment variable LDW_PRECISION gives the initial value. */
+int ldw_precision_set(int bits);
+int ldw_precision(void);
+
+/* Wait for all device work of the library */
+int ldw_device_synchronize(void);
+
+/* ------------------------------------------------------------------------ */
+/* Parallel environment and coordinates                                     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct pe_s pe_t;
+typedef enum {PE_QUIET = 0, PE_VERBOSE, PE_OPTION_MAX} pe_enum_t;
+
+/* pe.h:26 pe_create(MPI_Comm, pe_enum_t, pe_t **); one rank, so the
+ * communicator argument is an integer placeholder and is ignored */
+int pe_create(int comm, pe_enum_t flag, pe_t ** ppe);
+int pe_free(pe_t * pe);                                       /* pe.h:27 */
+
+typedef struct cs_s cs_t;
+
+int cs_create(pe_t * pe, cs_t ** pcs);                        /* coords.h:73 */
+int cs_free(cs_t * cs);                                       /* coords.h:74 */
+int cs_init(cs_t * cs);                                       /* coords.h:76 */
+int cs_ntotal_set(cs_t * cs, const int ntotal[3]);            /* coords.h:82 */
+int cs_nhalo_set(cs_t * cs, int nhalo);                       /* coords.h:83 */
+int cs_nhalo(cs_t * cs, int * nhalo);                         /* coords.h:88 */
+int cs_ntotal(cs_t * cs, int ntotal[3]);                      /* coords.h:90 */
+int cs_nlocal(cs_t * cs, int n[3]);                           /* coords.h:95 */
+int cs_nsites(cs_t * cs, int * nsites);                       /* coords.h:104 */
+int cs_index(cs_t * cs, int ic, int jc, int kc);              /* coords.h:102 */
+
+/* ------------------------------------------------------------------------ */
+/* Physical constants (physics.h)                                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct physics_s physics_t;
+
+int physics_create(pe_t * pe, physics_t ** phys);             /* physics.h:20 */
+int physics_free(physics_t * phys);                           /* physics.h:21 */
+int physics_ref(physics_t ** ref);                            /* physics.h:39 */
+int physics_rho0_set(physics_t * phys, double rho0);          /* physics.h:23 */
+int physics_phi0_set(physics_t * phys, double phi0);          /* physics.h:24 */
+int physics_eta_shear_set(physics_t * phys, double eta);      /* physics.h:25 */
+int physics_eta_bulk_set(physics_t * phys, double zeta);      /* physics.h:26 */
+int physics_fbody_set(physics_t * phys, double f[3]);         /* physics.h:31 */
+int physics_mobility_set(physics_t * phys, double mobility);  /* physics.h:32 */
+int physics_rho0(physics_t * phys, double * rho0);
+int physics_phi0(physics_t * phys, double * phi0);
+int physics_eta_shear(physics_t * phys, double * eta);
+int physics_eta_bulk(physics_t * phys, double * eta);
+int physics_mobility(physics_t * phys, double * mobility);
+int physics_fbody(physics_t * phys, double f[3]);
+
+/* ------------------------------------------------------------------------ */
+/* Host/device copies (target/target.h tdpMemcpyKind)                       */
+/* ------------------------------------------------------------------------ */
+
+typedef enum {
+  tdpMemcpyHostToHost = 0,
+  tdpMemcpyHostToDevice = 1,
+  tdpMemcpyDeviceToHost = 2,
+  tdpMemcpyDeviceToDevice = 3
+} tdpMemcpyKind;
+
+/* ------------------------------------------------------------------------ */
+/* Lattice Boltzmann distributions (lb_data.h)                              */
+/* ------------------------------------------------------------------------ */
+
+typedef enum lb_relaxation_enum {LB_RELAXATION_M10,
+				 LB_RELAXATION_BGK,
+				 LB_RELAXATION_TRT} lb_relaxation_enum_t;
+typedef enum lb_halo_enum {LB_HALO_FULL = 1, LB_HALO_REDUCED = 2} lb_halo_enum_t;
+typedef enum lb_dist_enum_type {LB_RHO = 0, LB_PHI = 1} lb_dist_enum_t;
+
+/* lb_data_options.h:32-42 without the I/O member */
+typedef struct lb_data_options_s {
+  int ndim;
+  int nvel;
+  int ndist;
+  lb_relaxation_enum_t nrelax;
+  lb_halo_enum_t halo;
+  int reportimbalance;
+  int usefirsttouch;
+} lb_data_options_t;
+
+typedef struct lb_data_s lb_t;
+typedef struct hydro_s hydro_t;
+typedef struct map_s map_t;
+typedef struct noise_s noise_t;
+typedef struct fe_s fe_t;
+typedef struct visc_s visc_t;
+
+lb_data_options_t lb_data_options_default(void);              /* lb_data_options.h:44 */
+int lb_data_create(pe_t * pe, cs_t * cs, const lb_data_options_t * opts,
+		   lb_t ** lb);                               /* lb_data.h:162 */
+int lb_free(lb_t * lb);                                       /* lb_data.h:164 */
+int lb_memcpy(lb_t * lb, tdpMemcpyKind flag);                 /* lb_data.c:478 */
+int lb_halo(lb_t * lb);                                       /* lb_data.c:754 */
+int lb_init_rest_f(lb_t * lb, double rho0);                   /* lb_data.c:659 */
+int lb_1st_moment_equilib_set(lb_t * lb, int index, double rho,
+			      double u[3]);                   /* lb_data.c:809 */
+int lb_ndist(lb_t * lb, int * ndist);
+int lb_nvel(lb_t * lb, int * nvel);
+
+/* Host accessors, static inline in the reference (lb_data.h:198-290) */
+int lb_f(lb_t * lb, int index, int p, int n, double * f);
+int lb_f_set(lb_t * lb, int index, int p, int n, double f);
+int lb_0th_moment(lb_t * lb, int index, lb_dist_enum_t nd, double * rho);
+int lb_1st_moment(lb_t * lb, int index, lb_dist_enum_t nd, double g[3]);
+
+/* collision.h:27-35 */
+int lb_collision_relaxation_set(lb_t * lb, lb_relaxation_enum_t nrelax);
+int lb_collision_ghost_modes_on(lb_t * lb);
+int lb_collision_ghost_modes_off(lb_t * lb);
+int lb_collide(lb_t * lb, hydro_t * hydro, map_t * map, noise_t * noise,
+	       fe_t * fe, visc_t * visc);                     /* collision.c:143 */
+
+int lb_propagation(lb_t * lb);                                /* propagation.c:43 */
+
+/* ------------------------------------------------------------------------ */
+/* Lees-Edwards, map, wall: the periodic, all-fluid, wall-free case only    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct lees_edw_s lees_edw_t;
+typedef struct lees_edw_options_s {                           /* lees_edwards_options.h:32 */
+  int nplanes;
+  int type;
+  int period;
+  int nt0;
+  double uy;
+} lees_edw_options_t;
+
+int lees_edw_create(pe_t * pe, cs_t * cs, const lees_edw_options_t * opts,
+		    lees_edw_t ** le);                        /* leesedwards.h:26 */
+int lees_edw_free(lees_edw_t * le);                           /* leesedwards.h:29 */
+
+typedef struct map_options_s {int ndata; int is_porous_media;} map_options_t;
+map_options_t map_options_default(void);                      /* map_options.h:29 */
+int map_create(pe_t * pe, cs_t * cs, const map_options_t * options,
+	       map_t ** map);                                 /* map.h:51 */
+int map_free(map_t ** map);                                   /* map.h:53 */
+
+typedef struct wall_s wall_t;
+int wall_create(pe_t * pe, cs_t * cs, map_t * map, lb_t * lb,
+		wall_t ** p);                                 /* wall.h:88 */
+int wall_free(wall_t * wall);
+
+/* ------------------------------------------------------------------------ */
+/* Fields (field.h, field_grad.h)                                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct field_options_s {                              /* field_options.h:28 */
+  int ndata;
+  int nhcomm;
+  int haloscheme;
+  int haloverbose;
+  int usefirsttouch;
+} field_options_t;
+
+typedef struct field_s field_t;
+typedef struct field_grad_s field_grad_t;
+typedef int (* grad_ft)(field_grad_t * fgrad);
+
+field_options_t field_options_ndata_nhalo(int ndata, int nhalo); /* field_options.h:40 */
+int field_create(pe_t * pe, cs_t * cs, lees_edw_t * le, const char * name,
+		 const field_options_t * opts, field_t ** pobj); /* field.h:97 */
+int field_free(field_t * obj);                                /* field.h:101 */
+int field_memcpy(field_t * obj, tdpMemcpyKind flag);          /* field.h:102 */
+int field_halo(field_t * obj);                                /* field.h:104 */
+int field_nf(field_t * obj, int * nf);
+int field_scalar(field_t * obj, int index, double * phi);     /* field.h:109 */
+int field_scalar_set(field_t * obj, int index, double phi);   /* field.h:110 */
+
+int field_grad_create(pe_t * pe, field_t * f, int level,
+		      field_grad_t ** pobj);                  /* field_grad.h:44 */
+void field_grad_free(field_grad_t * obj);                     /* field_grad.h:46 */
+int field_grad_set(field_grad_t * obj, grad_ft d2, grad_ft d4); /* field_grad.h:47 */
+int field_grad_compute(field_grad_t * obj);                   /* field_grad.c:319 */
+int field_grad_memcpy(field_grad_t * obj, tdpMemcpyKind flag);
+int field_grad_scalar_grad(field_grad_t * obj, int index, double grad[3]);
+int field_grad_scalar_delsq(field_grad_t * obj, int index, double * delsq);
+
+/* The 7-point fluid stencil (gradient_3d_7pt_fluid.c:73) */
+int grad_3d_7pt_fluid_d2(field_grad_t * fgrad);
+
+/* ------------------------------------------------------------------------ */
+/* Hydrodynamic fields (hydro.h)                                            */
+/* ------------------------------------------------------------------------ */
+
+typedef struct hydro_options_s {int nhcomm;} hydro_options_t;
+
+hydro_options_t hydro_options_default(void);                  /* hydro_options.h:32 */
+int hydro_create(pe_t * pe, cs_t * cs, lees_edw_t * le,
+		 const hydro_options_t * opts, hydro_t ** pobj); /* hydro.h:49 */
+int hydro_free(hydro_t * obj);                                /* hydro.h:51 */
+int hydro_memcpy(hydro_t * obj, tdpMemcpyKind flag);          /* hydro.h:52 */
+int hydro_u_halo(hydro_t * obj);                              /* hydro.h:53 */
+int hydro_f_zero(hydro_t * obj, const double fzero[3]);       /* hydro.h:59 */
+int hydro_u_zero(hydro_t * obj, const double uzero[3]);       /* hydro.h:60 */
+/* accessors, static inline in the reference (hydro_impl.h:46-140) */
+int hydro_u(hydro_t * obj, int index, double u[3]);
+int hydro_u_set(hydro_t * obj, int index, const double u[3]);
+int hydro_rho(hydro_t * obj, int index, double * rho);
+int hydro_f_local(hydro_t * obj, int index, double force[3]);
+int hydro_f_local_set(hydro_t * obj, int index, const double force[3]);
+
+/* ------------------------------------------------------------------------ */
+/* Symmetric free energy (symmetric.h)                                      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct fe_symm_s fe_symm_t;
+typedef struct fe_symm_param_s {                              /* symmetric.h:40 */
+  double a;
+  double b;
+  double kappa;
+  double c;
+  double h;
+} fe_symm_param_t;
+
+int fe_symm_create(pe_t * pe, cs_t * cs, field_t * f, field_grad_t * grd,
+		   fe_symm_t ** p);                           /* symmetric.h:48 */
+int fe_symm_free(fe_symm_t * fe);                             /* symmetric.h:50 */
+int fe_symm_param_set(fe_symm_t * fe, fe_symm_param_t values); /* symmetric.h:51 */
+int fe_symm_param(fe_symm_t * fe, fe_symm_param_t * values);
+/* host evaluation on the host copies of phi and its gradients */
+int fe_symm_mu(fe_symm_t * fe, int index, double * mu);       /* symmetric.c:308 */
+int fe_symm_fed(fe_symm_t * fe, int index, double * fed);     /* symmetric.c:285 */
+int fe_symm_str(fe_symm_t * fe, int index, double s[3][3]);   /* symmetric.c:334 */
+
+/* ------------------------------------------------------------------------ */
+/* Thermodynamic force (fe_force_method.h, phi_force_stress.h, phi_force.h) */
+/* ------------------------------------------------------------------------ */
+
+typedef enum {
+  FE_FORCE_METHOD_INVALID,
+  FE_FORCE_METHOD_NO_FORCE,
+  FE_FORCE_METHOD_STRESS_DIVERGENCE,
+  FE_FORCE_METHOD_PHI_GRADMU,
+  FE_FORCE_METHOD_PHI_GRADMU_CORRECTION,
+  FE_FORCE_METHOD_RELAXATION_SYMM,
+  FE_FORCE_METHOD_RELAXATION_ANTI,
+  FE_FORCE_METHOD_MAX
+} fe_force_method_enum_t;                                     /* fe_force_method.h:20 */
+
+typedef struct pth_s pth_t;
+int pth_create(pe_t * pe, cs_t * cs, int method, pth_t ** pth); /* phi_force_stress.h:36 */
+int pth_free(pth_t * pth);                                    /* phi_force_stress.h:37 */
+int phi_force_calculation(pe_t * pe, cs_t * cs, lees_edw_t * le,
+			  wall_t * wall, pth_t * pth, fe_t * fe,
+			  map_t * map, field_t * phi,
+			  hydro_t * hydro);                   /* phi_force.c:74 */
+
+/* ------------------------------------------------------------------------ */
+/* Cahn-Hilliard (phi_cahn_hilliard.h, advection.h)                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct phi_ch_s phi_ch_t;
+typedef struct phi_ch_info_s {                                /* phi_cahn_hilliard.h:36 */
+  int conserve;
+  int noise;
+} phi_ch_info_t;
+
+int phi_ch_create(pe_t * pe, cs_t * cs, lees_edw_t * le,
+		  phi_ch_info_t * info, phi_ch_t ** pch);     /* phi_cahn_hilliard.c:131 */
+int phi_ch_free(phi_ch_t * pch);                              /* phi_cahn_hilliard.c:171 */
+int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi,
+		      hydro_t * hydro, map_t * map,
+		      noise_t * noise);                       /* phi_cahn_hilliard.c:213 */
+int advection_order_set(const int order);                     /* advection.h:40 */
+int advection_order(int * order);                             /* advection.h:41 */
+
+/* ------------------------------------------------------------------------ */
+/* Initial conditions                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* phi = phi0 + amp (r - 1/2), r from the per-site lattice generator of the
+ * reference (field_phi_init.c:484-518, noise.c:278-341, noise.h:96-109);
+ * writes the host copy and the device copy */
+int field_phi_init_spinodal(field_t * phi, int seed, double phi0, double amp);
+
+/* ------------------------------------------------------------------------ */
+/* Additions (no counterpart in the reference)                              */
+/* ------------------------------------------------------------------------ */
+
+/* Plain-fluid start: u_a = amp (2 r - 1) from three draws of the same
+ * per-site generator, per-component mean removed, f = equilibrium(1, u) */
+int ldw_lb_init_random_velocity(lb_t * lb, int seed, double amp);
+
+/* Bulk copies of interior values in the device layout
+ * (site = (x*ny + y)*nz + z, component major), as FP64 on the host side.
+ * ldw_lb_f_read returns the distributions the reference's lb->f would
+ * hold at this point of the step. */
+int ldw_lb_f_read(lb_t * lb, double * f);
+int ldw_lb_f_write(lb_t * lb, const double * f);
+int ldw_field_read(field_t * obj, double * data);
+int ldw_field_write(field_t * obj, const double * data);
+int ldw_hydro_read(hydro_t * obj, double * rho, double * u, double * force);
+int ldw_hydro_write(hydro_t * obj, const double * rho, const double * u,
+		    const double * force);
+
+/* nsteps of the binary-fluid time step in the reference's loop order
+ * (ludwig.c:528-860): hydro_f_zero, field_halo, field_grad_compute,
+ * phi_force_calculation, phi_cahn_hilliard, hydro_u_zero, lb_collide,
+ * lb_halo, lb_propagation; the same result as calling those in turn. */
+int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi,
+		    field_grad_t * dphi, fe_t * fe, pth_t * pth,
+		    phi_ch_t * pch, int nsteps);
+
+/* nsteps of the single-fluid step (free_energy none) */
+int ldw_step_single(lb_t * lb, hydro_t * hydro, int nsteps);
+
+/* Compensated FP64 sums on the device: out[0] = sum_x sum_p f (total
+ * density), out[1] = sum_x phi (phi may be NULL -> 0) */
+int ldw_conserved_sums(lb_t * lb, field_t * phi, double out[2]);
+
+/* Per-kernel device timing with HIP events on the library's stream.
+ * ldw_timer_enable(1) starts recording; ldw_timer_read(name, &ms, &n)
+ * returns the accumulated time and launch count of a kernel family:
+ * "collide", "phi_force_mu", "mu", "cahn_hilliard", "propagate". */
+int ldw_timer_enable(int on);
+int ldw_timer_reset(void);
+int ldw_timer_read(const char * name, double * ms, int * count);
+
+/* Raw device arrays (layout as described at the top of this file) for
+ * callers that keep data on the GPU; bytes per element in *elsize. */
+int ldw_lb_device_f(lb_t * lb, void ** ptr, int * elsize, int * shifted);
+int ldw_field_device(field_t * obj, void ** ptr, int * elsize);
+int ldw_hydro_device(hydro_t * obj, void ** rho, void ** u, void ** force,
+		     int * elsize);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

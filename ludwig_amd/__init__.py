@@ -1,0 +1,361 @@
+"""ludwig_amd -- Python host side of libludwig_hip.so (the MI355X D3Q19
+binary-fluid step behind Ludwig's C object API).
+
+The compute lives in ``libludwig_hip.so`` (HIP kernels for gfx950 plus the
+C ABI declared in ``include/ludwig_hip.h``).  This module binds that ABI with
+ctypes and offers two conveniences that build the reference's object graph
+the way ``ludwig_rt`` does (ludwig.c:202-429):
+
+* :class:`BinaryFluid` -- symmetric free energy + Cahn-Hilliard + D3Q19 MRT
+  (the ``free_energy symmetric`` branch, ludwig.c:1190-1262);
+* :class:`SingleFluid` -- ``free_energy none`` (ludwig.c:1175-1189).
+
+There is no CPU fallback: loading fails loudly when the library is not
+built, and object creation fails when no GPU is present.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libludwig_hip.so")
+HEADER = os.path.join(REPO, "include", "ludwig_hip.h")
+
+# enums of the reference (lb_data_options.h:20, fe_force_method.h:20)
+M10, BGK, TRT = 0, 1, 2
+H2D, D2H = 1, 2
+FORCE_NO_FORCE, FORCE_STRESS_DIVERGENCE, FORCE_PHI_GRADMU = 1, 2, 3
+
+_lib = None
+
+
+class LbDataOptions(ctypes.Structure):
+    _fields_ = [("ndim", ctypes.c_int), ("nvel", ctypes.c_int), ("ndist", ctypes.c_int),
+                ("nrelax", ctypes.c_int), ("halo", ctypes.c_int),
+                ("reportimbalance", ctypes.c_int), ("usefirsttouch", ctypes.c_int)]
+
+
+class FieldOptions(ctypes.Structure):
+    _fields_ = [("ndata", ctypes.c_int), ("nhcomm", ctypes.c_int), ("haloscheme", ctypes.c_int),
+                ("haloverbose", ctypes.c_int), ("usefirsttouch", ctypes.c_int)]
+
+
+class HydroOptions(ctypes.Structure):
+    _fields_ = [("nhcomm", ctypes.c_int)]
+
+
+class MapOptions(ctypes.Structure):
+    _fields_ = [("ndata", ctypes.c_int), ("is_porous_media", ctypes.c_int)]
+
+
+class LeOptions(ctypes.Structure):
+    _fields_ = [("nplanes", ctypes.c_int), ("type", ctypes.c_int), ("period", ctypes.c_int),
+                ("nt0", ctypes.c_int), ("uy", ctypes.c_double)]
+
+
+class FeSymmParam(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_double), ("b", ctypes.c_double), ("kappa", ctypes.c_double),
+                ("c", ctypes.c_double), ("h", ctypes.c_double)]
+
+
+class PhiChInfo(ctypes.Structure):
+    _fields_ = [("conserve", ctypes.c_int), ("noise", ctypes.c_int)]
+
+
+def header_functions():
+    """Names of the functions declared in include/ludwig_hip.h."""
+    import re
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = "\n".join(l for l in text.splitlines() if "typedef" not in l)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(",
+                       text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("typedef",)))
+
+
+def lib():
+    """Load libludwig_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `make -C ludwig_amd` "
+                           "(or __graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    pvp = ctypes.POINTER(ctypes.c_void_p)
+    dp = ctypes.c_void_p
+    sig = {
+        "ldw_error_mode_set": [ctypes.c_int], "ldw_precision_set": [ctypes.c_int],
+        "ldw_device_synchronize": [],
+        "pe_create": [ctypes.c_int, ctypes.c_int, pvp], "pe_free": [vp],
+        "cs_create": [vp, pvp], "cs_free": [vp], "cs_init": [vp],
+        "cs_ntotal_set": [ctypes.POINTER(ctypes.c_int)], "cs_nhalo_set": [vp, ctypes.c_int],
+        "cs_index": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "physics_create": [vp, pvp], "physics_free": [vp],
+        "physics_rho0_set": [vp, ctypes.c_double], "physics_eta_shear_set": [vp, ctypes.c_double],
+        "physics_eta_bulk_set": [vp, ctypes.c_double], "physics_mobility_set": [vp, ctypes.c_double],
+        "physics_fbody_set": [vp, ctypes.POINTER(ctypes.c_double)],
+        "lb_data_create": [vp, vp, ctypes.POINTER(LbDataOptions), pvp], "lb_free": [vp],
+        "lb_memcpy": [vp, ctypes.c_int], "lb_halo": [vp], "lb_init_rest_f": [vp, ctypes.c_double],
+        "lb_collision_relaxation_set": [vp, ctypes.c_int],
+        "lb_collide": [vp, vp, vp, vp, vp, vp], "lb_propagation": [vp],
+        "lees_edw_create": [vp, vp, ctypes.POINTER(LeOptions), pvp], "lees_edw_free": [vp],
+        "map_create": [vp, vp, ctypes.POINTER(MapOptions), pvp], "map_free": [pvp],
+        "wall_create": [vp, vp, vp, vp, pvp], "wall_free": [vp],
+        "field_create": [vp, vp, vp, ctypes.c_char_p, ctypes.POINTER(FieldOptions), pvp],
+        "field_free": [vp], "field_memcpy": [vp, ctypes.c_int], "field_halo": [vp],
+        "field_grad_create": [vp, vp, ctypes.c_int, pvp], "field_grad_free": [vp],
+        "field_grad_compute": [vp], "field_grad_memcpy": [vp, ctypes.c_int],
+        "hydro_create": [vp, vp, vp, ctypes.POINTER(HydroOptions), pvp], "hydro_free": [vp],
+        "hydro_f_zero": [vp, ctypes.POINTER(ctypes.c_double)],
+        "hydro_u_zero": [vp, ctypes.POINTER(ctypes.c_double)],
+        "hydro_memcpy": [vp, ctypes.c_int],
+        "fe_symm_create": [vp, vp, vp, vp, pvp], "fe_symm_free": [vp],
+        "fe_symm_param_set": [vp, FeSymmParam],
+        "pth_create": [vp, vp, ctypes.c_int, pvp], "pth_free": [vp],
+        "phi_force_calculation": [vp, vp, vp, vp, vp, vp, vp, vp, vp],
+        "phi_ch_create": [vp, vp, vp, ctypes.POINTER(PhiChInfo), pvp], "phi_ch_free": [vp],
+        "phi_cahn_hilliard": [vp, vp, vp, vp, vp, vp],
+        "advection_order_set": [ctypes.c_int],
+        "field_phi_init_spinodal": [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double],
+        "ldw_lb_init_random_velocity": [vp, ctypes.c_int, ctypes.c_double],
+        "ldw_step_binary": [vp, vp, vp, vp, vp, vp, vp, ctypes.c_int],
+        "ldw_step_single": [vp, vp, ctypes.c_int],
+        "ldw_conserved_sums": [vp, vp, ctypes.POINTER(ctypes.c_double)],
+        "ldw_timer_enable": [ctypes.c_int], "ldw_timer_reset": [],
+        "ldw_timer_read": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)],
+        "ldw_lb_f_read": [vp, dp], "ldw_lb_f_write": [vp, dp],
+        "ldw_field_read": [vp, dp], "ldw_field_write": [vp, dp],
+        "ldw_hydro_read": [vp, dp, dp, dp], "ldw_hydro_write": [vp, dp, dp, dp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.cs_ntotal_set.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.ldw_last_error.restype = ctypes.c_char_p
+    L.ldw_last_error.argtypes = []
+    L.ldw_precision.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _ck(rc, what):
+    if rc != 0:
+        msg = lib().ldw_last_error().decode()
+        raise RuntimeError(f"{what} failed: {msg}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def set_precision(bits):
+    _ck(lib().ldw_precision_set(bits), "ldw_precision_set")
+
+
+class _Base:
+    """The reference's pe/cs/physics/lb/hydro/map objects for an
+    nx*ny*nz periodic lattice."""
+
+    def __init__(self, shape, nhalo, tau=None, eta=None, eta_bulk=None, rho0=1.0,
+                 relax=M10, fbody=(0.0, 0.0, 0.0), precision=64, fatal=False):
+        L = lib()
+        L.ldw_error_mode_set(1 if fatal else 0)
+        set_precision(precision)
+        self.precision = precision
+        self.shape = tuple(int(s) for s in shape)
+        self.nsite = int(np.prod(self.shape))
+        if eta is None:
+            eta = (tau - 0.5) / 3.0
+        self.eta = eta
+        self.eta_bulk = eta if eta_bulk is None else eta_bulk
+        vp = ctypes.c_void_p
+        self.pe, self.cs, self.phys = vp(), vp(), vp()
+        _ck(L.pe_create(0, 0, ctypes.byref(self.pe)), "pe_create")
+        _ck(L.physics_create(self.pe, ctypes.byref(self.phys)), "physics_create")
+        _ck(L.physics_rho0_set(self.phys, rho0), "physics_rho0_set")
+        _ck(L.physics_eta_shear_set(self.phys, eta), "physics_eta_shear_set")
+        _ck(L.physics_eta_bulk_set(self.phys, self.eta_bulk), "physics_eta_bulk_set")
+        fb = (ctypes.c_double * 3)(*fbody)
+        _ck(L.physics_fbody_set(self.phys, fb), "physics_fbody_set")
+        _ck(L.cs_create(self.pe, ctypes.byref(self.cs)), "cs_create")
+        _ck(L.cs_nhalo_set(self.cs, nhalo), "cs_nhalo_set")
+        nt = (ctypes.c_int * 3)(*self.shape)
+        _ck(L.cs_ntotal_set(self.cs, nt), "cs_ntotal_set")
+        _ck(L.cs_init(self.cs), "cs_init")
+        self.le = vp()
+        _ck(L.lees_edw_create(self.pe, self.cs, ctypes.byref(LeOptions()), ctypes.byref(self.le)),
+            "lees_edw_create")
+        opts = LbDataOptions(3, 19, 1, relax, 1, 0, 0)
+        self.lb = vp()
+        _ck(L.lb_data_create(self.pe, self.cs, ctypes.byref(opts), ctypes.byref(self.lb)),
+            "lb_data_create")
+        _ck(L.lb_collision_relaxation_set(self.lb, relax), "lb_collision_relaxation_set")
+        self.hydro = vp()
+        _ck(L.hydro_create(self.pe, self.cs, self.le, ctypes.byref(HydroOptions(1)),
+                           ctypes.byref(self.hydro)), "hydro_create")
+        self.map = vp()
+        _ck(L.map_create(self.pe, self.cs, ctypes.byref(MapOptions(0, 0)), ctypes.byref(self.map)),
+            "map_create")
+
+    # state access (interior, device layout, float64)
+    def f(self):
+        out = np.empty(19 * self.nsite)
+        _ck(lib().ldw_lb_f_read(self.lb, _ptr(out)), "ldw_lb_f_read")
+        return out
+
+    def set_f(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.size == 19 * self.nsite
+        _ck(lib().ldw_lb_f_write(self.lb, _ptr(f)), "ldw_lb_f_write")
+
+    def hydro_fields(self):
+        rho = np.empty(self.nsite)
+        u = np.empty(3 * self.nsite)
+        force = np.empty(3 * self.nsite)
+        _ck(lib().ldw_hydro_read(self.hydro, _ptr(rho), _ptr(u), _ptr(force)), "ldw_hydro_read")
+        return rho, u, force
+
+    def set_hydro(self, rho=None, u=None, force=None):
+        c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        rho, u, force = c(rho), c(u), c(force)
+        _ck(lib().ldw_hydro_write(self.hydro, _ptr(rho), _ptr(u), _ptr(force)), "ldw_hydro_write")
+
+    def conserved_sums(self, phi=None):
+        out = (ctypes.c_double * 2)()
+        _ck(lib().ldw_conserved_sums(self.lb, phi, out), "ldw_conserved_sums")
+        return out[0], out[1]
+
+    def synchronize(self):
+        _ck(lib().ldw_device_synchronize(), "ldw_device_synchronize")
+
+    # the reference's per-call API
+    def collide(self, fe=None):
+        _ck(lib().lb_collide(self.lb, self.hydro, self.map, None, fe, None), "lb_collide")
+
+    def propagate(self):
+        _ck(lib().lb_halo(self.lb), "lb_halo")
+        _ck(lib().lb_propagation(self.lb), "lb_propagation")
+
+    def f_zero(self, fzero=(0.0, 0.0, 0.0)):
+        _ck(lib().hydro_f_zero(self.hydro, (ctypes.c_double * 3)(*fzero)), "hydro_f_zero")
+
+    def u_zero(self, uzero=(0.0, 0.0, 0.0)):
+        _ck(lib().hydro_u_zero(self.hydro, (ctypes.c_double * 3)(*uzero)), "hydro_u_zero")
+
+    def close(self):
+        L = _lib
+        if L is None or self.lb is None:
+            return
+        L.ldw_device_synchronize()
+        L.map_free(ctypes.byref(self.map))
+        L.hydro_free(self.hydro)
+        L.lb_free(self.lb)
+        L.lees_edw_free(self.le)
+        L.cs_free(self.cs)
+        L.physics_free(self.phys)
+        L.pe_free(self.pe)
+        self.lb = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SingleFluid(_Base):
+    """``free_energy none``: plain D3Q19 MRT fluid (nhalo 1)."""
+
+    def __init__(self, shape, **kw):
+        super().__init__(shape, 1, **kw)
+
+    def init_rest(self, rho0=1.0):
+        _ck(lib().lb_init_rest_f(self.lb, rho0), "lb_init_rest_f")
+
+    def init_random_velocity(self, seed, amp=1.0e-3):
+        _ck(lib().ldw_lb_init_random_velocity(self.lb, seed, amp), "ldw_lb_init_random_velocity")
+
+    def step(self, nsteps=1):
+        _ck(lib().ldw_step_single(self.lb, self.hydro, nsteps), "ldw_step_single")
+
+
+class BinaryFluid(_Base):
+    """``free_energy symmetric`` with the finite-difference Cahn-Hilliard
+    solver and the 7-point gradient (nhalo 2)."""
+
+    def __init__(self, shape, a=-0.0625, b=0.0625, kappa=0.04, mobility=0.45,
+                 order=1, force_method=FORCE_STRESS_DIVERGENCE, **kw):
+        super().__init__(shape, 2, **kw)
+        L = lib()
+        vp = ctypes.c_void_p
+        _ck(L.physics_mobility_set(self.phys, mobility), "physics_mobility_set")
+        _ck(L.advection_order_set(order), "advection_order_set")
+        self.order = order
+        self.phi = vp()
+        fo = FieldOptions(1, 2, 0, 0, 0)
+        _ck(L.field_create(self.pe, self.cs, self.le, b"phi", ctypes.byref(fo), ctypes.byref(self.phi)),
+            "field_create")
+        self.dphi = vp()
+        _ck(L.field_grad_create(self.pe, self.phi, 2, ctypes.byref(self.dphi)), "field_grad_create")
+        self.fe = vp()
+        _ck(L.fe_symm_create(self.pe, self.cs, self.phi, self.dphi, ctypes.byref(self.fe)),
+            "fe_symm_create")
+        _ck(L.fe_symm_param_set(self.fe, FeSymmParam(a, b, kappa, 0.0, 0.0)), "fe_symm_param_set")
+        self.pth = vp()
+        _ck(L.pth_create(self.pe, self.cs, force_method, ctypes.byref(self.pth)), "pth_create")
+        self.pch = vp()
+        _ck(L.phi_ch_create(self.pe, self.cs, self.le, ctypes.byref(PhiChInfo(0, 0)),
+                            ctypes.byref(self.pch)), "phi_ch_create")
+        self.wall = vp()
+        _ck(L.wall_create(self.pe, self.cs, self.map, self.lb, ctypes.byref(self.wall)), "wall_create")
+
+    def init_spinodal(self, seed, phi0=0.0, amp=0.1, rho0=1.0):
+        _ck(lib().field_phi_init_spinodal(self.phi, seed, phi0, amp), "field_phi_init_spinodal")
+        _ck(lib().lb_init_rest_f(self.lb, rho0), "lb_init_rest_f")
+
+    def step(self, nsteps=1):
+        _ck(lib().advection_order_set(self.order), "advection_order_set")
+        _ck(lib().ldw_step_binary(self.lb, self.hydro, self.phi, self.dphi, self.fe, self.pth,
+                                  self.pch, nsteps), "ldw_step_binary")
+
+    # individual stages, as the reference's time loop calls them
+    def force_stage(self):
+        _ck(lib().field_halo(self.phi), "field_halo")
+        _ck(lib().field_grad_compute(self.dphi), "field_grad_compute")
+        _ck(lib().phi_force_calculation(self.pe, self.cs, self.le, self.wall, self.pth, self.fe,
+                                        self.map, self.phi, self.hydro), "phi_force_calculation")
+
+    def cahn_hilliard_stage(self, with_hydro=True):
+        _ck(lib().phi_cahn_hilliard(self.pch, self.fe, self.phi,
+                                    self.hydro if with_hydro else None, self.map, None),
+            "phi_cahn_hilliard")
+
+    def phi_field(self):
+        out = np.empty(self.nsite)
+        _ck(lib().ldw_field_read(self.phi, _ptr(out)), "ldw_field_read")
+        return out
+
+    def set_phi(self, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        _ck(lib().ldw_field_write(self.phi, _ptr(phi)), "ldw_field_write")
+
+    def conserved_sums(self):
+        return super().conserved_sums(self.phi)
+
+    def close(self):
+        L = _lib
+        if L is not None and getattr(self, "lb", None) is not None:
+            L.ldw_device_synchronize()
+            L.wall_free(self.wall)
+            L.phi_ch_free(self.pch)
+            L.pth_free(self.pth)
+            L.fe_symm_free(self.fe)
+            L.field_grad_free(self.dphi)
+            L.field_free(self.phi)
+        super().close()

@@ -1,0 +1,107 @@
+// kernels.h -- launch interface of the HIP kernels (internal to
+// libludwig_hip.so; the public C ABI is include/ludwig_hip.h).
+
+#ifndef LDW_KERNELS_H
+#define LDW_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ldw {
+
+// Lattice geometry: a periodic nx*ny*nz box, site = (x*ny + y)*nz + z.
+struct Geom {
+  int nx, ny, nz;
+  int64_t nsite;            // nx*ny*nz
+};
+
+// Collision parameters, precomputed on the host exactly as the reference
+// does (collision.c:1163-1246).
+struct LbParams {
+  double rtau_shear;
+  double rtau_bulk;
+  double rtau_ghost[19];    // entries 10..18 used
+  double fglobal[3];        // uniform body force
+  double rho0;              // shift for the FP32 storage
+  int relax;                // 0 M10, 1 BGK, 2 TRT
+};
+
+// Symmetric free energy and Cahn-Hilliard parameters
+struct FeParams {
+  double a, b, kappa;
+  double mobility;
+  int order;                // advection order 1, 2, 3
+  int force_method;         // 0 stress divergence, 1 phi grad mu
+};
+
+// K1: fused D3Q19 MRT collision + pull propagation.
+//   g_in/g_out: 19 planes of nsite (post-collision distributions; when
+//   `pull` is 1 the pre-collision value at x is g_in(x - c_p), else g_in(x)).
+//   force: 3 planes or nullptr (then only fglobal acts).
+//   rho, u: outputs (u 3 planes); may be nullptr.
+void launch_collide_f64(const Geom & g, const LbParams & p, const double * g_in,
+			double * g_out, const double * force, double * rho,
+			double * u, int pull, hipStream_t s);
+
+// Same with FP32 storage of f - w rho0 and FP32 arithmetic.
+void launch_collide_f32(const Geom & g, const LbParams & p, const float * g_in,
+			float * g_out, const float * force, float * rho,
+			float * u, int pull, hipStream_t s);
+
+// Plain pull propagation (materialises one streaming step).
+void launch_propagate_f64(const Geom & g, const double * in, double * out,
+			  hipStream_t s);
+void launch_propagate_f32(const Geom & g, const float * in, float * out,
+			  hipStream_t s);
+
+// K2: 7-point gradient + mu + thermodynamic force in one pass.
+//   force_mode: 0 -> F = fzero + force (F not read);
+//               1 -> F = F + force (read-modify-write).
+void launch_phi_force_mu_f64(const Geom & g, const FeParams & fe,
+			     const double * phi, double * force, double * mu,
+			     int force_mode, const double fzero[3],
+			     hipStream_t s);
+void launch_phi_force_mu_f32(const Geom & g, const FeParams & fe,
+			     const float * phi, float * force, float * mu,
+			     int force_mode, const double fzero[3],
+			     hipStream_t s);
+
+// mu only (when no force is computed before the Cahn-Hilliard update)
+void launch_mu_f64(const Geom & g, const FeParams & fe, const double * phi,
+		   double * mu, hipStream_t s);
+void launch_mu_f32(const Geom & g, const FeParams & fe, const float * phi,
+		   float * mu, hipStream_t s);
+
+// Gradient and Laplacian materialised (for field_grad reads)
+void launch_grad7_f64(const Geom & g, const double * phi, double * grad,
+		      double * delsq, hipStream_t s);
+
+// K3: Cahn-Hilliard advective + diffusive fluxes and the phi update fused.
+//   u may be nullptr (no hydrodynamics: zero advective flux).
+void launch_cahn_hilliard_f64(const Geom & g, const FeParams & fe,
+			      const double * phi, const double * mu,
+			      const double * u, double * phi_out,
+			      hipStream_t s);
+void launch_cahn_hilliard_f32(const Geom & g, const FeParams & fe,
+			      const float * phi, const float * mu,
+			      const float * u, float * phi_out,
+			      hipStream_t s);
+
+// Fill n doubles/floats with a value
+void launch_fill_f64(double * a, int64_t n, double v, hipStream_t s);
+void launch_fill_f32(float * a, int64_t n, float v, hipStream_t s);
+
+// Compensated (Neumaier/Klein) FP64 sum of n values: partial results in
+// work (>= 3*nblocks doubles), final in out[0..2] (sum, cs, ccs).
+int  sum_blocks(int64_t n);
+void launch_klein_sum_f64(const double * a, int64_t n, double * work,
+			  double * out, hipStream_t s);
+void launch_klein_sum_f32(const float * a, int64_t n, double * work,
+			  double * out, hipStream_t s);
+// sum of the 19 planes of g (distribution total), plus rho0*nsite shift
+void launch_klein_sum_planes_f32(const float * g, int64_t nsite, int nplane,
+				 double * work, double * out, hipStream_t s);
+
+}  // namespace ldw
+
+#endif

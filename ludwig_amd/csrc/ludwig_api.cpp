@@ -1,0 +1,1547 @@
+// ludwig_api.cpp -- the C ABI of libludwig_hip.so (include/ludwig_hip.h).
+//
+// Host objects mirror the reference's lb_t / field_t / hydro_t / fe_symm_t
+// / pth_t / phi_ch_t (lb_data.h:107-138, field.h:68-88, hydro.h:32-47,
+// symmetric.h:28-42, phi_force_stress.h, phi_cahn_hilliard.h:36-46): a host
+// copy in the reference's SoA layout with halos, and a device copy in the
+// device layout (periodic, no halo, SoA).  Device work is ordered on one
+// HIP stream of default priority.
+//
+// The time-step calls keep the reference's semantics while letting the
+// kernels fuse:
+//   - lb_propagation() marks the distributions as "streamed"; the next
+//     lb_collide() reads them through the pull indexing (K1 fuses both);
+//   - hydro_f_zero()/hydro_u_zero() record the uniform value; the next
+//     kernel that needs the array takes it as a constant;
+//   - field_grad_compute() is deferred into phi_force_calculation() (K2),
+//     which writes the force and mu; phi_cahn_hilliard() (K3) consumes mu;
+//   - field_halo(), lb_halo(), hydro_u_halo() are no-ops: the kernels wrap
+//     indices periodically.
+// Any *_memcpy(DeviceToHost) materialises the deferred state first, so the
+// host sees exactly what the reference would hold.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ludwig_hip.h"
+#include "d3q19.h"
+#include "kernels.h"
+
+using ldw::Geom;
+using ldw::LbParams;
+using ldw::FeParams;
+
+/* ------------------------------------------------------------------------ */
+/* Errors                                                                   */
+/* ------------------------------------------------------------------------ */
+
+static int g_fatal = 1;
+static char g_errmsg[1024] = "";
+
+static int ldw_fail(const char * file, int line, const char * fmt, ...) {
+  char buf[768];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  snprintf(g_errmsg, sizeof(g_errmsg), "%s (%s:%d)", buf, file, line);
+  if (g_fatal) {
+    fprintf(stderr, "libludwig_hip: fatal: %s\n", g_errmsg);
+    fflush(stderr);
+    abort();
+  }
+  return -1;
+}
+
+#define FAIL(...) return ldw_fail(__FILE__, __LINE__, __VA_ARGS__)
+
+#define HIP_CHECK(call)							\
+  do {									\
+    hipError_t err_ = (call);						\
+    if (err_ != hipSuccess) {						\
+      return ldw_fail(__FILE__, __LINE__, "%s: %s", #call,		\
+		      hipGetErrorString(err_));				\
+    }									\
+  } while (0)
+
+#define CHECK_LAUNCH() HIP_CHECK(hipGetLastError())
+
+extern "C" int ldw_error_mode_set(int fatal) { g_fatal = fatal ? 1 : 0; return 0; }
+extern "C" const char * ldw_last_error(void) { return g_errmsg; }
+
+/* ------------------------------------------------------------------------ */
+/* Device, stream, precision, timers                                        */
+/* ------------------------------------------------------------------------ */
+
+static hipStream_t g_stream = nullptr;
+static int g_ready = 0;
+static int g_precision = 0;
+
+static int device_init(void) {
+  if (g_ready) return 0;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    FAIL("no HIP device available: libludwig_hip runs on an MI355X (gfx950)");
+  }
+  HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_ready = 1;
+  return 0;
+}
+
+static int precision_now(void) {
+  if (g_precision == 0) {
+    const char * e = getenv("LDW_PRECISION");
+    g_precision = (e && atoi(e) == 32) ? 32 : 64;
+  }
+  return g_precision;
+}
+
+extern "C" int ldw_precision_set(int bits) {
+  if (bits != 32 && bits != 64) FAIL("precision must be 32 or 64, got %d", bits);
+  g_precision = bits;
+  return 0;
+}
+
+extern "C" int ldw_precision(void) { return precision_now(); }
+
+extern "C" int ldw_device_synchronize(void) {
+  if (!g_ready) return 0;
+  HIP_CHECK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+struct TimerAcc {
+  double ms = 0.0;
+  int count = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+static int g_timing = 0;
+static std::map<std::string, TimerAcc> g_timers;
+static std::vector<hipEvent_t> g_event_pool;
+
+static hipEvent_t event_get(void) {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+static void timers_drain(void) {
+  for (auto & kv : g_timers) {
+    for (auto & pr : kv.second.pending) {
+      float ms = 0.0f;
+      (void) hipEventSynchronize(pr.second);
+      (void) hipEventElapsedTime(&ms, pr.first, pr.second);
+      kv.second.ms += ms;
+      kv.second.count += 1;
+      g_event_pool.push_back(pr.first);
+      g_event_pool.push_back(pr.second);
+    }
+    kv.second.pending.clear();
+  }
+}
+
+struct KTimer {
+  const char * name;
+  hipEvent_t a = nullptr, b = nullptr;
+  explicit KTimer(const char * n) : name(n) {
+    if (g_timing) {
+      a = event_get(); b = event_get();
+      if (a) (void) hipEventRecord(a, g_stream);
+    }
+  }
+  ~KTimer() {
+    if (g_timing && a && b) {
+      (void) hipEventRecord(b, g_stream);
+      g_timers[name].pending.push_back({a, b});
+      if (g_timers[name].pending.size() > 256) timers_drain();
+    }
+  }
+};
+
+extern "C" int ldw_timer_enable(int on) { g_timing = on ? 1 : 0; return 0; }
+
+extern "C" int ldw_timer_reset(void) {
+  timers_drain();
+  g_timers.clear();
+  return 0;
+}
+
+extern "C" int ldw_timer_read(const char * name, double * ms, int * count) {
+  timers_drain();
+  auto it = g_timers.find(name);
+  if (it == g_timers.end()) { *ms = 0.0; *count = 0; return 0; }
+  *ms = it->second.ms;
+  *count = it->second.count;
+  return 0;
+}
+
+static int dmalloc(void ** p, size_t bytes) {
+  if (device_init()) return -1;
+  HIP_CHECK(hipMalloc(p, bytes));
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* pe, cs, physics                                                          */
+/* ------------------------------------------------------------------------ */
+
+struct pe_s { int verbose; };
+
+extern "C" int pe_create(int, pe_enum_t flag, pe_t ** ppe) {
+  if (!ppe) FAIL("pe_create: null argument");
+  pe_t * pe = (pe_t *) calloc(1, sizeof(pe_t));
+  pe->verbose = (flag == PE_VERBOSE);
+  *ppe = pe;
+  return 0;
+}
+
+extern "C" int pe_free(pe_t * pe) { free(pe); return 0; }
+
+struct cs_s {
+  pe_t * pe;
+  int ntotal[3];
+  int nhalo;
+  int init;
+  int str[3];
+  int nsites;
+};
+
+extern "C" int cs_create(pe_t * pe, cs_t ** pcs) {
+  if (!pcs) FAIL("cs_create: null argument");
+  cs_t * cs = (cs_t *) calloc(1, sizeof(cs_t));
+  cs->pe = pe;
+  cs->ntotal[0] = cs->ntotal[1] = cs->ntotal[2] = 64;   /* coords.c default */
+  cs->nhalo = 1;
+  *pcs = cs;
+  return 0;
+}
+
+extern "C" int cs_free(cs_t * cs) { free(cs); return 0; }
+
+extern "C" int cs_ntotal_set(cs_t * cs, const int n[3]) {
+  if (cs->init) FAIL("cs_ntotal_set after cs_init");
+  for (int a = 0; a < 3; a++) {
+    if (n[a] < 1) FAIL("cs_ntotal_set: bad size %d", n[a]);
+    cs->ntotal[a] = n[a];
+  }
+  return 0;
+}
+
+extern "C" int cs_nhalo_set(cs_t * cs, int nhalo) {
+  if (cs->init) FAIL("cs_nhalo_set after cs_init");
+  if (nhalo < 1) FAIL("cs_nhalo_set: nhalo %d < 1", nhalo);
+  cs->nhalo = nhalo;
+  return 0;
+}
+
+/* strides as coords.c:211-215 */
+extern "C" int cs_init(cs_t * cs) {
+  cs->str[2] = 1;
+  cs->str[1] = cs->str[2]*(cs->ntotal[2] + 2*cs->nhalo);
+  cs->str[0] = cs->str[1]*(cs->ntotal[1] + 2*cs->nhalo);
+  long long ns = (long long) cs->str[0]*(cs->ntotal[0] + 2*cs->nhalo);
+  if (ns > 2147483647LL) FAIL("cs_init: lattice too large for int host indexing");
+  cs->nsites = (int) ns;
+  cs->init = 1;
+  return 0;
+}
+
+extern "C" int cs_nhalo(cs_t * cs, int * nhalo) { *nhalo = cs->nhalo; return 0; }
+extern "C" int cs_ntotal(cs_t * cs, int n[3]) { for (int a = 0; a < 3; a++) n[a] = cs->ntotal[a]; return 0; }
+extern "C" int cs_nlocal(cs_t * cs, int n[3]) { for (int a = 0; a < 3; a++) n[a] = cs->ntotal[a]; return 0; }
+extern "C" int cs_nsites(cs_t * cs, int * nsites) { *nsites = cs->nsites; return 0; }
+
+/* coords.c:617-631 */
+extern "C" int cs_index(cs_t * cs, int ic, int jc, int kc) {
+  return cs->str[0]*(cs->nhalo + ic - 1) + cs->str[1]*(cs->nhalo + jc - 1)
+    + cs->str[2]*(cs->nhalo + kc - 1);
+}
+
+static Geom geom_of(cs_t * cs) {
+  Geom g;
+  g.nx = cs->ntotal[0]; g.ny = cs->ntotal[1]; g.nz = cs->ntotal[2];
+  g.nsite = (int64_t) g.nx*g.ny*g.nz;
+  return g;
+}
+
+struct physics_s {
+  double rho0, phi0, eta_shear, eta_bulk, mobility, fbody[3];
+};
+
+static physics_t * g_phys = nullptr;
+
+extern "C" int physics_create(pe_t *, physics_t ** phys) {
+  physics_t * p = (physics_t *) calloc(1, sizeof(physics_t));
+  p->rho0 = 1.0;            /* physics.c RHO_DEFAULT */
+  p->eta_shear = 0.1;       /* physics.c ETA_DEFAULT */
+  p->eta_bulk = 0.1;
+  p->mobility = 0.0;
+  *phys = p;
+  g_phys = p;
+  return 0;
+}
+
+extern "C" int physics_free(physics_t * phys) {
+  if (g_phys == phys) g_phys = nullptr;
+  free(phys);
+  return 0;
+}
+
+extern "C" int physics_ref(physics_t ** ref) {
+  if (g_phys == nullptr) physics_create(nullptr, &g_phys);
+  *ref = g_phys;
+  return 0;
+}
+
+extern "C" int physics_rho0_set(physics_t * p, double v) { p->rho0 = v; return 0; }
+extern "C" int physics_phi0_set(physics_t * p, double v) { p->phi0 = v; return 0; }
+extern "C" int physics_eta_shear_set(physics_t * p, double v) { p->eta_shear = v; return 0; }
+extern "C" int physics_eta_bulk_set(physics_t * p, double v) { p->eta_bulk = v; return 0; }
+extern "C" int physics_mobility_set(physics_t * p, double v) { p->mobility = v; return 0; }
+extern "C" int physics_fbody_set(physics_t * p, double f[3]) { for (int a = 0; a < 3; a++) p->fbody[a] = f[a]; return 0; }
+extern "C" int physics_rho0(physics_t * p, double * v) { *v = p->rho0; return 0; }
+extern "C" int physics_phi0(physics_t * p, double * v) { *v = p->phi0; return 0; }
+extern "C" int physics_eta_shear(physics_t * p, double * v) { *v = p->eta_shear; return 0; }
+extern "C" int physics_eta_bulk(physics_t * p, double * v) { *v = p->eta_bulk; return 0; }
+extern "C" int physics_mobility(physics_t * p, double * v) { *v = p->mobility; return 0; }
+extern "C" int physics_fbody(physics_t * p, double f[3]) { for (int a = 0; a < 3; a++) f[a] = p->fbody[a]; return 0; }
+
+/* ------------------------------------------------------------------------ */
+/* Host <-> device layout                                                   */
+/* ------------------------------------------------------------------------ */
+
+// device site of interior point (ic, jc, kc), 1-based
+static inline int64_t dsite(const Geom & g, int ic, int jc, int kc) {
+  return ((int64_t) (ic - 1)*g.ny + (jc - 1))*g.nz + (kc - 1);
+}
+
+static inline int wrap1(int i, int n) {   // 1-based periodic image
+  while (i < 1) i += n;
+  while (i > n) i -= n;
+  return i;
+}
+
+// Copy component planes between a host SoA array with halo (nhost sites,
+// component stride nhost) and a device-layout array (component stride
+// nsite).  Host halos receive the periodic images.
+template <typename T>
+static void pack_host(cs_t * cs, const double * host, int ncomp, T * dev) {
+  Geom g = geom_of(cs);
+  for (int c = 0; c < ncomp; c++)
+    for (int ic = 1; ic <= g.nx; ic++)
+      for (int jc = 1; jc <= g.ny; jc++)
+	for (int kc = 1; kc <= g.nz; kc++)
+	  dev[c*g.nsite + dsite(g, ic, jc, kc)]
+	    = (T) host[(int64_t) c*cs->nsites + cs_index(cs, ic, jc, kc)];
+}
+
+template <typename T>
+static void unpack_host(cs_t * cs, const T * dev, int ncomp, double * host) {
+  Geom g = geom_of(cs);
+  const int h = cs->nhalo;
+  for (int c = 0; c < ncomp; c++)
+    for (int ic = 1 - h; ic <= g.nx + h; ic++)
+      for (int jc = 1 - h; jc <= g.ny + h; jc++)
+	for (int kc = 1 - h; kc <= g.nz + h; kc++)
+	  host[(int64_t) c*cs->nsites + cs_index(cs, ic, jc, kc)]
+	    = (double) dev[c*g.nsite + dsite(g, wrap1(ic, g.nx), wrap1(jc, g.ny), wrap1(kc, g.nz))];
+}
+
+/* ------------------------------------------------------------------------ */
+/* lb_t                                                                     */
+/* ------------------------------------------------------------------------ */
+
+struct lb_data_s {
+  pe_t * pe;
+  cs_t * cs;
+  int ndim, nvel, ndist;
+  lb_relaxation_enum_t nrelax;
+  int isghost;
+  int nsite;               // host sites incl. halo
+  double * f;              // host copy, SoA f[p*nsite + index]
+  int prec;                // 64 or 32
+  void * d_g[2];           // device buffers (post-collision values)
+  int cur;                 // current buffer
+  int streamed;            // 1: logical f(x)[p] = g(x - c_p)[p]
+  Geom geo;
+  double * d_work;         // reduction scratch
+  double * d_out;
+};
+
+extern "C" lb_data_options_t lb_data_options_default(void) {
+  lb_data_options_t o;
+  o.ndim = 3; o.nvel = 19; o.ndist = 1;
+  o.nrelax = LB_RELAXATION_M10; o.halo = LB_HALO_FULL;
+  o.reportimbalance = 0; o.usefirsttouch = 0;
+  return o;
+}
+
+extern "C" int lb_data_create(pe_t * pe, cs_t * cs, const lb_data_options_t * opts,
+			      lb_t ** plb) {
+  if (!cs || !cs->init || !opts || !plb) FAIL("lb_data_create: bad arguments");
+  if (opts->ndim != 3 || opts->nvel != 19) FAIL("lb_data_create: only D3Q19 is built (ndim %d nvel %d)", opts->ndim, opts->nvel);
+  if (opts->ndist != 1) FAIL("lb_data_create: ndist %d not supported (the two-distribution binary LB is out of scope)", opts->ndist);
+  if (device_init()) return -1;
+
+  lb_t * lb = (lb_t *) calloc(1, sizeof(lb_t));
+  lb->pe = pe; lb->cs = cs;
+  lb->ndim = 3; lb->nvel = 19; lb->ndist = 1;
+  lb->nrelax = opts->nrelax;
+  lb->isghost = 1;
+  lb->nsite = cs->nsites;
+  lb->geo = geom_of(cs);
+  lb->prec = precision_now();
+  lb->f = (double *) calloc((size_t) 19*lb->nsite, sizeof(double));
+  size_t es = (lb->prec == 64) ? 8 : 4;
+  for (int b = 0; b < 2; b++) {
+    if (dmalloc(&lb->d_g[b], es*19*(size_t) lb->geo.nsite)) return -1;
+  }
+  if (dmalloc((void **) &lb->d_work, sizeof(double)*2*1024)) return -1;
+  if (dmalloc((void **) &lb->d_out, sizeof(double)*4)) return -1;
+  lb->cur = 0;
+  lb->streamed = 0;
+  *plb = lb;
+  return 0;
+}
+
+extern "C" int lb_free(lb_t * lb) {
+  if (!lb) return 0;
+  if (g_ready) (void) hipStreamSynchronize(g_stream);
+  for (int b = 0; b < 2; b++) (void) hipFree(lb->d_g[b]);
+  (void) hipFree(lb->d_work);
+  (void) hipFree(lb->d_out);
+  free(lb->f);
+  free(lb);
+  return 0;
+}
+
+extern "C" int lb_ndist(lb_t * lb, int * ndist) { *ndist = lb->ndist; return 0; }
+extern "C" int lb_nvel(lb_t * lb, int * nvel) { *nvel = lb->nvel; return 0; }
+
+extern "C" int lb_f(lb_t * lb, int index, int p, int n, double * f) {
+  *f = lb->f[((int64_t) n*19 + p)*lb->nsite + index];
+  return 0;
+}
+
+extern "C" int lb_f_set(lb_t * lb, int index, int p, int n, double f) {
+  lb->f[((int64_t) n*19 + p)*lb->nsite + index] = f;
+  return 0;
+}
+
+/* lb_data.h:140-190 */
+extern "C" int lb_0th_moment(lb_t * lb, int index, lb_dist_enum_t nd, double * rho) {
+  *rho = 0.0;
+  for (int p = 0; p < 19; p++) *rho += lb->f[((int64_t) nd*19 + p)*lb->nsite + index];
+  return 0;
+}
+
+extern "C" int lb_1st_moment(lb_t * lb, int index, lb_dist_enum_t nd, double g[3]) {
+  for (int a = 0; a < 3; a++) g[a] = 0.0;
+  for (int p = 0; p < 19; p++)
+    for (int a = 0; a < 3; a++)
+      g[a] += ldw::CV.v[p][a]*lb->f[((int64_t) nd*19 + p)*lb->nsite + index];
+  return 0;
+}
+
+/* lb_data.c:809-840, evaluated on the host copy */
+extern "C" int lb_1st_moment_equilib_set(lb_t * lb, int index, double rho, double u[3]) {
+  for (int p = 0; p < 19; p++) {
+    double cs2 = (1.0/3.0);
+    double rcs2 = 1.0/cs2;
+    double udotc = 0.0;
+    double sdotq = 0.0;
+    for (int ia = 0; ia < 3; ia++) {
+      udotc += u[ia]*ldw::CV.v[p][ia];
+      for (int ib = 0; ib < 3; ib++) {
+	double dab = (ia == ib);
+	sdotq += (ldw::CV.v[p][ia]*ldw::CV.v[p][ib] - cs2*dab)*u[ia]*u[ib];
+      }
+    }
+    lb->f[(int64_t) p*lb->nsite + index]
+      = rho*ldw::weight(p)*(1.0 + rcs2*udotc + 0.5*rcs2*rcs2*sdotq);
+  }
+  return 0;
+}
+
+/* lb_data.c:659-685 */
+extern "C" int lb_init_rest_f(lb_t * lb, double rho0) {
+  Geom g = lb->geo;
+  for (int ic = 1; ic <= g.nx; ic++)
+    for (int jc = 1; jc <= g.ny; jc++)
+      for (int kc = 1; kc <= g.nz; kc++) {
+	double u0[3] = {0.0, 0.0, 0.0};
+	lb_1st_moment_equilib_set(lb, cs_index(lb->cs, ic, jc, kc), rho0, u0);
+      }
+  return lb_memcpy(lb, tdpMemcpyHostToDevice);
+}
+
+static double lb_rho0(void) {
+  physics_t * p = nullptr;
+  physics_ref(&p);
+  return p->rho0;
+}
+
+extern "C" int lb_memcpy(lb_t * lb, tdpMemcpyKind flag) {
+  const Geom g = lb->geo;
+  const size_t n = (size_t) 19*g.nsite;
+  const double rho0 = lb_rho0();
+
+  if (flag == tdpMemcpyHostToDevice) {
+    // device holds g with streamed = 0: logical f = g
+    if (lb->prec == 64) {
+      std::vector<double> buf(n);
+      pack_host<double>(lb->cs, lb->f, 19, buf.data());
+      HIP_CHECK(hipMemcpyAsync(lb->d_g[lb->cur], buf.data(), n*8, hipMemcpyHostToDevice, g_stream));
+      HIP_CHECK(hipStreamSynchronize(g_stream));
+    }
+    else {
+      std::vector<float> buf(n);
+      Geom gg = g;
+      for (int p = 0; p < 19; p++)
+	for (int ic = 1; ic <= gg.nx; ic++)
+	  for (int jc = 1; jc <= gg.ny; jc++)
+	    for (int kc = 1; kc <= gg.nz; kc++)
+	      buf[p*gg.nsite + dsite(gg, ic, jc, kc)]
+		= (float) (lb->f[(int64_t) p*lb->nsite + cs_index(lb->cs, ic, jc, kc)]
+			   - ldw::weight(p)*rho0);
+      HIP_CHECK(hipMemcpyAsync(lb->d_g[lb->cur], buf.data(), n*4, hipMemcpyHostToDevice, g_stream));
+      HIP_CHECK(hipStreamSynchronize(g_stream));
+    }
+    lb->streamed = 0;
+    return 0;
+  }
+
+  if (flag == tdpMemcpyDeviceToHost) {
+    std::vector<double> gd(n);
+    if (lb->prec == 64) {
+      HIP_CHECK(hipMemcpyAsync(gd.data(), lb->d_g[lb->cur], n*8, hipMemcpyDeviceToHost, g_stream));
+      HIP_CHECK(hipStreamSynchronize(g_stream));
+    }
+    else {
+      std::vector<float> gf(n);
+      HIP_CHECK(hipMemcpyAsync(gf.data(), lb->d_g[lb->cur], n*4, hipMemcpyDeviceToHost, g_stream));
+      HIP_CHECK(hipStreamSynchronize(g_stream));
+      for (int p = 0; p < 19; p++)
+	for (int64_t s = 0; s < g.nsite; s++)
+	  gd[p*g.nsite + s] = (double) gf[p*g.nsite + s] + ldw::weight(p)*rho0;
+    }
+    // logical f: g itself, or g(x - c_p) once streamed; halos periodic
+    const int h = lb->cs->nhalo;
+    for (int p = 0; p < 19; p++) {
+      const int sh = lb->streamed ? 1 : 0;
+      const int cx = ldw::CV.v[p][0]*sh, cy = ldw::CV.v[p][1]*sh, cz = ldw::CV.v[p][2]*sh;
+      for (int ic = 1 - h; ic <= g.nx + h; ic++)
+	for (int jc = 1 - h; jc <= g.ny + h; jc++)
+	  for (int kc = 1 - h; kc <= g.nz + h; kc++) {
+	    int64_t s = dsite(g, wrap1(ic - cx, g.nx), wrap1(jc - cy, g.ny), wrap1(kc - cz, g.nz));
+	    lb->f[(int64_t) p*lb->nsite + cs_index(lb->cs, ic, jc, kc)] = gd[p*g.nsite + s];
+	  }
+    }
+    return 0;
+  }
+  return 0;
+}
+
+extern "C" int lb_halo(lb_t *) { return 0; }
+
+extern "C" int lb_collision_relaxation_set(lb_t * lb, lb_relaxation_enum_t nrelax) {
+  if (nrelax != LB_RELAXATION_M10 && nrelax != LB_RELAXATION_BGK && nrelax != LB_RELAXATION_TRT) {
+    FAIL("lb_collision_relaxation_set: bad scheme %d", (int) nrelax);
+  }
+  lb->nrelax = nrelax;
+  return 0;
+}
+
+extern "C" int lb_collision_ghost_modes_on(lb_t * lb) { lb->isghost = 1; return 0; }
+extern "C" int lb_collision_ghost_modes_off(lb_t * lb) { lb->isghost = 0; return 0; }
+
+/* ------------------------------------------------------------------------ */
+/* le, map, wall, noise, visc                                               */
+/* ------------------------------------------------------------------------ */
+
+struct lees_edw_s { int nplanes; };
+
+extern "C" int lees_edw_create(pe_t *, cs_t *, const lees_edw_options_t * opts, lees_edw_t ** le) {
+  if (opts && opts->nplanes != 0) FAIL("lees_edw_create: Lees-Edwards planes are not supported");
+  *le = (lees_edw_t *) calloc(1, sizeof(lees_edw_t));
+  return 0;
+}
+
+extern "C" int lees_edw_free(lees_edw_t * le) { free(le); return 0; }
+
+struct map_s { int ndata; };
+
+extern "C" map_options_t map_options_default(void) { map_options_t o = {0, 0}; return o; }
+
+extern "C" int map_create(pe_t *, cs_t *, const map_options_t * options, map_t ** pmap) {
+  if (options && options->is_porous_media) FAIL("map_create: porous media not supported");
+  map_t * m = (map_t *) calloc(1, sizeof(map_t));
+  m->ndata = options ? options->ndata : 0;
+  *pmap = m;
+  return 0;
+}
+
+extern "C" int map_free(map_t ** map) { if (map) { free(*map); *map = nullptr; } return 0; }
+
+struct wall_s { int present; };
+
+extern "C" int wall_create(pe_t *, cs_t *, map_t *, lb_t *, wall_t ** p) {
+  *p = (wall_t *) calloc(1, sizeof(wall_t));
+  return 0;
+}
+
+extern "C" int wall_free(wall_t * wall) { free(wall); return 0; }
+
+/* ------------------------------------------------------------------------ */
+/* field_t, field_grad_t                                                    */
+/* ------------------------------------------------------------------------ */
+
+struct field_s {
+  pe_t * pe;
+  cs_t * cs;
+  char name[64];
+  int nf;
+  int nhcomm;
+  int nsites;
+  double * data;           // host, SoA data[n*nsites + index]
+  int prec;
+  void * d_data[2];        // device ping-pong
+  int cur;
+  long version;            // bumped whenever the device values change
+  Geom geo;
+};
+
+struct field_grad_s {
+  field_t * field;
+  int level;
+  double * grad;           // host grad[a*nsite + index]
+  double * delsq;          // host
+  void * d_mu;             // device mu for the version mu_version
+  long mu_version;
+  void * d_grad;           // materialised only for reads
+  void * d_delsq;
+  grad_ft d2;
+};
+
+extern "C" field_options_t field_options_ndata_nhalo(int ndata, int nhalo) {
+  field_options_t o = {ndata, nhalo, 0, 0, 0};
+  return o;
+}
+
+extern "C" int field_create(pe_t * pe, cs_t * cs, lees_edw_t *, const char * name,
+			    const field_options_t * opts, field_t ** pobj) {
+  if (!cs || !cs->init || !opts || !pobj) FAIL("field_create: bad arguments");
+  if (opts->ndata != 1) FAIL("field_create: only scalar fields (ndata 1) are on the path, got %d", opts->ndata);
+  if (opts->nhcomm > cs->nhalo) FAIL("field_create: halo %d exceeds coordinate halo %d", opts->nhcomm, cs->nhalo);
+  field_t * f = (field_t *) calloc(1, sizeof(field_t));
+  f->pe = pe; f->cs = cs;
+  snprintf(f->name, sizeof(f->name), "%s", name ? name : "");
+  f->nf = opts->ndata;
+  f->nhcomm = opts->nhcomm;
+  f->nsites = cs->nsites;
+  f->geo = geom_of(cs);
+  f->prec = precision_now();
+  f->data = (double *) calloc((size_t) f->nf*f->nsites, sizeof(double));
+  size_t es = (f->prec == 64) ? 8 : 4;
+  for (int b = 0; b < 2; b++) {
+    if (dmalloc(&f->d_data[b], es*f->geo.nsite)) return -1;
+    HIP_CHECK(hipMemsetAsync(f->d_data[b], 0, es*f->geo.nsite, g_stream));
+  }
+  f->version = 1;
+  *pobj = f;
+  return 0;
+}
+
+extern "C" int field_free(field_t * f) {
+  if (!f) return 0;
+  if (g_ready) (void) hipStreamSynchronize(g_stream);
+  for (int b = 0; b < 2; b++) (void) hipFree(f->d_data[b]);
+  free(f->data);
+  free(f);
+  return 0;
+}
+
+extern "C" int field_nf(field_t * f, int * nf) { *nf = f->nf; return 0; }
+
+extern "C" int field_scalar(field_t * f, int index, double * phi) {
+  *phi = f->data[index];
+  return 0;
+}
+
+extern "C" int field_scalar_set(field_t * f, int index, double phi) {
+  f->data[index] = phi;
+  return 0;
+}
+
+extern "C" int field_memcpy(field_t * f, tdpMemcpyKind flag) {
+  const Geom g = f->geo;
+  if (flag == tdpMemcpyHostToDevice) {
+    if (f->prec == 64) {
+      std::vector<double> buf(g.nsite);
+      pack_host<double>(f->cs, f->data, 1, buf.data());
+      HIP_CHECK(hipMemcpyAsync(f->d_data[f->cur], buf.data(), 8*g.nsite, hipMemcpyHostToDevice, g_stream));
+      HIP_CHECK(hipStreamSynchronize(g_stream));
+    }
+    else {
+      std::vector<float> buf(g.nsite);
+      pack_host<float>(f->cs, f->data, 1, buf.data());
+      HIP_CHECK(hipMemcpyAsync(f->d_data[f->cur], buf.data(), 4*g.nsite, hipMemcpyHostToDevice, g_stream));
+      HIP_CHECK(hipStreamSynchronize(g_stream));
+    }
+    f->version++;
+  }
+  else if (flag == tdpMemcpyDeviceToHost) {
+    if (f->prec == 64) {
+      std::vector<double> buf(g.nsite);
+      HIP_CHECK(hipMemcpyAsync(buf.data(), f->d_data[f->cur], 8*g.nsite, hipMemcpyDeviceToHost, g_stream));
+      HIP_CHECK(hipStreamSynchronize(g_stream));
+      unpack_host<double>(f->cs, buf.data(), 1, f->data);
+    }
+    else {
+      std::vector<float> buf(g.nsite);
+      HIP_CHECK(hipMemcpyAsync(buf.data(), f->d_data[f->cur], 4*g.nsite, hipMemcpyDeviceToHost, g_stream));
+      HIP_CHECK(hipStreamSynchronize(g_stream));
+      unpack_host<float>(f->cs, buf.data(), 1, f->data);
+    }
+  }
+  return 0;
+}
+
+extern "C" int field_halo(field_t *) { return 0; }
+
+extern "C" int grad_3d_7pt_fluid_d2(field_grad_t *) { return 0; }
+
+extern "C" int field_grad_create(pe_t *, field_t * f, int level, field_grad_t ** pobj) {
+  if (!f || !pobj) FAIL("field_grad_create: bad arguments");
+  field_grad_t * g = (field_grad_t *) calloc(1, sizeof(field_grad_t));
+  g->field = f;
+  g->level = level;
+  g->grad = (double *) calloc((size_t) 3*f->nsites, sizeof(double));
+  g->delsq = (double *) calloc((size_t) f->nsites, sizeof(double));
+  size_t es = (f->prec == 64) ? 8 : 4;
+  if (dmalloc(&g->d_mu, es*f->geo.nsite)) return -1;
+  g->mu_version = 0;
+  g->d2 = grad_3d_7pt_fluid_d2;
+  *pobj = g;
+  return 0;
+}
+
+extern "C" void field_grad_free(field_grad_t * g) {
+  if (!g) return;
+  if (g_ready) (void) hipStreamSynchronize(g_stream);
+  (void) hipFree(g->d_mu);
+  if (g->d_grad) (void) hipFree(g->d_grad);
+  if (g->d_delsq) (void) hipFree(g->d_delsq);
+  free(g->grad);
+  free(g->delsq);
+  free(g);
+}
+
+extern "C" int field_grad_set(field_grad_t * g, grad_ft d2, grad_ft) {
+  if (d2 != grad_3d_7pt_fluid_d2) FAIL("field_grad_set: only grad_3d_7pt_fluid_d2 is built");
+  g->d2 = d2;
+  return 0;
+}
+
+/* Deferred: the stencil is evaluated inside the kernels that consume it. */
+extern "C" int field_grad_compute(field_grad_t *) { return 0; }
+
+extern "C" int field_grad_memcpy(field_grad_t * g, tdpMemcpyKind flag) {
+  if (flag != tdpMemcpyDeviceToHost) return 0;
+  field_t * f = g->field;
+  const Geom geo = f->geo;
+  if (f->prec != 64) FAIL("field_grad_memcpy: gradient read-back is built for FP64 only");
+  if (!g->d_grad) {
+    if (dmalloc(&g->d_grad, 8*3*(size_t) geo.nsite)) return -1;
+    if (dmalloc(&g->d_delsq, 8*(size_t) geo.nsite)) return -1;
+  }
+  ldw::launch_grad7_f64(geo, (const double *) f->d_data[f->cur], (double *) g->d_grad,
+			(double *) g->d_delsq, g_stream);
+  CHECK_LAUNCH();
+  std::vector<double> b1(3*geo.nsite), b2(geo.nsite);
+  HIP_CHECK(hipMemcpyAsync(b1.data(), g->d_grad, 8*3*geo.nsite, hipMemcpyDeviceToHost, g_stream));
+  HIP_CHECK(hipMemcpyAsync(b2.data(), g->d_delsq, 8*geo.nsite, hipMemcpyDeviceToHost, g_stream));
+  HIP_CHECK(hipStreamSynchronize(g_stream));
+  unpack_host<double>(f->cs, b1.data(), 3, g->grad);
+  unpack_host<double>(f->cs, b2.data(), 1, g->delsq);
+  return 0;
+}
+
+extern "C" int field_grad_scalar_grad(field_grad_t * g, int index, double grad[3]) {
+  for (int a = 0; a < 3; a++) grad[a] = g->grad[(int64_t) a*g->field->nsites + index];
+  return 0;
+}
+
+extern "C" int field_grad_scalar_delsq(field_grad_t * g, int index, double * delsq) {
+  *delsq = g->delsq[index];
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* hydro_t                                                                  */
+/* ------------------------------------------------------------------------ */
+
+struct hydro_s {
+  pe_t * pe;
+  cs_t * cs;
+  int nhcomm;
+  int nsite;
+  double * rho, * u, * force;      // host SoA with halo
+  int prec;
+  void * d_rho, * d_u, * d_force;
+  int force_uniform;               // 1: force is uniform fzero (not materialised)
+  double fzero[3];
+  int u_uniform;                   // 1: u is uniform uzero
+  double uzero[3];
+  Geom geo;
+};
+
+extern "C" hydro_options_t hydro_options_default(void) { hydro_options_t o = {1}; return o; }
+
+extern "C" int hydro_create(pe_t * pe, cs_t * cs, lees_edw_t *, const hydro_options_t * opts,
+			    hydro_t ** pobj) {
+  if (!cs || !cs->init || !pobj) FAIL("hydro_create: bad arguments");
+  hydro_t * h = (hydro_t *) calloc(1, sizeof(hydro_t));
+  h->pe = pe; h->cs = cs;
+  h->nhcomm = opts ? opts->nhcomm : 1;
+  h->nsite = cs->nsites;
+  h->geo = geom_of(cs);
+  h->prec = precision_now();
+  h->rho = (double *) calloc((size_t) h->nsite, sizeof(double));
+  h->u = (double *) calloc((size_t) 3*h->nsite, sizeof(double));
+  h->force = (double *) calloc((size_t) 3*h->nsite, sizeof(double));
+  size_t es = (h->prec == 64) ? 8 : 4;
+  if (dmalloc(&h->d_rho, es*h->geo.nsite)) return -1;
+  if (dmalloc(&h->d_u, es*3*h->geo.nsite)) return -1;
+  if (dmalloc(&h->d_force, es*3*h->geo.nsite)) return -1;
+  // hydro_create zeroes rho? The reference initialises rho to rho0 at
+  // start (hydro_rho0); u and force are zero.
+  HIP_CHECK(hipMemsetAsync(h->d_rho, 0, es*h->geo.nsite, g_stream));
+  h->u_uniform = 1;
+  h->force_uniform = 1;
+  *pobj = h;
+  return 0;
+}
+
+extern "C" int hydro_free(hydro_t * h) {
+  if (!h) return 0;
+  if (g_ready) (void) hipStreamSynchronize(g_stream);
+  (void) hipFree(h->d_rho); (void) hipFree(h->d_u); (void) hipFree(h->d_force);
+  free(h->rho); free(h->u); free(h->force);
+  free(h);
+  return 0;
+}
+
+static int hydro_materialise(hydro_t * h) {
+  const Geom g = h->geo;
+  if (h->force_uniform) {
+    for (int a = 0; a < 3; a++) {
+      if (h->prec == 64) ldw::launch_fill_f64((double *) h->d_force + a*g.nsite, g.nsite, h->fzero[a], g_stream);
+      else ldw::launch_fill_f32((float *) h->d_force + a*g.nsite, g.nsite, (float) h->fzero[a], g_stream);
+    }
+    CHECK_LAUNCH();
+    h->force_uniform = 0;
+  }
+  if (h->u_uniform) {
+    for (int a = 0; a < 3; a++) {
+      if (h->prec == 64) ldw::launch_fill_f64((double *) h->d_u + a*g.nsite, g.nsite, h->uzero[a], g_stream);
+      else ldw::launch_fill_f32((float *) h->d_u + a*g.nsite, g.nsite, (float) h->uzero[a], g_stream);
+    }
+    CHECK_LAUNCH();
+    h->u_uniform = 0;
+  }
+  return 0;
+}
+
+extern "C" int hydro_memcpy(hydro_t * h, tdpMemcpyKind flag) {
+  const Geom g = h->geo;
+  if (flag == tdpMemcpyHostToDevice) {
+    if (h->prec == 64) {
+      std::vector<double> b(g.nsite), b3(3*g.nsite), f3(3*g.nsite);
+      pack_host<double>(h->cs, h->rho, 1, b.data());
+      pack_host<double>(h->cs, h->u, 3, b3.data());
+      pack_host<double>(h->cs, h->force, 3, f3.data());
+      HIP_CHECK(hipMemcpyAsync(h->d_rho, b.data(), 8*g.nsite, hipMemcpyHostToDevice, g_stream));
+      HIP_CHECK(hipMemcpyAsync(h->d_u, b3.data(), 24*g.nsite, hipMemcpyHostToDevice, g_stream));
+      HIP_CHECK(hipMemcpyAsync(h->d_force, f3.data(), 24*g.nsite, hipMemcpyHostToDevice, g_stream));
+      HIP_CHECK(hipStreamSynchronize(g_stream));
+    }
+    else {
+      std::vector<float> b(g.nsite), b3(3*g.nsite), f3(3*g.nsite);
+      pack_host<float>(h->cs, h->rho, 1, b.data());
+      pack_host<float>(h->cs, h->u, 3, b3.data());
+      pack_host<float>(h->cs, h->force, 3, f3.data());
+      HIP_CHECK(hipMemcpyAsync(h->d_rho, b.data(), 4*g.nsite, hipMemcpyHostToDevice, g_stream));
+      HIP_CHECK(hipMemcpyAsync(h->d_u, b3.data(), 12*g.nsite, hipMemcpyHostToDevice, g_stream));
+      HIP_CHECK(hipMemcpyAsync(h->d_force, f3.data(), 12*g.nsite, hipMemcpyHostToDevice, g_stream));
+      HIP_CHECK(hipStreamSynchronize(g_stream));
+    }
+    h->force_uniform = 0;
+    h->u_uniform = 0;
+    return 0;
+  }
+  if (flag == tdpMemcpyDeviceToHost) {
+    if (hydro_materialise(h)) return -1;
+    if (h->prec == 64) {
+      std::vector<double> b(g.nsite), b3(3*g.nsite), f3(3*g.nsite);
+      HIP_CHECK(hipMemcpyAsync(b.data(), h->d_rho, 8*g.nsite, hipMemcpyDeviceToHost, g_stream));
+      HIP_CHECK(hipMemcpyAsync(b3.data(), h->d_u, 24*g.nsite, hipMemcpyDeviceToHost, g_stream));
+      HIP_CHECK(hipMemcpyAsync(f3.data(), h->d_force, 24*g.nsite, hipMemcpyDeviceToHost, g_stream));
+      HIP_CHECK(hipStreamSynchronize(g_stream));
+      unpack_host<double>(h->cs, b.data(), 1, h->rho);
+      unpack_host<double>(h->cs, b3.data(), 3, h->u);
+      unpack_host<double>(h->cs, f3.data(), 3, h->force);
+    }
+    else {
+      std::vector<float> b(g.nsite), b3(3*g.nsite), f3(3*g.nsite);
+      HIP_CHECK(hipMemcpyAsync(b.data(), h->d_rho, 4*g.nsite, hipMemcpyDeviceToHost, g_stream));
+      HIP_CHECK(hipMemcpyAsync(b3.data(), h->d_u, 12*g.nsite, hipMemcpyDeviceToHost, g_stream));
+      HIP_CHECK(hipMemcpyAsync(f3.data(), h->d_force, 12*g.nsite, hipMemcpyDeviceToHost, g_stream));
+      HIP_CHECK(hipStreamSynchronize(g_stream));
+      unpack_host<float>(h->cs, b.data(), 1, h->rho);
+      unpack_host<float>(h->cs, b3.data(), 3, h->u);
+      unpack_host<float>(h->cs, f3.data(), 3, h->force);
+    }
+  }
+  return 0;
+}
+
+extern "C" int hydro_u_halo(hydro_t *) { return 0; }
+
+extern "C" int hydro_f_zero(hydro_t * h, const double fzero[3]) {
+  h->force_uniform = 1;
+  for (int a = 0; a < 3; a++) h->fzero[a] = fzero[a];
+  return 0;
+}
+
+extern "C" int hydro_u_zero(hydro_t * h, const double uzero[3]) {
+  h->u_uniform = 1;
+  for (int a = 0; a < 3; a++) h->uzero[a] = uzero[a];
+  return 0;
+}
+
+extern "C" int hydro_u(hydro_t * h, int index, double u[3]) {
+  for (int a = 0; a < 3; a++) u[a] = h->u[(int64_t) a*h->nsite + index];
+  return 0;
+}
+
+extern "C" int hydro_u_set(hydro_t * h, int index, const double u[3]) {
+  for (int a = 0; a < 3; a++) h->u[(int64_t) a*h->nsite + index] = u[a];
+  return 0;
+}
+
+extern "C" int hydro_rho(hydro_t * h, int index, double * rho) { *rho = h->rho[index]; return 0; }
+
+extern "C" int hydro_f_local(hydro_t * h, int index, double force[3]) {
+  for (int a = 0; a < 3; a++) force[a] = h->force[(int64_t) a*h->nsite + index];
+  return 0;
+}
+
+extern "C" int hydro_f_local_set(hydro_t * h, int index, const double force[3]) {
+  for (int a = 0; a < 3; a++) h->force[(int64_t) a*h->nsite + index] = force[a];
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Symmetric free energy                                                    */
+/* ------------------------------------------------------------------------ */
+
+enum {LDW_FE_SYMMETRIC = 1};
+
+struct fe_s {
+  int id;
+  int use_stress_relaxation;
+};
+
+struct fe_symm_s {
+  fe_t super;
+  pe_t * pe;
+  cs_t * cs;
+  field_t * phi;
+  field_grad_t * dphi;
+  fe_symm_param_t param;
+};
+
+extern "C" int fe_symm_create(pe_t * pe, cs_t * cs, field_t * f, field_grad_t * grd,
+			      fe_symm_t ** p) {
+  fe_symm_t * fe = (fe_symm_t *) calloc(1, sizeof(fe_symm_t));
+  fe->super.id = LDW_FE_SYMMETRIC;
+  fe->pe = pe; fe->cs = cs; fe->phi = f; fe->dphi = grd;
+  fe->param.a = -0.003125;      /* symmetric.c:40-42 defaults */
+  fe->param.b = +0.003125;
+  fe->param.kappa = +0.002;
+  *p = fe;
+  return 0;
+}
+
+extern "C" int fe_symm_free(fe_symm_t * fe) { free(fe); return 0; }
+
+extern "C" int fe_symm_param_set(fe_symm_t * fe, fe_symm_param_t values) {
+  fe->param = values;
+  fe->dphi->mu_version = 0;     // mu depends on the parameters
+  return 0;
+}
+
+extern "C" int fe_symm_param(fe_symm_t * fe, fe_symm_param_t * values) {
+  *values = fe->param;
+  return 0;
+}
+
+extern "C" int fe_symm_mu(fe_symm_t * fe, int index, double * mu) {
+  double phi = fe->phi->data[index];
+  double delsq = fe->dphi->delsq[index];
+  *mu = fe->param.a*phi + fe->param.b*phi*phi*phi - fe->param.kappa*delsq;
+  return 0;
+}
+
+extern "C" int fe_symm_fed(fe_symm_t * fe, int index, double * fed) {
+  double phi = fe->phi->data[index];
+  double g[3];
+  field_grad_scalar_grad(fe->dphi, index, g);
+  *fed = (0.5*fe->param.a + 0.25*fe->param.b*phi*phi)*phi*phi
+    + 0.5*fe->param.kappa*(g[0]*g[0] + g[1]*g[1] + g[2]*g[2]);
+  return 0;
+}
+
+extern "C" int fe_symm_str(fe_symm_t * fe, int index, double s[3][3]) {
+  double phi = fe->phi->data[index];
+  double delsq = fe->dphi->delsq[index];
+  double g[3];
+  field_grad_scalar_grad(fe->dphi, index, g);
+  const double kappa = fe->param.kappa;
+  double p0 = 0.5*fe->param.a*phi*phi + 0.75*fe->param.b*phi*phi*phi*phi
+    - kappa*phi*delsq - 0.5*kappa*(g[0]*g[0] + g[1]*g[1] + g[2]*g[2]);
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) s[a][b] = p0*(a == b) + kappa*g[a]*g[b];
+  return 0;
+}
+
+static FeParams fe_params_of(fe_symm_t * fe, int method) {
+  FeParams p;
+  physics_t * phys = nullptr;
+  physics_ref(&phys);
+  p.a = fe->param.a; p.b = fe->param.b; p.kappa = fe->param.kappa;
+  p.mobility = phys->mobility;
+  int order = 1;
+  advection_order(&order);
+  p.order = order;
+  p.force_method = (method == FE_FORCE_METHOD_PHI_GRADMU) ? 1 : 0;
+  return p;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Force                                                                    */
+/* ------------------------------------------------------------------------ */
+
+struct pth_s { int method; };
+
+extern "C" int pth_create(pe_t *, cs_t *, int method, pth_t ** pth) {
+  if (method != FE_FORCE_METHOD_NO_FORCE && method != FE_FORCE_METHOD_STRESS_DIVERGENCE
+      && method != FE_FORCE_METHOD_PHI_GRADMU) {
+    FAIL("pth_create: force method %d is not built (no_force, stress_divergence, phi_gradmu are)", method);
+  }
+  pth_t * p = (pth_t *) calloc(1, sizeof(pth_t));
+  p->method = method;
+  *pth = p;
+  return 0;
+}
+
+extern "C" int pth_free(pth_t * pth) { free(pth); return 0; }
+
+static fe_symm_t * as_symm(fe_t * fe) {
+  if (fe == nullptr || fe->id != LDW_FE_SYMMETRIC) return nullptr;
+  return (fe_symm_t *) fe;
+}
+
+/* phi_force.c:74-148, fluid-only branch */
+extern "C" int phi_force_calculation(pe_t *, cs_t *, lees_edw_t * le, wall_t *, pth_t * pth,
+				     fe_t * fe, map_t *, field_t * phi, hydro_t * hydro) {
+  if (hydro == nullptr) return 0;
+  if (pth == nullptr || pth->method == FE_FORCE_METHOD_NO_FORCE) return 0;
+  if (le && le->nplanes > 0) FAIL("phi_force_calculation: Lees-Edwards planes not supported");
+  fe_symm_t * fs = as_symm(fe);
+  if (!fs) FAIL("phi_force_calculation: only the symmetric free energy is built");
+  if (phi->prec != hydro->prec) FAIL("phi_force_calculation: precision mismatch");
+  field_grad_t * dphi = fs->dphi;
+  FeParams p = fe_params_of(fs, pth->method);
+  const Geom g = phi->geo;
+  int mode = hydro->force_uniform ? 0 : 1;
+  {
+    KTimer t("phi_force_mu");
+    if (phi->prec == 64) {
+      ldw::launch_phi_force_mu_f64(g, p, (const double *) phi->d_data[phi->cur],
+				   (double *) hydro->d_force, (double *) dphi->d_mu,
+				   mode, hydro->fzero, g_stream);
+    }
+    else {
+      ldw::launch_phi_force_mu_f32(g, p, (const float *) phi->d_data[phi->cur],
+				   (float *) hydro->d_force, (float *) dphi->d_mu,
+				   mode, hydro->fzero, g_stream);
+    }
+  }
+  CHECK_LAUNCH();
+  hydro->force_uniform = 0;
+  dphi->mu_version = phi->version;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cahn-Hilliard                                                            */
+/* ------------------------------------------------------------------------ */
+
+static int g_adv_order = 1;    /* advection.c:74 default */
+
+extern "C" int advection_order_set(const int order) {
+  if (order < 1 || order > 3) FAIL("advection_order_set: order %d not built (1, 2, 3 are)", order);
+  g_adv_order = order;
+  return 0;
+}
+
+extern "C" int advection_order(int * order) { *order = g_adv_order; return 0; }
+
+struct phi_ch_s {
+  phi_ch_info_t info;
+  pe_t * pe;
+  cs_t * cs;
+  lees_edw_t * le;
+};
+
+extern "C" int phi_ch_create(pe_t * pe, cs_t * cs, lees_edw_t * le, phi_ch_info_t * info,
+			     phi_ch_t ** pch) {
+  if (info && info->noise) FAIL("phi_ch_create: order parameter noise not supported");
+  if (info && info->conserve) FAIL("phi_ch_create: cahn_hilliard_options_conserve %d not supported", info->conserve);
+  phi_ch_t * p = (phi_ch_t *) calloc(1, sizeof(phi_ch_t));
+  if (info) p->info = *info;
+  p->pe = pe; p->cs = cs; p->le = le;
+  *pch = p;
+  return 0;
+}
+
+extern "C" int phi_ch_free(phi_ch_t * pch) { free(pch); return 0; }
+
+/* phi_cahn_hilliard.c:213-290 */
+extern "C" int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi, hydro_t * hydro,
+				 map_t *, noise_t * noise) {
+  if (noise && pch->info.noise) FAIL("phi_cahn_hilliard: noise not supported");
+  fe_symm_t * fs = as_symm(fe);
+  if (!fs) FAIL("phi_cahn_hilliard: only the symmetric free energy is built");
+  field_grad_t * dphi = fs->dphi;
+  FeParams p = fe_params_of(fs, FE_FORCE_METHOD_STRESS_DIVERGENCE);
+  const Geom g = phi->geo;
+
+  if (dphi->mu_version != phi->version) {
+    KTimer t("mu");
+    if (phi->prec == 64) ldw::launch_mu_f64(g, p, (const double *) phi->d_data[phi->cur], (double *) dphi->d_mu, g_stream);
+    else ldw::launch_mu_f32(g, p, (const float *) phi->d_data[phi->cur], (float *) dphi->d_mu, g_stream);
+    CHECK_LAUNCH();
+    dphi->mu_version = phi->version;
+  }
+
+  const void * u = nullptr;
+  if (hydro) {
+    if (hydro->u_uniform) {
+      if (hydro->uzero[0] != 0.0 || hydro->uzero[1] != 0.0 || hydro->uzero[2] != 0.0) {
+	if (hydro_materialise(hydro)) return -1;
+	u = hydro->d_u;
+      }
+      // uniform zero velocity: the advective flux is u_f phi = 0 exactly
+      // (sign of zero aside), the same as no hydrodynamics
+    }
+    else {
+      u = hydro->d_u;
+    }
+  }
+
+  {
+    KTimer t("cahn_hilliard");
+    if (phi->prec == 64) {
+      ldw::launch_cahn_hilliard_f64(g, p, (const double *) phi->d_data[phi->cur],
+				    (const double *) dphi->d_mu, (const double *) u,
+				    (double *) phi->d_data[1 - phi->cur], g_stream);
+    }
+    else {
+      ldw::launch_cahn_hilliard_f32(g, p, (const float *) phi->d_data[phi->cur],
+				    (const float *) dphi->d_mu, (const float *) u,
+				    (float *) phi->d_data[1 - phi->cur], g_stream);
+    }
+  }
+  CHECK_LAUNCH();
+  phi->cur = 1 - phi->cur;
+  phi->version++;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Collision and propagation                                                */
+/* ------------------------------------------------------------------------ */
+
+/* Relaxation rates as collision.c:1163-1246 and the per-site helpers
+ * (collision.c:1255-1420) with a fixed viscosity */
+static LbParams lb_params_of(lb_t * lb, const double fextra[3]) {
+  LbParams p;
+  memset(&p, 0, sizeof(p));
+  physics_t * phys = nullptr;
+  physics_ref(&phys);
+  const double cs2 = (1.0/3.0);
+  const double rho0 = phys->rho0;
+  const double eta = phys->eta_shear;
+  const double eta_nu = phys->eta_bulk;
+  const double rtau_shear = 1.0/(0.5 + eta/(rho0*cs2));
+  p.rtau_shear = rtau_shear;
+  p.relax = (int) lb->nrelax;
+  if (lb->nrelax == LB_RELAXATION_M10) {
+    p.rtau_bulk = 1.0/(0.5 + eta_nu/(rho0*cs2));
+    for (int m = 10; m < 19; m++) p.rtau_ghost[m] = 1.0;
+  }
+  else if (lb->nrelax == LB_RELAXATION_BGK) {
+    p.rtau_bulk = 1.0/(0.5 + eta/(rho0*cs2));
+    for (int m = 10; m < 19; m++) p.rtau_ghost[m] = rtau_shear;
+  }
+  else {
+    p.rtau_bulk = 1.0/(0.5 + eta_nu/(rho0*cs2));
+    double tau = eta/(rho0*cs2);
+    double rg = 0.5 + 2.0*tau/(tau + 3.0/8.0);
+    if (rg > 2.0) rg = 2.0;
+    p.rtau_ghost[10] = rtau_shear; p.rtau_ghost[14] = rtau_shear; p.rtau_ghost[18] = rtau_shear;
+    p.rtau_ghost[11] = rg; p.rtau_ghost[12] = rg; p.rtau_ghost[13] = rg;
+    p.rtau_ghost[15] = rg; p.rtau_ghost[16] = rg; p.rtau_ghost[17] = rg;
+  }
+  // collision.c:1929-1940: force_global = fbody + pulsatile (zero); the
+  // site force is force_global + hydro force
+  for (int a = 0; a < 3; a++) {
+    p.fglobal[a] = phys->fbody[a];
+    if (fextra) p.fglobal[a] = p.fglobal[a] + fextra[a];
+  }
+  p.rho0 = rho0;
+  return p;
+}
+
+/* collision.c:143-162 */
+extern "C" int lb_collide(lb_t * lb, hydro_t * hydro, map_t *, noise_t * noise, fe_t * fe,
+			  visc_t * visc) {
+  if (hydro == nullptr) return 0;
+  if (noise) FAIL("lb_collide: lattice fluctuations are not supported");
+  if (visc) FAIL("lb_collide: viscosity models are not supported");
+  if (fe && fe->use_stress_relaxation) FAIL("lb_collide: stress relaxation force method is not supported");
+  if (hydro->prec != lb->prec) FAIL("lb_collide: precision mismatch between lb and hydro");
+
+  const Geom g = lb->geo;
+  const void * force = nullptr;
+  LbParams p;
+  if (hydro->force_uniform) {
+    p = lb_params_of(lb, hydro->fzero);
+  }
+  else {
+    p = lb_params_of(lb, nullptr);
+    force = hydro->d_force;
+  }
+  {
+    KTimer t("collide");
+    if (lb->prec == 64) {
+      ldw::launch_collide_f64(g, p, (const double *) lb->d_g[lb->cur], (double *) lb->d_g[1 - lb->cur],
+			      (const double *) force, (double *) hydro->d_rho,
+			      (double *) hydro->d_u, lb->streamed, g_stream);
+    }
+    else {
+      ldw::launch_collide_f32(g, p, (const float *) lb->d_g[lb->cur], (float *) lb->d_g[1 - lb->cur],
+			      (const float *) force, (float *) hydro->d_rho,
+			      (float *) hydro->d_u, lb->streamed, g_stream);
+    }
+  }
+  CHECK_LAUNCH();
+  lb->cur = 1 - lb->cur;
+  lb->streamed = 0;
+  hydro->u_uniform = 0;
+  return 0;
+}
+
+/* propagation.c:43-105: the streaming is deferred into the next collision
+ * (pull); two propagations in a row materialise the first. */
+extern "C" int lb_propagation(lb_t * lb) {
+  if (lb->streamed) {
+    KTimer t("propagate");
+    if (lb->prec == 64) ldw::launch_propagate_f64(lb->geo, (const double *) lb->d_g[lb->cur], (double *) lb->d_g[1 - lb->cur], g_stream);
+    else ldw::launch_propagate_f32(lb->geo, (const float *) lb->d_g[lb->cur], (float *) lb->d_g[1 - lb->cur], g_stream);
+    CHECK_LAUNCH();
+    lb->cur = 1 - lb->cur;
+  }
+  lb->streamed = 1;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Additions                                                                */
+/* ------------------------------------------------------------------------ */
+
+extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_grad_t * dphi,
+			       fe_t * fe, pth_t * pth, phi_ch_t * pch, int nsteps) {
+  const double fzero[3] = {0.0, 0.0, 0.0};
+  const double uzero[3] = {0.0, 0.0, 0.0};
+  for (int step = 0; step < nsteps; step++) {
+    if (hydro_f_zero(hydro, fzero)) return -1;
+    if (field_halo(phi)) return -1;
+    if (field_grad_compute(dphi)) return -1;
+    if (phi_force_calculation(nullptr, nullptr, nullptr, nullptr, pth, fe, nullptr, phi, hydro)) return -1;
+    if (phi_cahn_hilliard(pch, fe, phi, hydro, nullptr, nullptr)) return -1;
+    if (hydro_u_zero(hydro, uzero)) return -1;
+    if (lb_collide(lb, hydro, nullptr, nullptr, fe, nullptr)) return -1;
+    if (lb_halo(lb)) return -1;
+    if (lb_propagation(lb)) return -1;
+  }
+  return 0;
+}
+
+extern "C" int ldw_step_single(lb_t * lb, hydro_t * hydro, int nsteps) {
+  const double fzero[3] = {0.0, 0.0, 0.0};
+  const double uzero[3] = {0.0, 0.0, 0.0};
+  for (int step = 0; step < nsteps; step++) {
+    if (hydro_f_zero(hydro, fzero)) return -1;
+    if (hydro_u_zero(hydro, uzero)) return -1;
+    if (lb_collide(lb, hydro, nullptr, nullptr, nullptr, nullptr)) return -1;
+    if (lb_propagation(lb)) return -1;
+  }
+  return 0;
+}
+
+extern "C" int ldw_conserved_sums(lb_t * lb, field_t * phi, double out[2]) {
+  double h[3];
+  const Geom g = lb->geo;
+  if (lb->prec == 64) {
+    ldw::launch_klein_sum_f64((const double *) lb->d_g[lb->cur], 19*g.nsite, lb->d_work, lb->d_out, g_stream);
+  }
+  else {
+    ldw::launch_klein_sum_planes_f32((const float *) lb->d_g[lb->cur], g.nsite, 19, lb->d_work, lb->d_out, g_stream);
+  }
+  CHECK_LAUNCH();
+  HIP_CHECK(hipMemcpyAsync(h, lb->d_out, 3*sizeof(double), hipMemcpyDeviceToHost, g_stream));
+  HIP_CHECK(hipStreamSynchronize(g_stream));
+  out[0] = h[0];
+  if (lb->prec == 32) out[0] = (h[1] + lb_rho0()*(double) g.nsite) + h[2];
+  out[1] = 0.0;
+  if (phi) {
+    if (phi->prec == 64) ldw::launch_klein_sum_f64((const double *) phi->d_data[phi->cur], g.nsite, lb->d_work, lb->d_out, g_stream);
+    else ldw::launch_klein_sum_f32((const float *) phi->d_data[phi->cur], g.nsite, lb->d_work, lb->d_out, g_stream);
+    CHECK_LAUNCH();
+    HIP_CHECK(hipMemcpyAsync(h, lb->d_out, 3*sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIP_CHECK(hipStreamSynchronize(g_stream));
+    out[1] = h[0];
+  }
+  return 0;
+}
+
+extern "C" int ldw_lb_device_f(lb_t * lb, void ** ptr, int * elsize, int * shifted) {
+  *ptr = lb->d_g[lb->cur];
+  *elsize = (lb->prec == 64) ? 8 : 4;
+  *shifted = lb->streamed;
+  return 0;
+}
+
+extern "C" int ldw_field_device(field_t * f, void ** ptr, int * elsize) {
+  *ptr = f->d_data[f->cur];
+  *elsize = (f->prec == 64) ? 8 : 4;
+  return 0;
+}
+
+extern "C" int ldw_hydro_device(hydro_t * h, void ** rho, void ** u, void ** force, int * elsize) {
+  if (hydro_materialise(h)) return -1;
+  *rho = h->d_rho; *u = h->d_u; *force = h->d_force;
+  *elsize = (h->prec == 64) ? 8 : 4;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Initial conditions and bulk copies                                       */
+/* ------------------------------------------------------------------------ */
+
+/* ns_uniform (noise.h:96-109) */
+static inline unsigned int ns_uniform(unsigned int s[4]) {
+  unsigned int b;
+  s[0] = 69069u*s[0] + 1234567u;
+  b = s[1] ^ (s[1] << 17);
+  b ^= (b >> 13);
+  s[1] = b ^ (b << 5);
+  s[2] = 36969u*(s[2] & 0xffffu) + (s[2] >> 16);
+  s[3] = 18000u*(s[3] & 0xffffu) + (s[3] >> 16);
+  b = (s[2] << 16) + s[3];
+  return (s[1] + (s[0] ^ b));
+}
+
+/* per-site state for global site (ig, jg, kg), 1-based (noise.c:278-341);
+ * seeds <= 1 fall back to the default 13 (noise.c:87-94) */
+static void site_state(int seed, int ig, int jg, int kg, unsigned int s[4]) {
+  unsigned int sd = (seed > 1) ? (unsigned int) seed : 13u;
+  unsigned int local[4] = {sd + (unsigned int) ig, 12953u + (unsigned int) jg,
+			   712357u + (unsigned int) kg, 22383979u};
+  s[0] = ns_uniform(local);
+  s[1] = ns_uniform(local);
+  s[2] = ns_uniform(local);
+  s[3] = ns_uniform(local);
+}
+
+extern "C" int field_phi_init_spinodal(field_t * phi, int seed, double phi0, double amp) {
+  const Geom g = phi->geo;
+  for (int ic = 1; ic <= g.nx; ic++)
+    for (int jc = 1; jc <= g.ny; jc++)
+      for (int kc = 1; kc <= g.nz; kc++) {
+	unsigned int s[4];
+	site_state(seed, ic, jc, kc, s);
+	double ran = (1.0/4294967295u)*ns_uniform(s);
+	phi->data[cs_index(phi->cs, ic, jc, kc)] = phi0 + amp*(ran - 0.5);
+      }
+  return field_memcpy(phi, tdpMemcpyHostToDevice);
+}
+
+extern "C" int ldw_lb_init_random_velocity(lb_t * lb, int seed, double amp) {
+  const Geom g = lb->geo;
+  std::vector<double> u(3*g.nsite);
+  for (int ic = 1; ic <= g.nx; ic++)
+    for (int jc = 1; jc <= g.ny; jc++)
+      for (int kc = 1; kc <= g.nz; kc++) {
+	unsigned int s[4];
+	site_state(seed, ic, jc, kc, s);
+	int64_t is = dsite(g, ic, jc, kc);
+	for (int a = 0; a < 3; a++) {
+	  double r = (1.0/4294967295u)*ns_uniform(s);
+	  u[a*g.nsite + is] = amp*(2.0*r - 1.0);
+	}
+      }
+  for (int a = 0; a < 3; a++) {
+    double mean = 0.0;
+    for (int64_t is = 0; is < g.nsite; is++) mean += u[a*g.nsite + is];
+    mean = mean/g.nsite;
+    for (int64_t is = 0; is < g.nsite; is++) u[a*g.nsite + is] -= mean;
+  }
+  for (int ic = 1; ic <= g.nx; ic++)
+    for (int jc = 1; jc <= g.ny; jc++)
+      for (int kc = 1; kc <= g.nz; kc++) {
+	int64_t is = dsite(g, ic, jc, kc);
+	double u0[3] = {u[is], u[g.nsite + is], u[2*g.nsite + is]};
+	lb_1st_moment_equilib_set(lb, cs_index(lb->cs, ic, jc, kc), 1.0, u0);
+      }
+  return lb_memcpy(lb, tdpMemcpyHostToDevice);
+}
+
+extern "C" int ldw_lb_f_read(lb_t * lb, double * f) {
+  const Geom g = lb->geo;
+  const size_t n = (size_t) 19*g.nsite;
+  const double rho0 = lb_rho0();
+  std::vector<double> gd(n);
+  if (lb->prec == 64) {
+    HIP_CHECK(hipMemcpyAsync(gd.data(), lb->d_g[lb->cur], n*8, hipMemcpyDeviceToHost, g_stream));
+    HIP_CHECK(hipStreamSynchronize(g_stream));
+  }
+  else {
+    std::vector<float> gf(n);
+    HIP_CHECK(hipMemcpyAsync(gf.data(), lb->d_g[lb->cur], n*4, hipMemcpyDeviceToHost, g_stream));
+    HIP_CHECK(hipStreamSynchronize(g_stream));
+    for (int p = 0; p < 19; p++)
+      for (int64_t s = 0; s < g.nsite; s++) gd[p*g.nsite + s] = (double) gf[p*g.nsite + s] + ldw::weight(p)*rho0;
+  }
+  if (!lb->streamed) {
+    memcpy(f, gd.data(), n*sizeof(double));
+    return 0;
+  }
+  for (int p = 0; p < 19; p++) {
+    const int cx = ldw::CV.v[p][0], cy = ldw::CV.v[p][1], cz = ldw::CV.v[p][2];
+    for (int ic = 1; ic <= g.nx; ic++)
+      for (int jc = 1; jc <= g.ny; jc++)
+	for (int kc = 1; kc <= g.nz; kc++)
+	  f[p*g.nsite + dsite(g, ic, jc, kc)]
+	    = gd[p*g.nsite + dsite(g, wrap1(ic - cx, g.nx), wrap1(jc - cy, g.ny), wrap1(kc - cz, g.nz))];
+  }
+  return 0;
+}
+
+template <typename T>
+static int upload(void * dst, const double * src, int64_t n, double shift_per_plane_rho0,
+		  int64_t plane) {
+  std::vector<T> buf(n);
+  for (int64_t i = 0; i < n; i++) {
+    double v = src[i];
+    if (shift_per_plane_rho0 != 0.0) v -= ldw::weight((int) (i/plane))*shift_per_plane_rho0;
+    buf[i] = (T) v;
+  }
+  HIP_CHECK(hipMemcpyAsync(dst, buf.data(), n*sizeof(T), hipMemcpyHostToDevice, g_stream));
+  HIP_CHECK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+template <typename T>
+static int download(double * dst, const void * src, int64_t n) {
+  std::vector<T> buf(n);
+  HIP_CHECK(hipMemcpyAsync(buf.data(), src, n*sizeof(T), hipMemcpyDeviceToHost, g_stream));
+  HIP_CHECK(hipStreamSynchronize(g_stream));
+  for (int64_t i = 0; i < n; i++) dst[i] = (double) buf[i];
+  return 0;
+}
+
+extern "C" int ldw_lb_f_write(lb_t * lb, const double * f) {
+  const Geom g = lb->geo;
+  int rc = (lb->prec == 64)
+    ? upload<double>(lb->d_g[lb->cur], f, 19*g.nsite, 0.0, g.nsite)
+    : upload<float>(lb->d_g[lb->cur], f, 19*g.nsite, lb_rho0(), g.nsite);
+  lb->streamed = 0;
+  return rc;
+}
+
+extern "C" int ldw_field_read(field_t * obj, double * data) {
+  return (obj->prec == 64) ? download<double>(data, obj->d_data[obj->cur], obj->geo.nsite)
+    : download<float>(data, obj->d_data[obj->cur], obj->geo.nsite);
+}
+
+extern "C" int ldw_field_write(field_t * obj, const double * data) {
+  int rc = (obj->prec == 64) ? upload<double>(obj->d_data[obj->cur], data, obj->geo.nsite, 0.0, 1)
+    : upload<float>(obj->d_data[obj->cur], data, obj->geo.nsite, 0.0, 1);
+  obj->version++;
+  return rc;
+}
+
+extern "C" int ldw_hydro_read(hydro_t * h, double * rho, double * u, double * force) {
+  if (hydro_materialise(h)) return -1;
+  const int64_t n = h->geo.nsite;
+  int rc = 0;
+  if (h->prec == 64) {
+    if (rho) rc |= download<double>(rho, h->d_rho, n);
+    if (u) rc |= download<double>(u, h->d_u, 3*n);
+    if (force) rc |= download<double>(force, h->d_force, 3*n);
+  }
+  else {
+    if (rho) rc |= download<float>(rho, h->d_rho, n);
+    if (u) rc |= download<float>(u, h->d_u, 3*n);
+    if (force) rc |= download<float>(force, h->d_force, 3*n);
+  }
+  return rc;
+}
+
+extern "C" int ldw_hydro_write(hydro_t * h, const double * rho, const double * u,
+			       const double * force) {
+  if (hydro_materialise(h)) return -1;
+  const int64_t n = h->geo.nsite;
+  int rc = 0;
+  if (h->prec == 64) {
+    if (rho) rc |= upload<double>(h->d_rho, rho, n, 0.0, 1);
+    if (u) rc |= upload<double>(h->d_u, u, 3*n, 0.0, 1);
+    if (force) rc |= upload<double>(h->d_force, force, 3*n, 0.0, 1);
+  }
+  else {
+    if (rho) rc |= upload<float>(h->d_rho, rho, n, 0.0, 1);
+    if (u) rc |= upload<float>(h->d_u, u, 3*n, 0.0, 1);
+    if (force) rc |= upload<float>(h->d_force, force, 3*n, 0.0, 1);
+  }
+  return rc;
+}

@@ -1,0 +1,117 @@
+// reduce.hip -- compensated FP64 sums on the device (conservation of rho
+// and phi).  The reference accumulates Kahan or Klein sums per site
+// (util_sum.c:30-110, cahn_hilliard_stats.c:225-270); here each thread
+// keeps a Neumaier (sum, compensation) pair over a grid-stride range and
+// pairs are merged with the error-free TwoSum, first within a block and
+// then across blocks, so the result is the exactly rounded sum to within a
+// few ulp regardless of the order.
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace ldw {
+
+namespace {
+
+constexpr int RB = 256;          // threads per block
+constexpr int RMAXB = 1024;      // maximum blocks in stage one
+
+struct DD { double s, c; };
+
+__device__ __forceinline__ DD add_value(DD a, double x) {
+  double t = a.s + x;
+  if (fabs(a.s) >= fabs(x)) a.c += (a.s - t) + x;
+  else a.c += (x - t) + a.s;
+  a.s = t;
+  return a;
+}
+
+__device__ __forceinline__ DD merge(DD a, DD b) {
+  double s = a.s + b.s;
+  double bp = s - a.s;
+  double e = (a.s - (s - bp)) + (b.s - bp);
+  DD r = {s, (a.c + b.c) + e};
+  return r;
+}
+
+__device__ DD block_merge(DD v) {
+  __shared__ double ss[RB], cc[RB];
+  ss[threadIdx.x] = v.s; cc[threadIdx.x] = v.c;
+  __syncthreads();
+  for (int w = RB/2; w > 0; w >>= 1) {
+    if ((int) threadIdx.x < w) {
+      DD a = {ss[threadIdx.x], cc[threadIdx.x]};
+      DD b = {ss[threadIdx.x + w], cc[threadIdx.x + w]};
+      DD r = merge(a, b);
+      ss[threadIdx.x] = r.s; cc[threadIdx.x] = r.c;
+    }
+    __syncthreads();
+  }
+  DD r = {ss[0], cc[0]};
+  return r;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RB)
+sum_stage1(const T * __restrict__ a, int64_t n, int nplane, int64_t nsite,
+	   double * __restrict__ work) {
+  DD acc = {0.0, 0.0};
+  const int64_t ntot = (nplane > 0) ? (int64_t) nplane*nsite : n;
+  for (int64_t i = (int64_t) blockIdx.x*RB + threadIdx.x; i < ntot;
+       i += (int64_t) gridDim.x*RB) {
+    acc = add_value(acc, (double) a[i]);
+  }
+  DD r = block_merge(acc);
+  if (threadIdx.x == 0) {
+    work[2*blockIdx.x] = r.s;
+    work[2*blockIdx.x + 1] = r.c;
+  }
+}
+
+__global__ void __launch_bounds__(RB)
+sum_stage2(const double * __restrict__ work, int nb, double * __restrict__ out) {
+  DD acc = {0.0, 0.0};
+  for (int i = threadIdx.x; i < nb; i += RB) {
+    DD b = {work[2*i], work[2*i + 1]};
+    acc = merge(acc, b);
+  }
+  DD r = block_merge(acc);
+  if (threadIdx.x == 0) {
+    out[0] = r.s + r.c;   // rounded sum
+    out[1] = r.s;         // leading part
+    out[2] = r.c;         // compensation
+  }
+}
+
+}  // namespace
+
+int sum_blocks(int64_t n) {
+  int64_t nb = (n + (int64_t) RB*8 - 1)/((int64_t) RB*8);
+  if (nb < 1) nb = 1;
+  if (nb > RMAXB) nb = RMAXB;
+  return (int) nb;
+}
+
+void launch_klein_sum_f64(const double * a, int64_t n, double * work,
+			  double * out, hipStream_t s) {
+  int nb = sum_blocks(n);
+  hipLaunchKernelGGL(sum_stage1<double>, dim3(nb), dim3(RB), 0, s, a, n, 0, (int64_t) 0, work);
+  hipLaunchKernelGGL(sum_stage2, dim3(1), dim3(RB), 0, s, work, nb, out);
+}
+
+void launch_klein_sum_f32(const float * a, int64_t n, double * work,
+			  double * out, hipStream_t s) {
+  int nb = sum_blocks(n);
+  hipLaunchKernelGGL(sum_stage1<float>, dim3(nb), dim3(RB), 0, s, a, n, 0, (int64_t) 0, work);
+  hipLaunchKernelGGL(sum_stage2, dim3(1), dim3(RB), 0, s, work, nb, out);
+}
+
+void launch_klein_sum_planes_f32(const float * g, int64_t nsite, int nplane,
+				 double * work, double * out, hipStream_t s) {
+  int nb = sum_blocks((int64_t) nplane*nsite);
+  hipLaunchKernelGGL(sum_stage1<float>, dim3(nb), dim3(RB), 0, s, g, (int64_t) 0, nplane, nsite, work);
+  hipLaunchKernelGGL(sum_stage2, dim3(1), dim3(RB), 0, s, work, nb, out);
+}
+
+}  // namespace ldw
