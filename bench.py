@@ -4,7 +4,8 @@
 Workload (a step = one full time step of the path over the whole lattice):
 symmetric free energy + finite-difference Cahn-Hilliard + D3Q19 MRT, the
 plain binary-fluid configuration: 128^3 periodic, FP64, A = -0.0625,
-B = 0.0625, kappa = 0.04, M = 0.45, tau = 1, phi0 = 0 with +-0.05 spinodal
+B = 0.0625, kappa = 0.04, M = 0.25 (the stated 0.45 is unstable for the explicit
+Cahn-Hilliard step, in the reference too), tau = 1, phi0 = 0 with +-0.05 spinodal
 noise, rho = 1, u = 0.  Inputs are resident in HBM before the timed region.
 
 Prints one JSON line:
@@ -35,7 +36,9 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 BYTES_SITE = {64: {"collide": 360, "phi_force_mu": 40, "cahn_hilliard": 48, "step": 448},
               32: {"collide": 180, "phi_force_mu": 20, "cahn_hilliard": 24, "step": 224}}
-C2 = dict(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.45, tau=1.0, order=1)
+# C2 with M = 0.25: at the stated M = 0.45 the explicit Cahn-Hilliard step is
+# unstable (the reference reaches NaN by step ~30; tests/golden/binary32_m045_*)
+C2 = dict(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25, tau=1.0, order=1)
 SEED = 8361235
 
 
@@ -89,7 +92,7 @@ free_energy symmetric
 symmetric_a -0.0625
 symmetric_b 0.0625
 symmetric_kappa 0.04
-mobility 0.45
+mobility 0.25
 phi0 0.0
 phi_initialisation spinodal
 noise 0.1
@@ -179,7 +182,7 @@ def main():
         "data": "synthetic (seeded spinodal initial condition)",
         "config": {"workload": f"binary fluid D3Q19 MRT + symmetric FE + Cahn-Hilliard, "
                                f"{args.size}^3 periodic, FP64, tau=1, A=-0.0625, B=0.0625, "
-                               f"kappa=0.04, M=0.45, advection order 1",
+                               f"kappa=0.04, M=0.25 (stated 0.45 is unstable), advection order 1",
                    "lattice": [args.size] * 3, "precision": "fp64"},
         "hbm_frac_step": main_run["hbm_frac_step"],
         "kernel_ms": main_run["kernel_ms"],

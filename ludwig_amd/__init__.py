@@ -289,7 +289,7 @@ class BinaryFluid(_Base):
     """``free_energy symmetric`` with the finite-difference Cahn-Hilliard
     solver and the 7-point gradient (nhalo 2)."""
 
-    def __init__(self, shape, a=-0.0625, b=0.0625, kappa=0.04, mobility=0.45,
+    def __init__(self, shape, a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25,
                  order=1, force_method=FORCE_STRESS_DIVERGENCE, **kw):
         super().__init__(shape, 2, **kw)
         L = lib()

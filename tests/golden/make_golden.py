@@ -32,6 +32,7 @@ HARNESS = os.path.join(REPO, "oracle", "_ref", "ref_harness")
 
 # C2 parameters (symmetric free energy, Cahn-Hilliard)
 A, B, K, M = -0.0625, 0.0625, 0.04, 0.45
+MS = 0.25   # largest round mobility that is stable for |phi| <= 1.1 with A, B, K
 
 # name: (harness args, shape, arrays stored in full, arrays stored by hash+samples)
 CASES = {
@@ -52,8 +53,19 @@ CASES = {
                         ["phi", "rho", "u"], ["f"]),
     "binary16_o2_s5": (["binary", 16, 5, 77, 2, A, B, K, M, 0.8, 0.1, 0.2], 16,
                        ["phi", "rho", "u"], ["f"]),
-    "binary32_o1_s100": (["binary", 32, 100, 2024, 1, A, B, K, M, 1.0, 0.0, 0.1], 32,
+    # C2 physics with M = 0.25: the stated M = 0.45 is linearly unstable for
+    # the explicit Cahn-Hilliard step (M k^2 (A + 3B phi^2 + kappa k^2) > 2 at
+    # the grid scale) and the reference itself reaches NaN by step ~30
+    "binary32_o1_s100": (["binary", 32, 100, 2024, 1, A, B, K, MS, 1.0, 0.0, 0.1], 32,
                          [], ["phi", "rho", "u", "f"]),
+    "binary32_o3_s200": (["binary", 32, 200, 31337, 3, A, B, K, MS, 1.0, 0.0, 0.1], 32,
+                         [], ["phi", "rho", "u", "f"]),
+    # the reference's own run at the stated M = 0.45: finite at step 20,
+    # non-finite by step 40
+    "binary32_m045_s20": (["binary", 32, 20, 2024, 1, A, B, K, M, 1.0, 0.0, 0.1], 32,
+                          [], ["phi", "rho", "u", "f"]),
+    "binary32_m045_s40": (["binary", 32, 40, 2024, 1, A, B, K, M, 1.0, 0.0, 0.1], 32,
+                          [], ["phi"]),
     # C1: plain D3Q19 MRT, 32^3, tau 0.8, 100 steps, random-velocity start
     "single32_s100": (["single", 32, 100, 12345, 0.8], 32, [], ["u_init", "f_init", "f", "rho", "u"]),
 }
@@ -85,6 +97,7 @@ def main():
                 out[key + "_sha256"] = np.array(sha(a))
                 out[key + "_samples"] = a.reshape(ncomp, nsite)[:, samples]
                 out[key + "_sum"] = np.array(a.reshape(ncomp, nsite).sum(axis=1))
+                out[key + "_finite"] = np.array(bool(np.isfinite(a).all()))
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
         print("wrote", name)
 

@@ -118,8 +118,9 @@ def test_binary_step_golden(golden, case):
     b.close()
 
 
-def test_binary32_100_steps_golden(golden):
-    g = golden("binary32_o1_s100")
+@pytest.mark.parametrize("case", ["binary32_o1_s100", "binary32_o3_s200", "binary32_m045_s20"])
+def test_binary32_long_golden(golden, case):
+    g = golden(case)
     n, nsteps, seed, order, a, b_, k, m, tau, phi0, amp = _binary_args(g)
     b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, order=order, tau=tau)
     b.init_spinodal(seed, phi0, amp)
@@ -186,10 +187,21 @@ def test_staged_calls_equal_fused_step():
     b2.close()
 
 
+def test_stated_c2_mobility_blows_up_like_the_reference(golden):
+    g = golden("binary32_m045_s40")
+    n, nsteps, seed, order, a, b_, k, m, tau, phi0, amp = _binary_args(g)
+    assert m == 0.45 and not bool(g["phi_finite"])
+    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, order=order, tau=tau)
+    b.init_spinodal(seed, phi0, amp)
+    b.step(nsteps)
+    assert not np.isfinite(b.phi_field()).all()
+    b.close()
+
+
 def test_conservation_128_fp64():
-    """C2 physics at 128^3: total density and phi conserved to round-off over
-    200 steps (compensated device sums)."""
-    b = L.BinaryFluid((128, 128, 128), tau=1.0)
+    """C2 physics (M = 0.25) at 128^3: total density and phi conserved to
+    round-off over 200 steps (compensated device sums)."""
+    b = L.BinaryFluid((128, 128, 128), tau=1.0, mobility=0.25)
     b.init_spinodal(8361235)
     r0, p0 = b.conserved_sums()
     phi = b.phi_field()

@@ -101,12 +101,24 @@ def test_binary_steps_full_fields(golden, case):
     assert sha(f1) == str(g["f_sha256"])
 
 
-def test_binary32_100_steps(golden):
-    g = golden("binary32_o1_s100")
+@pytest.mark.parametrize("case", ["binary32_o1_s100", "binary32_o3_s200", "binary32_m045_s20"])
+def test_binary32_long(golden, case):
+    g = golden(case)
     s, nsteps, lb, fe, phi, f = _binary(g)
     f1, phi1, rho1, u1 = O.binary_steps(s, nsteps, lb, fe, f, phi)
     for key, a in (("phi", phi1), ("rho", rho1), ("u", u1), ("f", f1)):
         assert sha(a) == str(g[key + "_sha256"]), key
+
+
+def test_stated_c2_mobility_is_unstable_in_the_reference(golden):
+    """At M = 0.45 (with A = -0.0625, B = 0.0625, kappa = 0.04) the explicit
+    Cahn-Hilliard step amplifies the grid-scale mode: the reference's own run
+    is non-finite by step 40, and so is the oracle's."""
+    g = golden("binary32_m045_s40")
+    assert not bool(g["phi_finite"])
+    s, nsteps, lb, fe, phi, f = _binary(g)
+    _, phi1, _, _ = O.binary_steps(s, nsteps, lb, fe, f, phi)
+    assert not np.isfinite(phi1).all()
 
 
 def test_single_fluid_c1(golden):
