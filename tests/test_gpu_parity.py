@@ -67,7 +67,9 @@ def test_propagation_golden(golden):
 def test_force_golden(golden, case, method):
     g = golden(case)
     n = int(g["n"])
-    b = L.BinaryFluid((n, n, n), tau=1.0, force_method=method)
+    args = [str(a) for a in g["args"]]
+    a, b_, k = map(float, args[4:7])
+    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, tau=1.0, force_method=method)
     b.set_phi(g["phi"])
     b.f_zero()
     b.force_stage()
@@ -80,7 +82,9 @@ def test_force_golden(golden, case, method):
 def test_cahn_hilliard_golden(golden, order):
     g = golden(f"ch_order{order}")
     n = int(g["n"])
-    b = L.BinaryFluid((n, n, n), tau=1.0, order=order)
+    args = [str(a) for a in g["args"]]
+    a, b_, k, m = map(float, args[4:8])
+    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, tau=1.0, order=order)
     b.set_phi(g["phi_in"])
     b.set_hydro(u=g["u"])
     b.cahn_hilliard_stage()
@@ -150,12 +154,12 @@ def test_single_fluid_c1_golden(golden):
                                                 ((33, 17, 9), 2, 3)])
 def test_binary_nonuniform_box_vs_oracle(shape, order, nsteps):
     """Non-cubic, odd sizes: the periodic wrap in every kernel."""
-    fe = O.fe_param(order=order)
+    fe = O.fe_param(order=order, mobility=0.25)
     lb = O.lb_param(tau=0.9)
     phi0 = O.spinodal(shape, 99, 0.05, 0.1)
     f0 = O.equilibrium(shape, rho_uniform=1.0)
     f1, phi1, rho1, u1 = O.binary_steps(shape, nsteps, lb, fe, f0, phi0)
-    b = L.BinaryFluid(shape, order=order, tau=0.9)
+    b = L.BinaryFluid(shape, order=order, tau=0.9, mobility=0.25)
     b.init_spinodal(99, 0.05, 0.1)
     b.step(nsteps)
     assert_exact(b.phi_field(), phi1, "phi")
