@@ -34,8 +34,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_SITE = {64: {"collide": 360, "phi_force_mu": 40, "cahn_hilliard": 48, "step": 448},
-              32: {"collide": 180, "phi_force_mu": 20, "cahn_hilliard": 24, "step": 224}}
+BYTES_SITE = {64: {"collide": 360, "phi_force_mu": 40, "cahn_hilliard": 48, "force_ch": 64, "step": 448},
+              32: {"collide": 180, "phi_force_mu": 20, "cahn_hilliard": 24, "force_ch": 32, "step": 224}}
 # C2 with M = 0.25: at the stated M = 0.45 the explicit Cahn-Hilliard step is
 # unstable (the reference reaches NaN by step ~30; tests/golden/binary32_m045_*)
 C2 = dict(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25, tau=1.0, order=1)
@@ -62,7 +62,7 @@ def run_gpu(n, steps, warmup, precision, timing_steps):
     lib.ldw_timer_enable(0)
     kt = {}
     import ctypes
-    for name in ("collide", "phi_force_mu", "cahn_hilliard", "mu", "propagate"):
+    for name in ("collide", "force_ch", "phi_force_mu", "cahn_hilliard", "mu", "propagate"):
         ms = ctypes.c_double()
         cnt = ctypes.c_int()
         lib.ldw_timer_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))

@@ -87,6 +87,15 @@ void launch_cahn_hilliard_f32(const Geom & g, const FeParams & fe,
 			      const float * u, float * phi_out,
 			      hipStream_t s);
 
+// K23: force (as K2) and the Cahn-Hilliard update (as K3) in one pass,
+// 2.5D-tiled through LDS.  Does not write mu.
+void launch_force_ch_f64(const Geom & g, const FeParams & fe, const double * phi,
+			 const double * u, double * force, double * phi_out,
+			 int force_mode, const double fzero[3], hipStream_t s);
+void launch_force_ch_f32(const Geom & g, const FeParams & fe, const float * phi,
+			 const float * u, float * force, float * phi_out,
+			 int force_mode, const double fzero[3], hipStream_t s);
+
 // Fill n doubles/floats with a value
 void launch_fill_f64(double * a, int64_t n, double v, hipStream_t s);
 void launch_fill_f32(float * a, int64_t n, float v, hipStream_t s);

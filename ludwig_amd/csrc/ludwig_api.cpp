@@ -83,6 +83,7 @@ extern "C" const char * ldw_last_error(void) { return g_errmsg; }
 
 static hipStream_t g_stream = nullptr;
 static int g_ready = 0;
+static int flush_force(void);
 static int g_precision = 0;
 
 static int device_init(void) {
@@ -115,6 +116,7 @@ extern "C" int ldw_precision(void) { return precision_now(); }
 
 extern "C" int ldw_device_synchronize(void) {
   if (!g_ready) return 0;
+  if (flush_force()) return -1;
   HIP_CHECK(hipStreamSynchronize(g_stream));
   return 0;
 }
@@ -611,6 +613,8 @@ extern "C" int wall_free(wall_t * wall) { free(wall); return 0; }
 /* field_t, field_grad_t                                                    */
 /* ------------------------------------------------------------------------ */
 
+static void pending_drop_field(field_t * f);
+
 struct field_s {
   pe_t * pe;
   cs_t * cs;
@@ -669,6 +673,7 @@ extern "C" int field_create(pe_t * pe, cs_t * cs, lees_edw_t *, const char * nam
 
 extern "C" int field_free(field_t * f) {
   if (!f) return 0;
+  pending_drop_field(f);
   if (g_ready) (void) hipStreamSynchronize(g_stream);
   for (int b = 0; b < 2; b++) (void) hipFree(f->d_data[b]);
   free(f->data);
@@ -689,6 +694,7 @@ extern "C" int field_scalar_set(field_t * f, int index, double phi) {
 }
 
 extern "C" int field_memcpy(field_t * f, tdpMemcpyKind flag) {
+  if (flag == tdpMemcpyHostToDevice && flush_force()) return -1;
   const Geom g = f->geo;
   if (flag == tdpMemcpyHostToDevice) {
     if (f->prec == 64) {
@@ -809,7 +815,23 @@ struct hydro_s {
   int u_uniform;                   // 1: u is uniform uzero
   double uzero[3];
   Geom geo;
+  // deferred phi_force_calculation (launched fused with the next
+  // phi_cahn_hilliard on the same phi, or alone when the force is needed)
+  int fp_pending;
+  field_t * fp_phi;
+  long fp_version;
+  FeParams fp_params;
+  field_grad_t * fp_dphi;
 };
+
+static hydro_t * g_pending = nullptr;   // at most one deferred force
+
+static void pending_drop_field(field_t * f) {
+  if (g_pending && g_pending->fp_phi == f) {
+    g_pending->fp_pending = 0;
+    g_pending = nullptr;
+  }
+}
 
 extern "C" hydro_options_t hydro_options_default(void) { hydro_options_t o = {1}; return o; }
 
@@ -840,6 +862,7 @@ extern "C" int hydro_create(pe_t * pe, cs_t * cs, lees_edw_t *, const hydro_opti
 
 extern "C" int hydro_free(hydro_t * h) {
   if (!h) return 0;
+  if (g_pending == h) g_pending = nullptr;
   if (g_ready) (void) hipStreamSynchronize(g_stream);
   (void) hipFree(h->d_rho); (void) hipFree(h->d_u); (void) hipFree(h->d_force);
   free(h->rho); free(h->u); free(h->force);
@@ -848,6 +871,7 @@ extern "C" int hydro_free(hydro_t * h) {
 }
 
 static int hydro_materialise(hydro_t * h) {
+  if (flush_force()) return -1;
   const Geom g = h->geo;
   if (h->force_uniform) {
     for (int a = 0; a < 3; a++) {
@@ -869,6 +893,7 @@ static int hydro_materialise(hydro_t * h) {
 }
 
 extern "C" int hydro_memcpy(hydro_t * h, tdpMemcpyKind flag) {
+  if (flush_force()) return -1;
   const Geom g = h->geo;
   if (flag == tdpMemcpyHostToDevice) {
     if (h->prec == 64) {
@@ -1077,19 +1102,36 @@ extern "C" int phi_force_calculation(pe_t *, cs_t *, lees_edw_t * le, wall_t *, 
   fe_symm_t * fs = as_symm(fe);
   if (!fs) FAIL("phi_force_calculation: only the symmetric free energy is built");
   if (phi->prec != hydro->prec) FAIL("phi_force_calculation: precision mismatch");
-  field_grad_t * dphi = fs->dphi;
-  FeParams p = fe_params_of(fs, pth->method);
+  if (flush_force()) return -1;
+  hydro->fp_pending = 1;
+  hydro->fp_phi = phi;
+  hydro->fp_version = phi->version;
+  hydro->fp_params = fe_params_of(fs, pth->method);
+  hydro->fp_dphi = fs->dphi;
+  g_pending = hydro;
+  return 0;
+}
+
+/* Launch a deferred force calculation on its own (K2), writing F and mu */
+static int flush_force(void) {
+  hydro_t * hydro = g_pending;
+  if (hydro == nullptr) return 0;
+  g_pending = nullptr;
+  hydro->fp_pending = 0;
+  field_t * phi = hydro->fp_phi;
+  field_grad_t * dphi = hydro->fp_dphi;
+  if (phi->version != hydro->fp_version) FAIL("deferred force: phi changed before the force was formed");
   const Geom g = phi->geo;
-  int mode = hydro->force_uniform ? 0 : 1;
+  const int mode = hydro->force_uniform ? 0 : 1;
   {
     KTimer t("phi_force_mu");
     if (phi->prec == 64) {
-      ldw::launch_phi_force_mu_f64(g, p, (const double *) phi->d_data[phi->cur],
+      ldw::launch_phi_force_mu_f64(g, hydro->fp_params, (const double *) phi->d_data[phi->cur],
 				   (double *) hydro->d_force, (double *) dphi->d_mu,
 				   mode, hydro->fzero, g_stream);
     }
     else {
-      ldw::launch_phi_force_mu_f32(g, p, (const float *) phi->d_data[phi->cur],
+      ldw::launch_phi_force_mu_f32(g, hydro->fp_params, (const float *) phi->d_data[phi->cur],
 				   (float *) hydro->d_force, (float *) dphi->d_mu,
 				   mode, hydro->fzero, g_stream);
     }
@@ -1143,6 +1185,53 @@ extern "C" int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi, hydro
   field_grad_t * dphi = fs->dphi;
   FeParams p = fe_params_of(fs, FE_FORCE_METHOD_STRESS_DIVERGENCE);
   const Geom g = phi->geo;
+
+  if (hydro && hydro->fp_pending && g_pending == hydro && hydro->fp_phi == phi
+      && hydro->fp_version == phi->version) {
+    // force and Cahn-Hilliard update in one pass (K23); a uniform zero
+    // velocity is passed as no velocity (advective flux exactly zero)
+    const void * uf = hydro->d_u;
+    if (hydro->u_uniform) {
+      if (hydro->uzero[0] == 0.0 && hydro->uzero[1] == 0.0 && hydro->uzero[2] == 0.0) {
+	uf = nullptr;
+      }
+      else {
+	// materialise u only (the pending force is launched fused below)
+	const Geom gg = hydro->geo;
+	for (int a = 0; a < 3; a++) {
+	  if (hydro->prec == 64) ldw::launch_fill_f64((double *) hydro->d_u + a*gg.nsite, gg.nsite, hydro->uzero[a], g_stream);
+	  else ldw::launch_fill_f32((float *) hydro->d_u + a*gg.nsite, gg.nsite, (float) hydro->uzero[a], g_stream);
+	}
+	CHECK_LAUNCH();
+	hydro->u_uniform = 0;
+      }
+    }
+    FeParams q = hydro->fp_params;
+    q.order = p.order;
+    q.mobility = p.mobility;
+    const int mode = hydro->force_uniform ? 0 : 1;
+    g_pending = nullptr;
+    hydro->fp_pending = 0;
+    {
+      KTimer t("force_ch");
+      if (phi->prec == 64) {
+	ldw::launch_force_ch_f64(g, q, (const double *) phi->d_data[phi->cur], (const double *) uf,
+				 (double *) hydro->d_force, (double *) phi->d_data[1 - phi->cur],
+				 mode, hydro->fzero, g_stream);
+      }
+      else {
+	ldw::launch_force_ch_f32(g, q, (const float *) phi->d_data[phi->cur], (const float *) uf,
+				 (float *) hydro->d_force, (float *) phi->d_data[1 - phi->cur],
+				 mode, hydro->fzero, g_stream);
+      }
+    }
+    CHECK_LAUNCH();
+    hydro->force_uniform = 0;
+    phi->cur = 1 - phi->cur;
+    phi->version++;
+    return 0;
+  }
+  if (flush_force()) return -1;
 
   if (dphi->mu_version != phi->version) {
     KTimer t("mu");
@@ -1239,6 +1328,7 @@ extern "C" int lb_collide(lb_t * lb, hydro_t * hydro, map_t *, noise_t * noise, 
   if (visc) FAIL("lb_collide: viscosity models are not supported");
   if (fe && fe->use_stress_relaxation) FAIL("lb_collide: stress relaxation force method is not supported");
   if (hydro->prec != lb->prec) FAIL("lb_collide: precision mismatch between lb and hydro");
+  if (flush_force()) return -1;
 
   const Geom g = lb->geo;
   const void * force = nullptr;
@@ -1505,6 +1595,7 @@ extern "C" int ldw_field_read(field_t * obj, double * data) {
 }
 
 extern "C" int ldw_field_write(field_t * obj, const double * data) {
+  if (flush_force()) return -1;
   int rc = (obj->prec == 64) ? upload<double>(obj->d_data[obj->cur], data, obj->geo.nsite, 0.0, 1)
     : upload<float>(obj->d_data[obj->cur], data, obj->geo.nsite, 0.0, 1);
   obj->version++;
