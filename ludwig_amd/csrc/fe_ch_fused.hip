@@ -22,142 +22,13 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "fe_tile.h"
 
 namespace ldw {
 
 namespace {
 
-constexpr int TZ = 64;
-constexpr int TY = 4;
-constexpr int NT = TZ*TY;
-constexpr int PZ = TZ + 4, PY = TY + 4;      // phi tile with halo 2
-constexpr int QZ = TZ + 2, QY = TY + 2;      // Q tile with halo 1
-constexpr int PPLANE = PZ*PY;
-constexpr int QPLANE = QZ*QY;
-
-__device__ __forceinline__ int modn(int i, int n) {
-  int r = i % n;
-  return (r < 0) ? r + n : r;
-}
-
-template <typename T>
-struct Smem {
-  T phi[5][PPLANE];
-  T q[3][5][QPLANE];          // g0, g1, g2, p0, mu
-};
-
-template <typename T>
-__device__ __forceinline__ void load_phi_plane(Smem<T> & s, const T * __restrict__ phi,
-					       const Geom & g, int x, int y0, int z0) {
-  const int slot = modn(x, 5);
-  const int xw = modn(x, g.nx);
-  for (int i = threadIdx.x + threadIdx.y*TZ; i < PPLANE; i += NT) {
-    const int yy = i / PZ, zz = i - yy*PZ;
-    const int y = modn(y0 - 2 + yy, g.ny);
-    const int z = modn(z0 - 2 + zz, g.nz);
-    s.phi[slot][i] = phi[((int64_t) xw*g.ny + y)*g.nz + z];
-  }
-}
-
-// Software pipelining: the phi plane needed by the next iteration is
-// fetched into registers one iteration ahead, so its memory latency overlaps
-// the LDS compute of the current plane.
-constexpr int NPRE = (PPLANE + NT - 1)/NT;
-
-template <typename T>
-__device__ __forceinline__ void prefetch_phi_plane(T (&pre)[NPRE], const T * __restrict__ phi,
-						   const Geom & g, int x, int y0, int z0) {
-  const int xw = modn(x, g.nx);
-#pragma unroll
-  for (int k = 0; k < NPRE; k++) {
-    const int i = threadIdx.x + threadIdx.y*TZ + k*NT;
-    if (i < PPLANE) {
-      const int yy = i / PZ, zz = i - yy*PZ;
-      const int y = modn(y0 - 2 + yy, g.ny);
-      const int z = modn(z0 - 2 + zz, g.nz);
-      pre[k] = phi[((int64_t) xw*g.ny + y)*g.nz + z];
-    }
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ void store_phi_plane(Smem<T> & s, const T (&pre)[NPRE], int x) {
-  const int slot = modn(x, 5);
-#pragma unroll
-  for (int k = 0; k < NPRE; k++) {
-    const int i = threadIdx.x + threadIdx.y*TZ + k*NT;
-    if (i < PPLANE) s.phi[slot][i] = pre[k];
-  }
-}
-
-// velocity components the fluxes of site (x, y, z) need:
-// ux(x), ux(x-1), ux(x+1), uy(y), uy(y+1), uy(y-1), uz(z), uz(z+1), uz(z-1)
-template <typename T>
-__device__ __forceinline__ void load_u(T (&v)[9], const T * __restrict__ u, const Geom & g,
-				       int x, int y, int z) {
-  const int xm = (x == 0) ? g.nx - 1 : x - 1;
-  const int xp = (x == g.nx - 1) ? 0 : x + 1;
-  const int ym = (y == 0) ? g.ny - 1 : y - 1;
-  const int yp = (y == g.ny - 1) ? 0 : y + 1;
-  const int zm = (z == 0) ? g.nz - 1 : z - 1;
-  const int zp = (z == g.nz - 1) ? 0 : z + 1;
-  const int64_t nyz = (int64_t) g.ny*g.nz;
-  const int64_t rowx = (int64_t) x*nyz;
-  const int64_t site = rowx + (int64_t) y*g.nz + z;
-  const T * __restrict__ ux = u;
-  const T * __restrict__ uy = u + g.nsite;
-  const T * __restrict__ uz = u + 2*g.nsite;
-  v[0] = ux[site];
-  v[1] = ux[(int64_t) xm*nyz + (int64_t) y*g.nz + z];
-  v[2] = ux[(int64_t) xp*nyz + (int64_t) y*g.nz + z];
-  v[3] = uy[site];
-  v[4] = uy[rowx + (int64_t) yp*g.nz + z];
-  v[5] = uy[rowx + (int64_t) ym*g.nz + z];
-  v[6] = uz[site];
-  v[7] = uz[rowx + (int64_t) y*g.nz + zp];
-  v[8] = uz[rowx + (int64_t) y*g.nz + zm];
-}
-
-// Q on plane x at every point of the halo-1 tile
-template <typename T>
-__device__ __forceinline__ void compute_q_plane(Smem<T> & s, const FeParams & fe, int x) {
-  const int sm = modn(x - 1, 5), s0 = modn(x, 5), sp = modn(x + 1, 5);
-  const int qs = modn(x, 3);
-  const T a = T(fe.a), b = T(fe.b), kappa = T(fe.kappa);
-  for (int i = threadIdx.x + threadIdx.y*TZ; i < QPLANE; i += NT) {
-    const int yy = i / QZ, zz = i - yy*QZ;
-    const int c = (yy + 1)*PZ + (zz + 1);         // same point in the phi tile
-    const T phi = s.phi[s0][c];
-    const T xp = s.phi[sp][c], xm = s.phi[sm][c];
-    const T yp = s.phi[s0][c + PZ], ym = s.phi[s0][c - PZ];
-    const T zp = s.phi[s0][c + 1], zm = s.phi[s0][c - 1];
-    const T g0 = T(0.5)*(xp - xm);
-    const T g1 = T(0.5)*(yp - ym);
-    const T g2 = T(0.5)*(zp - zm);
-    const T delsq = xp + xm + yp + ym + zp + zm - T(6.0)*phi;
-    const T p0 = T(0.5)*a*phi*phi + T(0.75)*b*phi*phi*phi*phi - kappa*phi*delsq
-      - T(0.5)*kappa*(g0*g0 + g1*g1 + g2*g2);
-    const T mu = a*phi + b*phi*phi*phi - kappa*delsq;
-    s.q[qs][0][i] = g0;
-    s.q[qs][1][i] = g1;
-    s.q[qs][2][i] = g2;
-    s.q[qs][3][i] = p0;
-    s.q[qs][4][i] = mu;
-  }
-}
-
-// P_ab at a Q point (symmetric.c:410-413)
-template <typename T>
-__device__ __forceinline__ T pab_q(const T (&q)[5][QPLANE], int i, T kappa, int a, int b) {
-  const T s = kappa*q[a][i]*q[b][i];
-  return (a == b) ? q[3][i] + s : s;
-}
-
-template <typename T>
-__device__ __forceinline__ T adv1(T uf, T phi_self, T phi_other, bool west) {
-  if (west) return uf*((uf > T(0.0)) ? phi_other : phi_self);
-  return uf*((uf < T(0.0)) ? phi_other : phi_self);
-}
+using namespace tile;
 
 template <typename T, int ORDER, int METHOD, bool HAVE_U>
 __global__ void __launch_bounds__(NT)
@@ -173,9 +44,6 @@ force_ch_kernel(Geom g, FeParams fe, int xchunk, const T * __restrict__ phi,
   const int xe = min(xb + xchunk, g.nx);
   const int y = y0 + ty, z = z0 + tz;
   const bool active = (y < g.ny) && (z < g.nz);
-  const T kappa = T(fe.kappa);
-  const T mob = T(fe.mobility);
-  const T a1 = T(-0.213933), a2 = T(0.927865), a3 = T(0.286067);
 
   // prologue: phi planes xb-2 .. xb+1, Q planes xb-1, xb; registers hold
   // phi plane xb+2 and the velocities of plane xb
@@ -205,34 +73,9 @@ force_ch_kernel(Geom g, FeParams fe, int xchunk, const T * __restrict__ phi,
 
     // C: outputs on plane x
     if (active) {
-      const T (&qm)[5][QPLANE] = s.q[modn(x - 1, 3)];
-      const T (&q0)[5][QPLANE] = s.q[modn(x, 3)];
-      const T (&qp)[5][QPLANE] = s.q[modn(x + 1, 3)];
-      const int c = (ty + 1)*QZ + (tz + 1);      // site in the Q tile
       const int64_t site = ((int64_t) x*g.ny + y)*g.nz + z;
-
-      T F[3];
-      if constexpr (METHOD == 0) {
-#pragma unroll
-	for (int a = 0; a < 3; a++) {
-	  T s0 = pab_q(q0, c, kappa, a, 0);
-	  T fa = T(-0.5)*(pab_q(qp, c, kappa, a, 0) + s0);
-	  fa += T(0.5)*(pab_q(qm, c, kappa, a, 0) + s0);
-	  s0 = pab_q(q0, c, kappa, a, 1);
-	  fa -= T(0.5)*(pab_q(q0, c + QZ, kappa, a, 1) + s0);
-	  fa += T(0.5)*(pab_q(q0, c - QZ, kappa, a, 1) + s0);
-	  s0 = pab_q(q0, c, kappa, a, 2);
-	  fa -= T(0.5)*(pab_q(q0, c + 1, kappa, a, 2) + s0);
-	  fa += T(0.5)*(pab_q(q0, c - 1, kappa, a, 2) + s0);
-	  F[a] = fa;
-	}
-      }
-      else {
-	const T phi0 = s.phi[modn(x, 5)][(ty + 2)*PZ + (tz + 2)];
-	F[0] = T(0.0) + -phi0*T(0.5)*(qp[4][c] - qm[4][c]);
-	F[1] = T(0.0) + -phi0*T(0.5)*(q0[4][c + QZ] - q0[4][c - QZ]);
-	F[2] = T(0.0) + -phi0*T(0.5)*(q0[4][c + 1] - q0[4][c - 1]);
-      }
+      T F[3], pnew;
+      force_ch_site<T, ORDER, METHOD, HAVE_U>(s, fe, x, ty, tz, uc, F, pnew);
       if (force_mode == 0) {
 	force[site] = fz0 + F[0];
 	force[g.nsite + site] = fz1 + F[1];
@@ -243,75 +86,6 @@ force_ch_kernel(Geom g, FeParams fe, int xchunk, const T * __restrict__ phi,
 	force[g.nsite + site] += F[1];
 	force[2*g.nsite + site] += F[2];
       }
-
-      // Cahn-Hilliard fluxes: phi from the ring, mu from Q
-      const T (&P0)[PPLANE] = s.phi[modn(x, 5)];
-      const int cp = (ty + 2)*PZ + (tz + 2);     // site in the phi tile
-      const T p0 = P0[cp];
-      T fw = T(0), fe_ = T(0), fyp = T(0), fym = T(0), fzp = T(0), fzm = T(0);
-      if constexpr (HAVE_U) {
-	const T ux0 = uc[0], uxm = uc[1], uxp = uc[2];
-	const T uy0 = uc[3], uyp = uc[4], uym = uc[5];
-	const T uz0 = uc[6], uzp = uc[7], uzm = uc[8];
-	const T uw = T(0.5)*(ux0 + uxm);
-	const T ue = T(0.5)*(ux0 + uxp);
-	const T uyf = T(0.5)*(uy0 + uyp);
-	const T uyb = T(0.5)*(uym + uy0);
-	const T uzf = T(0.5)*(uz0 + uzp);
-	const T uzb = T(0.5)*(uzm + uz0);
-	const T (&PM)[PPLANE] = s.phi[modn(x - 1, 5)];
-	const T (&PP)[PPLANE] = s.phi[modn(x + 1, 5)];
-	if constexpr (ORDER == 1) {
-	  fw = adv1(uw, p0, PM[cp], true);
-	  fe_ = adv1(ue, p0, PP[cp], false);
-	  fyp = adv1(uyf, p0, P0[cp + PZ], false);
-	  fym = adv1(uyb, P0[cp - PZ], p0, false);
-	  fzp = adv1(uzf, p0, P0[cp + 1], false);
-	  fzm = adv1(uzb, P0[cp - 1], p0, false);
-	}
-	else if constexpr (ORDER == 2) {
-	  fw = uw*T(0.5)*(PM[cp] + p0);
-	  fe_ = ue*T(0.5)*(p0 + PP[cp]);
-	  fyp = uyf*T(0.5)*(p0 + P0[cp + PZ]);
-	  fym = uyb*T(0.5)*(P0[cp - PZ] + p0);
-	  fzp = uzf*T(0.5)*(p0 + P0[cp + 1]);
-	  fzm = uzb*T(0.5)*(P0[cp - 1] + p0);
-	}
-	else {
-	  const T (&PM2)[PPLANE] = s.phi[modn(x - 2, 5)];
-	  const T (&PP2)[PPLANE] = s.phi[modn(x + 2, 5)];
-	  T d1, d2, d3;
-	  if (uw > T(0.0)) { d1 = PM2[cp]; d2 = PM[cp]; d3 = p0; }
-	  else             { d1 = PP[cp]; d2 = p0; d3 = PM[cp]; }
-	  fw = uw*(a1*d1 + a2*d2 + a3*d3);
-	  if (ue < T(0.0)) { d1 = PP2[cp]; d2 = PP[cp]; d3 = p0; }
-	  else             { d1 = PM[cp]; d2 = p0; d3 = PP[cp]; }
-	  fe_ = ue*(a1*d1 + a2*d2 + a3*d3);
-	  if (uyf < T(0.0)) { d1 = P0[cp + 2*PZ]; d2 = P0[cp + PZ]; d3 = p0; }
-	  else              { d1 = P0[cp - PZ]; d2 = p0; d3 = P0[cp + PZ]; }
-	  fyp = uyf*(a1*d1 + a2*d2 + a3*d3);
-	  if (uyb < T(0.0)) { d1 = P0[cp + PZ]; d2 = p0; d3 = P0[cp - PZ]; }
-	  else              { d1 = P0[cp - 2*PZ]; d2 = P0[cp - PZ]; d3 = p0; }
-	  fym = uyb*(a1*d1 + a2*d2 + a3*d3);
-	  if (uzf < T(0.0)) { d1 = P0[cp + 2]; d2 = P0[cp + 1]; d3 = p0; }
-	  else              { d1 = P0[cp - 1]; d2 = p0; d3 = P0[cp + 1]; }
-	  fzp = uzf*(a1*d1 + a2*d2 + a3*d3);
-	  if (uzb < T(0.0)) { d1 = P0[cp + 1]; d2 = p0; d3 = P0[cp - 1]; }
-	  else              { d1 = P0[cp - 2]; d2 = P0[cp - 1]; d3 = p0; }
-	  fzm = uzb*(a1*d1 + a2*d2 + a3*d3);
-	}
-      }
-      {
-	const T m0 = q0[4][c];
-	fw -= mob*(m0 - qm[4][c]);
-	fe_ -= mob*(qp[4][c] - m0);
-	fyp -= mob*(q0[4][c + QZ] - m0);
-	fym -= mob*(m0 - q0[4][c - QZ]);
-	fzp -= mob*(q0[4][c + 1] - m0);
-	fzm -= mob*(m0 - q0[4][c - 1]);
-      }
-      T pnew = p0;
-      pnew -= (+ fe_ - fw + fyp - fym + fzp - fzm);
       phi_out[site] = pnew;
     }
     __syncthreads();

@@ -96,6 +96,18 @@ void launch_force_ch_f32(const Geom & g, const FeParams & fe, const float * phi,
 			 const float * u, float * force, float * phi_out,
 			 int force_mode, const double fzero[3], hipStream_t s);
 
+// K123: the whole binary-fluid step (force, Cahn-Hilliard, collision with
+// pull propagation) in one pass.  phi -> phi_out, u_prev -> u_next (the
+// Cahn-Hilliard part reads u_prev), g_in -> g_out with pull indexing.
+void launch_binary_step_f64(const Geom & g, const FeParams & fe, const LbParams & prm,
+			    const double * phi, double * phi_out, const double * u_prev,
+			    double * u_next, double * rho, const double * g_in,
+			    double * g_out, const double fzero[3], hipStream_t s);
+void launch_binary_step_f32(const Geom & g, const FeParams & fe, const LbParams & prm,
+			    const float * phi, float * phi_out, const float * u_prev,
+			    float * u_next, float * rho, const float * g_in,
+			    float * g_out, const double fzero[3], hipStream_t s);
+
 // Fill n doubles/floats with a value
 void launch_fill_f64(double * a, int64_t n, double v, hipStream_t s);
 void launch_fill_f32(float * a, int64_t n, float v, hipStream_t s);

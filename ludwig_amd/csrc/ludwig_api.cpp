@@ -84,6 +84,7 @@ extern "C" const char * ldw_last_error(void) { return g_errmsg; }
 static hipStream_t g_stream = nullptr;
 static int g_ready = 0;
 static int flush_force(void);
+static int materialise_derived(void);
 static int g_precision = 0;
 
 static int device_init(void) {
@@ -694,7 +695,7 @@ extern "C" int field_scalar_set(field_t * f, int index, double phi) {
 }
 
 extern "C" int field_memcpy(field_t * f, tdpMemcpyKind flag) {
-  if (flag == tdpMemcpyHostToDevice && flush_force()) return -1;
+  if (flag == tdpMemcpyHostToDevice && (flush_force() || materialise_derived())) return -1;
   const Geom g = f->geo;
   if (flag == tdpMemcpyHostToDevice) {
     if (f->prec == 64) {
@@ -822,14 +823,27 @@ struct hydro_s {
   long fp_version;
   FeParams fp_params;
   field_grad_t * fp_dphi;
+  // after a single-pass step the force is not stored: it is re-formed from
+  // the phi the step started from (still in the other phi buffer) on demand
+  void * d_u2;                     // second velocity buffer for the step kernel
+  int fd_derived;
+  field_t * fd_phi;
+  int fd_buf;
+  FeParams fd_params;
+  field_grad_t * fd_dphi;
 };
 
 static hydro_t * g_pending = nullptr;   // at most one deferred force
+static hydro_t * g_derived = nullptr;   // at most one force to re-form
 
 static void pending_drop_field(field_t * f) {
   if (g_pending && g_pending->fp_phi == f) {
     g_pending->fp_pending = 0;
     g_pending = nullptr;
+  }
+  if (g_derived && g_derived->fd_phi == f) {
+    g_derived->fd_derived = 0;
+    g_derived = nullptr;
   }
 }
 
@@ -863,7 +877,9 @@ extern "C" int hydro_create(pe_t * pe, cs_t * cs, lees_edw_t *, const hydro_opti
 extern "C" int hydro_free(hydro_t * h) {
   if (!h) return 0;
   if (g_pending == h) g_pending = nullptr;
+  if (g_derived == h) g_derived = nullptr;
   if (g_ready) (void) hipStreamSynchronize(g_stream);
+  if (h->d_u2) (void) hipFree(h->d_u2);
   (void) hipFree(h->d_rho); (void) hipFree(h->d_u); (void) hipFree(h->d_force);
   free(h->rho); free(h->u); free(h->force);
   free(h);
@@ -872,6 +888,7 @@ extern "C" int hydro_free(hydro_t * h) {
 
 static int hydro_materialise(hydro_t * h) {
   if (flush_force()) return -1;
+  if (materialise_derived()) return -1;
   const Geom g = h->geo;
   if (h->force_uniform) {
     for (int a = 0; a < 3; a++) {
@@ -894,6 +911,7 @@ static int hydro_materialise(hydro_t * h) {
 
 extern "C" int hydro_memcpy(hydro_t * h, tdpMemcpyKind flag) {
   if (flush_force()) return -1;
+  if (materialise_derived()) return -1;
   const Geom g = h->geo;
   if (flag == tdpMemcpyHostToDevice) {
     if (h->prec == 64) {
@@ -949,6 +967,7 @@ extern "C" int hydro_memcpy(hydro_t * h, tdpMemcpyKind flag) {
 extern "C" int hydro_u_halo(hydro_t *) { return 0; }
 
 extern "C" int hydro_f_zero(hydro_t * h, const double fzero[3]) {
+  if (g_derived == h) { g_derived = nullptr; h->fd_derived = 0; }
   h->force_uniform = 1;
   for (int a = 0; a < 3; a++) h->fzero[a] = fzero[a];
   return 0;
@@ -1103,6 +1122,7 @@ extern "C" int phi_force_calculation(pe_t *, cs_t *, lees_edw_t * le, wall_t *, 
   if (!fs) FAIL("phi_force_calculation: only the symmetric free energy is built");
   if (phi->prec != hydro->prec) FAIL("phi_force_calculation: precision mismatch");
   if (flush_force()) return -1;
+  if (materialise_derived()) return -1;
   hydro->fp_pending = 1;
   hydro->fp_phi = phi;
   hydro->fp_version = phi->version;
@@ -1139,6 +1159,35 @@ static int flush_force(void) {
   CHECK_LAUNCH();
   hydro->force_uniform = 0;
   dphi->mu_version = phi->version;
+  return 0;
+}
+
+/* Re-form the force of the last single-pass step from the phi it started
+ * from (K2 on the other phi buffer); mu is not kept */
+static int materialise_derived(void) {
+  hydro_t * hydro = g_derived;
+  if (hydro == nullptr) return 0;
+  g_derived = nullptr;
+  hydro->fd_derived = 0;
+  field_t * phi = hydro->fd_phi;
+  const Geom g = phi->geo;
+  const double fz[3] = {0.0, 0.0, 0.0};
+  {
+    KTimer t("phi_force_mu");
+    if (phi->prec == 64) {
+      ldw::launch_phi_force_mu_f64(g, hydro->fd_params, (const double *) phi->d_data[hydro->fd_buf],
+				   (double *) hydro->d_force, (double *) hydro->fd_dphi->d_mu,
+				   0, fz, g_stream);
+    }
+    else {
+      ldw::launch_phi_force_mu_f32(g, hydro->fd_params, (const float *) phi->d_data[hydro->fd_buf],
+				   (float *) hydro->d_force, (float *) hydro->fd_dphi->d_mu,
+				   0, fz, g_stream);
+    }
+  }
+  CHECK_LAUNCH();
+  hydro->fd_dphi->mu_version = 0;   // mu of the previous phi, not the current one
+  hydro->force_uniform = 0;
   return 0;
 }
 
@@ -1185,6 +1234,7 @@ extern "C" int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi, hydro
   field_grad_t * dphi = fs->dphi;
   FeParams p = fe_params_of(fs, FE_FORCE_METHOD_STRESS_DIVERGENCE);
   const Geom g = phi->geo;
+  if (materialise_derived()) return -1;
 
   if (hydro && hydro->fp_pending && g_pending == hydro && hydro->fp_phi == phi
       && hydro->fp_version == phi->version) {
@@ -1329,6 +1379,7 @@ extern "C" int lb_collide(lb_t * lb, hydro_t * hydro, map_t *, noise_t * noise, 
   if (fe && fe->use_stress_relaxation) FAIL("lb_collide: stress relaxation force method is not supported");
   if (hydro->prec != lb->prec) FAIL("lb_collide: precision mismatch between lb and hydro");
   if (flush_force()) return -1;
+  if (materialise_derived()) return -1;
 
   const Geom g = lb->geo;
   const void * force = nullptr;
@@ -1378,20 +1429,93 @@ extern "C" int lb_propagation(lb_t * lb) {
 /* Additions                                                                */
 /* ------------------------------------------------------------------------ */
 
-extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_grad_t * dphi,
-			       fe_t * fe, pth_t * pth, phi_ch_t * pch, int nsteps) {
+static int step_staged(lb_t * lb, hydro_t * hydro, field_t * phi, field_grad_t * dphi,
+		       fe_t * fe, pth_t * pth, phi_ch_t * pch) {
   const double fzero[3] = {0.0, 0.0, 0.0};
   const double uzero[3] = {0.0, 0.0, 0.0};
+  if (hydro_f_zero(hydro, fzero)) return -1;
+  if (field_halo(phi)) return -1;
+  if (field_grad_compute(dphi)) return -1;
+  if (phi_force_calculation(nullptr, nullptr, nullptr, nullptr, pth, fe, nullptr, phi, hydro)) return -1;
+  if (phi_cahn_hilliard(pch, fe, phi, hydro, nullptr, nullptr)) return -1;
+  if (hydro_u_zero(hydro, uzero)) return -1;
+  if (lb_collide(lb, hydro, nullptr, nullptr, fe, nullptr)) return -1;
+  if (lb_halo(lb)) return -1;
+  if (lb_propagation(lb)) return -1;
+  return 0;
+}
+
+/* The same step as step_staged() in one kernel (binary_step.hip).  Needs
+ * streamed distributions, a stored velocity field and one precision. */
+static int step_fused(lb_t * lb, hydro_t * hydro, field_t * phi, fe_symm_t * fs, pth_t * pth) {
+  const Geom g = lb->geo;
+  if (hydro->d_u2 == nullptr) {
+    size_t es = (hydro->prec == 64) ? 8 : 4;
+    if (dmalloc(&hydro->d_u2, es*3*(size_t) g.nsite)) return -1;
+  }
+  if (flush_force()) return -1;
+  // the force of the previous fused step is superseded, not re-formed
+  if (g_derived == hydro) { g_derived = nullptr; hydro->fd_derived = 0; }
+  if (materialise_derived()) return -1;
+  FeParams p = fe_params_of(fs, pth->method);
+  LbParams q = lb_params_of(lb, nullptr);
+  const double fz[3] = {0.0, 0.0, 0.0};
+  {
+    KTimer t("binary_step");
+    if (lb->prec == 64) {
+      ldw::launch_binary_step_f64(g, p, q, (const double *) phi->d_data[phi->cur],
+				  (double *) phi->d_data[1 - phi->cur], (const double *) hydro->d_u,
+				  (double *) hydro->d_u2, (double *) hydro->d_rho,
+				  (const double *) lb->d_g[lb->cur], (double *) lb->d_g[1 - lb->cur],
+				  fz, g_stream);
+    }
+    else {
+      ldw::launch_binary_step_f32(g, p, q, (const float *) phi->d_data[phi->cur],
+				  (float *) phi->d_data[1 - phi->cur], (const float *) hydro->d_u,
+				  (float *) hydro->d_u2, (float *) hydro->d_rho,
+				  (const float *) lb->d_g[lb->cur], (float *) lb->d_g[1 - lb->cur],
+				  fz, g_stream);
+    }
+  }
+  CHECK_LAUNCH();
+  phi->cur = 1 - phi->cur;
+  phi->version++;
+  void * t = hydro->d_u; hydro->d_u = hydro->d_u2; hydro->d_u2 = t;
+  lb->cur = 1 - lb->cur;
+  lb->streamed = 1;
+  hydro->u_uniform = 0;
+  hydro->force_uniform = 0;
+  hydro->fd_derived = 1;
+  hydro->fd_phi = phi;
+  hydro->fd_buf = 1 - phi->cur;
+  hydro->fd_params = p;
+  hydro->fd_dphi = fs->dphi;
+  g_derived = hydro;
+  fs->dphi->mu_version = 0;
+  return 0;
+}
+
+static int g_fused_step = -1;     // LDW_FUSED_STEP=0 selects the staged calls
+
+extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_grad_t * dphi,
+			       fe_t * fe, pth_t * pth, phi_ch_t * pch, int nsteps) {
+  if (g_fused_step < 0) {
+    const char * e = getenv("LDW_FUSED_STEP");
+    g_fused_step = (e && atoi(e) == 0) ? 0 : 1;
+  }
+  fe_symm_t * fs = as_symm(fe);
   for (int step = 0; step < nsteps; step++) {
-    if (hydro_f_zero(hydro, fzero)) return -1;
-    if (field_halo(phi)) return -1;
-    if (field_grad_compute(dphi)) return -1;
-    if (phi_force_calculation(nullptr, nullptr, nullptr, nullptr, pth, fe, nullptr, phi, hydro)) return -1;
-    if (phi_cahn_hilliard(pch, fe, phi, hydro, nullptr, nullptr)) return -1;
-    if (hydro_u_zero(hydro, uzero)) return -1;
-    if (lb_collide(lb, hydro, nullptr, nullptr, fe, nullptr)) return -1;
-    if (lb_halo(lb)) return -1;
-    if (lb_propagation(lb)) return -1;
+    const bool fusable = g_fused_step && fs && pth && hydro
+      && (pth->method == FE_FORCE_METHOD_STRESS_DIVERGENCE || pth->method == FE_FORCE_METHOD_PHI_GRADMU)
+      && lb->streamed && !hydro->u_uniform && !hydro->fp_pending
+      && lb->prec == hydro->prec && phi->prec == lb->prec && fs->phi == phi
+      && pch && pch->info.conserve == 0;
+    if (fusable) {
+      if (step_fused(lb, hydro, phi, fs, pth)) return -1;
+    }
+    else {
+      if (step_staged(lb, hydro, phi, dphi, fe, pth, pch)) return -1;
+    }
   }
   return 0;
 }
@@ -1596,6 +1720,7 @@ extern "C" int ldw_field_read(field_t * obj, double * data) {
 
 extern "C" int ldw_field_write(field_t * obj, const double * data) {
   if (flush_force()) return -1;
+  if (materialise_derived()) return -1;
   int rc = (obj->prec == 64) ? upload<double>(obj->d_data[obj->cur], data, obj->geo.nsite, 0.0, 1)
     : upload<float>(obj->d_data[obj->cur], data, obj->geo.nsite, 0.0, 1);
   obj->version++;

@@ -187,6 +187,11 @@ def test_staged_calls_equal_fused_step():
         b2.propagate()
     assert_exact(b2.phi_field(), b1.phi_field(), "phi")
     assert_exact(b2.f(), b1.f(), "f")
+    r1, u1, F1 = b1.hydro_fields()      # force re-formed after the single-pass step
+    r2, u2, F2 = b2.hydro_fields()
+    assert_exact(r1, r2, "rho")
+    assert_exact(u1, u2, "u")
+    assert_exact(F1, F2, "force")
     b1.close()
     b2.close()
 
