@@ -145,7 +145,32 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--profile-run", type=int, default=0,
+                    help="only run this many steps of the main workload (for rocprofv3)")
+    ap.add_argument("--precision", type=int, default=64)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the plain copy kernel (1 GiB, 20 launches)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import ctypes
+        import ludwig_amd as L
+        ms = ctypes.c_double()
+        nbytes = 1 << 30
+        L._ck(L.lib().ldw_copy_calibration(ctypes.c_longlong(nbytes), 20, ctypes.byref(ms)),
+              "ldw_copy_calibration")
+        print(json.dumps({"copy_bytes": nbytes, "ms": ms.value,
+                          "GBps_read_plus_write": 2 * nbytes / (ms.value * 1e-3) / 1e9}))
+        return
+    if args.profile_run:
+        import ludwig_amd as L
+        n = args.size
+        b = L.BinaryFluid((n, n, n), precision=args.precision, fatal=True, **C2)
+        b.init_spinodal(SEED, 0.0, 0.1)
+        b.step(args.profile_run)
+        b.synchronize()
+        b.close()
+        print(f"profile run: {n}^3 fp{args.precision}, {args.profile_run} steps")
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design note, not built)", file=sys.stderr)

@@ -368,6 +368,10 @@ int ldw_timer_enable(int on);
 int ldw_timer_reset(void);
 int ldw_timer_read(const char * name, double * ms, int * count);
 
+/* Bandwidth calibration: mean time per launch of a plain device copy of
+ * nbytes (read nbytes + write nbytes) over reps launches */
+int ldw_copy_calibration(long long nbytes, int reps, double * ms);
+
 /* Raw device arrays (layout as described at the top of this file) for
  * callers that keep data on the GPU; bytes per element in *elsize. */
 int ldw_lb_device_f(lb_t * lb, void ** ptr, int * elsize, int * shifted);

@@ -131,6 +131,7 @@ def lib():
         "ldw_lb_f_read": [vp, dp], "ldw_lb_f_write": [vp, dp],
         "ldw_field_read": [vp, dp], "ldw_field_write": [vp, dp],
         "ldw_hydro_read": [vp, dp, dp, dp], "ldw_hydro_write": [vp, dp, dp, dp],
+        "ldw_copy_calibration": [ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -1,6 +1,6 @@
 // d3q19.h -- D3Q19 velocity set and mode transforms as compile-time tables.
 //
-// Velocities and weights follow lb_d3q19.h:24-36 (opposite of p is 18 - p).
+// Velocities and weights follow lb_d3q19.h:24-36 (the opposite of p >= 1 is 19 - p).
 // The mode matrix rows are {1, c, cc - delta/3, chi1, chi1 c, chi2, chi2 c,
 // chi3} (lb_d3q19.c:107-153), the normalisers na = 1/sum_p w ma^2
 // (lb_d3q19.c:72-80) and the inverse mi = w na ma^T (lb_data.c:644).  Every

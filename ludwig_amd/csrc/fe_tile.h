@@ -24,6 +24,27 @@ constexpr int QZ = TZ + 2, QY = TY + 2;      // Q tile with halo 1
 constexpr int PPLANE = PZ*PY;
 constexpr int QPLANE = QZ*QY;
 
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs (each with its own L2), so physical block k runs on XCD k % 8.  Tiles
+// are numbered z-fastest, then y, then x-chunk, and XCD g is given the
+// contiguous range of tiles [g*C, (g+1)*C): tiles that share halo rows and
+// cache lines at their edges then run close together in time on the same
+// L2.  Returns false for the padding blocks of the last range.
+constexpr int NXCD = 8;
+
+__device__ __forceinline__ bool tile_of_block(int nbz, int nby, int nbx, int & bzi,
+					      int & byi, int & bxi) {
+  const int ntile = nbz*nby*nbx;
+  const int per = (ntile + NXCD - 1)/NXCD;
+  const int k = blockIdx.x;
+  const int t = (k % NXCD)*per + k/NXCD;
+  if (t >= ntile) return false;
+  bzi = t % nbz;
+  byi = (t/nbz) % nby;
+  bxi = t/(nbz*nby);
+  return true;
+}
+
 __device__ __forceinline__ int modn(int i, int n) {
   int r = i % n;
   return (r < 0) ? r + n : r;

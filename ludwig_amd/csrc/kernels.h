@@ -108,6 +108,9 @@ void launch_binary_step_f32(const Geom & g, const FeParams & fe, const LbParams 
 			    float * u_next, float * rho, const float * g_in,
 			    float * g_out, const double fzero[3], hipStream_t s);
 
+// Plain device copy of nbytes (multiple of 32) for bandwidth calibration
+void launch_copy(const void * src, void * dst, int64_t nbytes, hipStream_t s);
+
 // Fill n doubles/floats with a value
 void launch_fill_f64(double * a, int64_t n, double v, hipStream_t s);
 void launch_fill_f32(float * a, int64_t n, float v, hipStream_t s);

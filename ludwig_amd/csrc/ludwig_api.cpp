@@ -1761,3 +1761,31 @@ extern "C" int ldw_hydro_write(hydro_t * h, const double * rho, const double * u
   }
   return rc;
 }
+
+/* Bandwidth calibration: a plain copy of nbytes between two device buffers,
+ * timed with HIP events over reps launches; *ms is the mean per launch. */
+extern "C" int ldw_copy_calibration(long long nbytes, int reps, double * ms) {
+  if (device_init()) return -1;
+  if (nbytes <= 0 || nbytes % 32) FAIL("ldw_copy_calibration: nbytes must be a positive multiple of 32");
+  void * a = nullptr, * b = nullptr;
+  HIP_CHECK(hipMalloc(&a, (size_t) nbytes));
+  HIP_CHECK(hipMalloc(&b, (size_t) nbytes));
+  HIP_CHECK(hipMemsetAsync(a, 0, (size_t) nbytes, g_stream));
+  ldw::launch_copy(a, b, nbytes, g_stream);           // warm-up
+  CHECK_LAUNCH();
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  HIP_CHECK(hipEventRecord(e0, g_stream));
+  for (int r = 0; r < reps; r++) ldw::launch_copy(a, b, nbytes, g_stream);
+  HIP_CHECK(hipEventRecord(e1, g_stream));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float t = 0.0f;
+  HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+  *ms = (double) t/reps;
+  (void) hipEventDestroy(e0);
+  (void) hipEventDestroy(e1);
+  (void) hipFree(a);
+  (void) hipFree(b);
+  return 0;
+}

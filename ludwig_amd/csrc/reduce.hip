@@ -84,7 +84,22 @@ sum_stage2(const double * __restrict__ work, int nb, double * __restrict__ out) 
   }
 }
 
+__global__ void __launch_bounds__(256)
+copy_kernel(const double4 * __restrict__ src, double4 * __restrict__ dst, int64_t n) {
+  int64_t i = (int64_t) blockIdx.x*256 + threadIdx.x;
+  const int64_t stride = (int64_t) gridDim.x*256;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
+
+void launch_copy(const void * src, void * dst, int64_t nbytes, hipStream_t s) {
+  const int64_t n = nbytes/32;
+  int64_t nb = (n + 255)/256;
+  if (nb > 8192) nb = 8192;
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned) nb), dim3(256), 0, s,
+		     (const double4 *) src, (double4 *) dst, n);
+}
 
 int sum_blocks(int64_t n) {
   int64_t nb = (n + (int64_t) RB*8 - 1)/((int64_t) RB*8);

@@ -25,8 +25,8 @@
 
 #define NVEL 19
 
-/* D3Q19 velocity set and weights (lb_d3q19.h:24-36); opposite of p is
- * 18 - p. Weights are stored as numerators over 36. */
+/* D3Q19 velocity set and weights (lb_d3q19.h:24-36); the opposite of
+ * p >= 1 is 19 - p. Weights are stored as numerators over 36. */
 
 static const int cv_[NVEL][3] = {
   { 0,  0,  0},
