@@ -48,7 +48,7 @@ __device__ __forceinline__ void load_f(T (&f)[NVEL], const T * __restrict__ g_in
 
 template <typename T, int ORDER, int METHOD, bool GZ>
 __global__ void __launch_bounds__(NT)
-binary_step_kernel(Geom g, FeParams fe, LbParams prm, int xchunk, int nbz, int nby, int nbx,
+binary_step_kernel(Geom g, FeParams fe, LbParams prm, int xchunk, int nbz, int nby, int nbx, int remap,
 		   const T * __restrict__ phi, T * __restrict__ phi_out,
 		   const T * __restrict__ u_prev, T * __restrict__ u_next,
 		   T * __restrict__ rho, const T * __restrict__ g_in,
@@ -57,7 +57,7 @@ binary_step_kernel(Geom g, FeParams fe, LbParams prm, int xchunk, int nbz, int n
   __shared__ Smem<T> s;
 
   int bzi, byi, bxi;
-  if (!tile_of_block(nbz, nby, nbx, bzi, byi, bxi)) return;
+  if (!tile_of_block(nbz, nby, nbx, remap, bzi, byi, bxi)) return;
   const int tz = threadIdx.x, ty = threadIdx.y;
   const int z0 = bzi*TZ, y0 = byi*TY;
   const int xb = bxi*xchunk;
@@ -117,14 +117,15 @@ void launch(const Geom & g, const FeParams & fe, const LbParams & prm, const T *
   const int bz = (g.nz + TZ - 1)/TZ;
   const int by = (g.ny + TY - 1)/TY;
   int xchunk = g.nx;
-  while (xchunk > 8 && (int64_t) bz*by*((g.nx + xchunk - 1)/xchunk) < 2048) xchunk = (xchunk + 1)/2;
+  const int want = tune_min_blocks();
+  while (xchunk > 8 && (int64_t) bz*by*((g.nx + xchunk - 1)/xchunk) < want) xchunk = (xchunk + 1)/2;
   const int bx = (g.nx + xchunk - 1)/xchunk;
   const int ntile = bz*by*bx;
   dim3 grid(((ntile + NXCD - 1)/NXCD)*NXCD), block(TZ, TY, 1);
   const T f0 = T(fzero[0]), f1 = T(fzero[1]), f2 = T(fzero[2]);
   const bool gz = (prm.relax == 0);
 #define L_(O, M, G) hipLaunchKernelGGL((binary_step_kernel<T, O, M, G>), grid, block, 0, st, \
-				       g, fe, prm, xchunk, bz, by, bx, phi, phi_out, u_prev, u_next, rho, \
+				       g, fe, prm, xchunk, bz, by, bx, tune_remap(), phi, phi_out, u_prev, u_next, rho, \
 				       g_in, g_out, f0, f1, f2)
 #define L_G(O, M) do { if (gz) L_(O, M, true); else L_(O, M, false); } while (0)
 #define L_M(O) do { if (fe.force_method == 0) L_G(O, 0); else L_G(O, 1); } while (0)

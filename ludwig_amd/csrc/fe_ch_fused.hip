@@ -32,14 +32,14 @@ using namespace tile;
 
 template <typename T, int ORDER, int METHOD, bool HAVE_U>
 __global__ void __launch_bounds__(NT)
-force_ch_kernel(Geom g, FeParams fe, int xchunk, int nbz, int nby, int nbx, const T * __restrict__ phi,
+force_ch_kernel(Geom g, FeParams fe, int xchunk, int nbz, int nby, int nbx, int remap, const T * __restrict__ phi,
 		const T * __restrict__ u, T * __restrict__ force, T * __restrict__ phi_out,
 		int force_mode, T fz0, T fz1, T fz2) {
 
   __shared__ Smem<T> s;
 
   int bzi, byi, bxi;
-  if (!tile_of_block(nbz, nby, nbx, bzi, byi, bxi)) return;
+  if (!tile_of_block(nbz, nby, nbx, remap, bzi, byi, bxi)) return;
   const int tz = threadIdx.x, ty = threadIdx.y;
   const int z0 = bzi*TZ, y0 = byi*TY;
   const int xb = bxi*xchunk;
@@ -105,13 +105,14 @@ void launch(const Geom & g, const FeParams & fe, const T * phi, const T * u, T *
   const int by = (g.ny + TY - 1)/TY;
   // enough blocks to fill 256 CUs several times; chunk of x planes per block
   int xchunk = g.nx;
-  while (xchunk > 8 && (int64_t) bz*by*((g.nx + xchunk - 1)/xchunk) < 2048) xchunk = (xchunk + 1)/2;
+  const int want = tune_min_blocks();
+  while (xchunk > 8 && (int64_t) bz*by*((g.nx + xchunk - 1)/xchunk) < want) xchunk = (xchunk + 1)/2;
   const int bx = (g.nx + xchunk - 1)/xchunk;
   const int ntile = bz*by*bx;
   dim3 grid(((ntile + NXCD - 1)/NXCD)*NXCD), block(TZ, TY, 1);
   const T f0 = T(fzero[0]), f1 = T(fzero[1]), f2 = T(fzero[2]);
 #define L_(O, M, U) hipLaunchKernelGGL((force_ch_kernel<T, O, M, U>), grid, block, 0, st, \
-				       g, fe, xchunk, bz, by, bx, phi, u, force, phi_out, force_mode, f0, f1, f2)
+				       g, fe, xchunk, bz, by, bx, tune_remap(), phi, u, force, phi_out, force_mode, f0, f1, f2)
 #define L_U(O, M) do { if (u) L_(O, M, true); else L_(O, M, false); } while (0)
 #define L_M(O) do { if (fe.force_method == 0) L_U(O, 0); else L_U(O, 1); } while (0)
   if (fe.order == 1) L_M(1);

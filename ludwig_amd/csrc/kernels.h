@@ -34,6 +34,12 @@ struct FeParams {
   int force_method;         // 0 stress divergence, 1 phi grad mu
 };
 
+// Launch tuning of the tiled kernels, read once from the environment:
+// LDW_XCD_REMAP (1: XCD-contiguous tile ranges, 0: linear order, default) and
+// LDW_MIN_BLOCKS (x-chunks are halved until the grid has this many blocks).
+int tune_remap(void);
+int tune_min_blocks(void);
+
 // K1: fused D3Q19 MRT collision + pull propagation.
 //   g_in/g_out: 19 planes of nsite (post-collision distributions; when
 //   `pull` is 1 the pre-collision value at x is g_in(x - c_p), else g_in(x)).

@@ -8,6 +8,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "kernels.h"
 
 namespace ldw {
@@ -99,6 +101,23 @@ void launch_copy(const void * src, void * dst, int64_t nbytes, hipStream_t s) {
   if (nb > 8192) nb = 8192;
   hipLaunchKernelGGL(copy_kernel, dim3((unsigned) nb), dim3(256), 0, s,
 		     (const double4 *) src, (double4 *) dst, n);
+}
+
+static int env_int(const char * name, int dflt) {
+  const char * e = getenv(name);
+  return e ? atoi(e) : dflt;
+}
+
+int tune_remap(void) {
+  static int v = -1;
+  if (v < 0) v = env_int("LDW_XCD_REMAP", 0) ? 1 : 0;
+  return v;
+}
+
+int tune_min_blocks(void) {
+  static int v = -1;
+  if (v < 0) v = env_int("LDW_MIN_BLOCKS", 2048);
+  return v;
 }
 
 int sum_blocks(int64_t n) {
