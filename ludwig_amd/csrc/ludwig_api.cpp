@@ -291,8 +291,8 @@ static physics_t * g_phys = nullptr;
 extern "C" int physics_create(pe_t *, physics_t ** phys) {
   physics_t * p = (physics_t *) calloc(1, sizeof(physics_t));
   p->rho0 = 1.0;            /* physics.c RHO_DEFAULT */
-  p->eta_shear = 0.1;       /* physics.c ETA_DEFAULT */
-  p->eta_bulk = 0.1;
+  p->eta_shear = 1.0/6.0;   /* physics.c:36 ETA_DEFAULT */
+  p->eta_bulk = 1.0/6.0;
   p->mobility = 0.0;
   *phys = p;
   g_phys = p;
@@ -641,6 +641,8 @@ struct field_grad_s {
   void * d_grad;           // materialised only for reads
   void * d_delsq;
   grad_ft d2;
+  long grad_version;       // phi version at the last field_grad_compute()
+  int grad_buf;            // phi buffer holding that version
 };
 
 extern "C" field_options_t field_options_ndata_nhalo(int ndata, int nhalo) {
@@ -765,8 +767,15 @@ extern "C" int field_grad_set(field_grad_t * g, grad_ft d2, grad_ft) {
   return 0;
 }
 
-/* Deferred: the stencil is evaluated inside the kernels that consume it. */
-extern "C" int field_grad_compute(field_grad_t *) { return 0; }
+/* Deferred: the stencil is evaluated inside the kernels that consume it.
+ * The phi version is recorded, so that a later field_grad_memcpy() returns
+ * the gradient of that phi, as the reference's stored gradient would be
+ * (its statistics after a step use the gradient from the step's start). */
+extern "C" int field_grad_compute(field_grad_t * g) {
+  g->grad_version = g->field->version;
+  g->grad_buf = g->field->cur;
+  return 0;
+}
 
 extern "C" int field_grad_memcpy(field_grad_t * g, tdpMemcpyKind flag) {
   if (flag != tdpMemcpyDeviceToHost) return 0;
@@ -777,7 +786,12 @@ extern "C" int field_grad_memcpy(field_grad_t * g, tdpMemcpyKind flag) {
     if (dmalloc(&g->d_grad, 8*3*(size_t) geo.nsite)) return -1;
     if (dmalloc(&g->d_delsq, 8*(size_t) geo.nsite)) return -1;
   }
-  ldw::launch_grad7_f64(geo, (const double *) f->d_data[f->cur], (double *) g->d_grad,
+  if (materialise_derived()) return -1;
+  // the phi of the last field_grad_compute(): the current one, or the one
+  // a single update ago (still in the other buffer)
+  int buf = f->cur;
+  if (f->version == g->grad_version + 1 && g->grad_buf == 1 - f->cur) buf = g->grad_buf;
+  ldw::launch_grad7_f64(geo, (const double *) f->d_data[buf], (double *) g->d_grad,
 			(double *) g->d_delsq, g_stream);
   CHECK_LAUNCH();
   std::vector<double> b1(3*geo.nsite), b2(geo.nsite);
@@ -1460,6 +1474,7 @@ static int step_fused(lb_t * lb, hydro_t * hydro, field_t * phi, fe_symm_t * fs,
   FeParams p = fe_params_of(fs, pth->method);
   LbParams q = lb_params_of(lb, nullptr);
   const double fz[3] = {0.0, 0.0, 0.0};
+  if (field_grad_compute(fs->dphi)) return -1;
   {
     KTimer t("binary_step");
     if (lb->prec == 64) {
