@@ -1,7 +1,8 @@
 """The host driver (ludwig_amd/ludwig_hip) runs unchanged Ludwig input files
 and its statistics lines are compared with logs written by the reference's
 own Ludwig.exe (and one committed regression log of the reference,
-tests/regression/d3q19-short/serial-dist-3du.log), token by token with the
+tests/regression/d3q19-short/serial-dist-3du.log, kept here as
+serial-dist-3du.ref), token by token with the
 absolute tolerance 1e-12 of the reference's tests/awk-fp-diff.sh."""
 
 import os
@@ -58,14 +59,14 @@ def run_driver(name, tmp_path):
                                   "gradmu-trt", "single-bgk"])
 def test_driver_matches_reference_log(name, tmp_path):
     got = run_driver(name, tmp_path)
-    want = open(os.path.join(LOGS, name + ".log")).read()
+    want = open(os.path.join(LOGS, name + ".ref")).read()
     print(name, "\n".join(stat_lines(got)[-4:]), flush=True)
     compare_logs(got, want)
 
 
 def test_log_comparison_catches_a_change():
     """The comparison itself: a one-digit change in a printed value fails."""
-    want = open(os.path.join(LOGS, "spin7-o1.log")).read()
+    want = open(os.path.join(LOGS, "spin7-o1.ref")).read()
     lines = want.splitlines()
     i = next(k for k, l in enumerate(lines) if l.startswith("[phi]") and "e-04" in l)
     lines[i] = lines[i].replace("e-04", "e-03", 1)
