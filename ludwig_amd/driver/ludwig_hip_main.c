@@ -217,19 +217,29 @@ static void stats_fed(fe_symm_t * fe, cs_t * cs, int step) {
   printf("[fed] %14d %17.10e %17.10e\n", step, rv*total, fluid/vol);
 }
 
-/* ludwig.c:1073-1124 with the momentum of stats_distribution.c */
+/* ludwig.c:1073-1124 with the momentum of stats_distribution.c:281-330:
+ * every term f_p c_p (p >= 1) is added to a Kahan sum per component */
 static void stats_momentum(lb_t * lb, cs_t * cs) {
+  static const int cv[19][3] = {
+    { 0,  0,  0}, { 1,  1,  0}, { 1,  0,  1}, { 1,  0,  0}, { 1,  0, -1},
+    { 1, -1,  0}, { 0,  1,  1}, { 0,  1,  0}, { 0,  1, -1}, { 0,  0,  1},
+    { 0,  0, -1}, { 0, -1,  1}, { 0, -1,  0}, { 0, -1, -1}, {-1,  1,  0},
+    {-1,  0,  1}, {-1,  0,  0}, {-1,  0, -1}, {-1, -1,  0}};
   volatile double g[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
   for (int ic = 1; ic <= nx; ic++)
     for (int jc = 1; jc <= ny; jc++)
       for (int kc = 1; kc <= nz; kc++) {
-	double gs[3];
-	lb_1st_moment(lb, cs_index(cs, ic, jc, kc), LB_RHO, gs);
-	for (int a = 0; a < 3; a++) {
-	  volatile double y = gs[a] + c[a];
-	  volatile double t = g[a] + y;
-	  c[a] = y - (t - g[a]);
-	  g[a] = t;
+	int index = cs_index(cs, ic, jc, kc);
+	for (int p = 1; p < 19; p++) {
+	  double f;
+	  lb_f(lb, index, p, 0, &f);
+	  for (int a = 0; a < 3; a++) {
+	    double term = f*cv[p][a];
+	    volatile double y = term + c[a];
+	    volatile double t = g[a] + y;
+	    c[a] = y - (t - g[a]);
+	    g[a] = t;
+	  }
 	}
       }
   double gt[3];
