@@ -10,9 +10,10 @@ noise, rho = 1, u = 0.  Inputs are resident in HBM before the timed region.
 
 Prints one JSON line:
   value        MLUPS = sites * steps / wall time of the timed region
-  roofline     for the dominant kernel (collide, K1): algorithmic bytes per
-               launch (360 B/site in FP64) / average kernel duration from HIP
-               events on the library's stream; peak 8000 GB/s
+  roofline     for the dominant kernel (binary_step_kernel): algorithmic bytes
+               per launch (376 B/site in FP64) / average kernel duration from
+               HIP events on the library's stream; peak 8000 GB/s; traffic =
+               HBM bytes per launch measured with rocprofv3 counters
   cpu_baseline the reference's own serial/OpenMP build (oracle/_ref/Ludwig.exe,
                compiled from the reference sources) on the host cores, or the
                CPU oracle when that binary is absent
@@ -44,6 +45,10 @@ KERNEL_DESC = {
     "collide": "collide_kernel (MRT collision + pull propagation)",
     "force_ch": "force_ch_kernel (force + Cahn-Hilliard)",
 }
+# HBM bytes per site of binary_step_kernel from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE, each scaled by the same counter's reading for the plain copy
+# kernel (fetch x 1.9884, write x 0.9702); profiles/binary_step_hbm_traffic_pmc.md
+TRAFFIC_B_SITE = {("binary_step", 64, 128): 439.5, ("binary_step", 64, 256): 431.8}
 C2 = dict(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25, tau=1.0, order=1)
 SEED = 8361235
 
@@ -194,7 +199,9 @@ def main():
         ms = kms[name]
         achieved = BYTES_SITE[64][name] * nsite / (ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": (TRAFFIC_B_SITE[(name, 64, args.size)] * nsite
+                            if (name, 64, args.size) in TRAFFIC_B_SITE else None),
                 "kernel": KERNEL_DESC.get(name, name),
                 "algorithmic_bytes_per_site": BYTES_SITE[64][name],
                 "avg_kernel_ms": ms}
