@@ -17,7 +17,8 @@ Prints one JSON line:
   cpu_baseline the reference's own serial/OpenMP build (oracle/_ref/Ludwig.exe,
                compiled from the reference sources) on the host cores, or the
                CPU oracle when that binary is absent
-  extra        256^3 FP64 and FP32-shifted throughput, per-kernel times
+  extra        256^3 FP64 and FP32-shifted throughput, per-kernel times; the
+               256-replica uneven-size ensemble (FP32) throughput
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
 """
@@ -88,6 +89,37 @@ def run_gpu(n, steps, warmup, precision, timing_steps):
             "ms_per_step": 1e3 * wall / steps, "mlups": mlups,
             "hbm_frac_step": mlups * 1e6 * BYTES_SITE[precision]["step"] / (HBM_PEAK_GBS * 1e9),
             "kernel_ms": kt, "mass_ok": ok}
+
+
+def run_ensemble(nrep, steps, warmup, precision=32, seed=2026):
+    """The uneven-replica ensemble: nrep cubic replicas with sizes from
+    {48, 64, 80, 96}, phi0, kappa, M drawn per replica; autotuned persistent
+    kernel, one graph launch per step."""
+    from ludwig_amd.ensemble import Ensemble, draw_replicas
+    specs = draw_replicas(nrep, seed=seed)
+    t0 = time.perf_counter()
+    e = Ensemble(specs, precision=precision, fatal=True)
+    t_init = time.perf_counter() - t0
+    rows = e.tune(reps=3)
+    cfg = e.config()
+    e.step(warmup)
+    import ludwig_amd as L
+    L.lib().ldw_device_synchronize()
+    t0 = time.perf_counter()
+    e.step(steps)
+    L.lib().ldw_device_synchronize()
+    wall = time.perf_counter() - t0
+    nsite = e.nsite
+    mass = [e.conserved_sums(r)[0] for r in (0, nrep - 1)]
+    e.close()
+    mlups = nsite * steps / wall / 1e6
+    bps = BYTES_SITE[precision]["binary_step"]
+    return {"replicas": nrep, "sites": nsite, "precision": precision, "steps": steps,
+            "ms_per_step": 1e3 * wall / steps, "mlups": mlups,
+            "hbm_GBps_algorithmic": mlups * 1e6 * bps / 1e9,
+            "hbm_frac_step": mlups * 1e6 * BYTES_SITE[precision]["step"] / (HBM_PEAK_GBS * 1e9),
+            "tuned": cfg, "tuner_trials": rows.tolist(), "init_s": t_init,
+            "mass_first_last": mass}
 
 
 def cpu_baseline(n=128, steps=10):
@@ -190,6 +222,10 @@ def main():
             extra[f"{n}^3_fp{prec}"] = {k: r[k] for k in ("mlups", "ms_per_step", "hbm_frac_step",
                                                           "kernel_ms", "mass_ok")}
             print(f"# extra {n} fp{prec}: {r}", file=sys.stderr, flush=True)
+
+        r = run_ensemble(256, max(20, args.steps // 4), 5)
+        extra["ensemble_256_replicas_fp32"] = r
+        print(f"# ensemble: {r}", file=sys.stderr, flush=True)
 
     nsite = args.size ** 3
     roof = None

@@ -51,6 +51,13 @@ int ldw_precision(void);
 /* Wait for all device work of the library */
 int ldw_device_synchronize(void);
 
+/* Device memory through the library's HIP runtime, for callers that keep
+ * data on the GPU.  ldw_memcpy takes a tdpMemcpyKind and is complete on
+ * return. */
+int ldw_device_alloc(long long bytes, void ** ptr);
+int ldw_device_free(void * ptr);
+int ldw_memcpy(void * dst, const void * src, long long bytes, int kind);
+
 /* ------------------------------------------------------------------------ */
 /* Parallel environment and coordinates                                     */
 /* ------------------------------------------------------------------------ */
@@ -378,6 +385,71 @@ int ldw_lb_device_f(lb_t * lb, void ** ptr, int * elsize, int * shifted);
 int ldw_field_device(field_t * obj, void ** ptr, int * elsize);
 int ldw_hydro_device(hydro_t * obj, void ** rho, void ** u, void ** force,
 		     int * elsize);
+
+/* ------------------------------------------------------------------------ */
+/* Replica ensemble (additive; no counterpart in the reference)             */
+/* ------------------------------------------------------------------------ */
+
+/* Many independent periodic binary-fluid lattices of different sizes and
+ * parameters advanced together, one persistent launch per time step
+ * (captured in a HIP graph).  Each replica follows exactly the arithmetic
+ * of a single lattice driven by ldw_step_binary() with the same size,
+ * parameters and seed:
+ *   phi = phi0 + noise (U - 1/2), the field_phi_init_spinodal() generator
+ *   (field_phi_init.c:484-518) with the replica's seed;
+ *   f = rest equilibrium at rho0 (lb_init_rest_f), u = 0;
+ *   M10 collision with eta_shear / eta_bulk; symmetric free energy with
+ *   a, b, kappa; Cahn-Hilliard with the mobility; 7-point gradient. */
+typedef struct {
+  int nx, ny, nz;
+  int seed;
+  double phi0, noise;
+  double a, b, kappa, mobility;
+  double eta_shear, eta_bulk;
+  double rho0;
+} ldw_replica_t;
+
+typedef struct ldw_ensemble_s ldw_ensemble_t;
+
+/* precision 64 or 32 (FP32 stores f - w rho0); adv_order 1..3;
+ * force_method FE_FORCE_METHOD_STRESS_DIVERGENCE or _PHI_GRADMU */
+int ldw_ensemble_create(int nreplica, const ldw_replica_t * spec, int precision,
+			int adv_order, int force_method, ldw_ensemble_t ** pens);
+int ldw_ensemble_free(ldw_ensemble_t * ens);
+
+/* Autotuner: times every tile shape (64x4, 32x8, 16x16 in z x y) and
+ * several x planes per work item on the current state, `reps` launches
+ * each, and keeps the fastest.  The state is not advanced.  Runs by itself
+ * before the first step unless a configuration was set. */
+int ldw_ensemble_tune(ldw_ensemble_t * ens, int reps);
+/* The autotuner's trials: up to maxrows rows of {tile z, tile y, x planes,
+ * blocks, ms per step} into rows (may be NULL); returns the number of trials */
+int ldw_ensemble_tune_results(ldw_ensemble_t * ens, int maxrows, double * rows);
+/* cfg = {tile z, tile y, x planes per item, blocks}; ms = the tuned step
+ * time (0 if not tuned) */
+int ldw_ensemble_config(ldw_ensemble_t * ens, int cfg[4], double * ms);
+/* Fix the configuration (tz x ty one of the shapes above; xplanes >= 1;
+ * blocks 0 -> as many as are resident at once) */
+int ldw_ensemble_config_set(ldw_ensemble_t * ens, int tz, int ty, int xplanes,
+			    int blocks);
+/* Use a captured HIP graph per step (default 1) or direct launches (0) */
+int ldw_ensemble_graph_set(ldw_ensemble_t * ens, int on);
+
+int ldw_ensemble_step(ldw_ensemble_t * ens, int nsteps);
+/* number of steps taken */
+long long ldw_ensemble_time(ldw_ensemble_t * ens);
+
+/* Replica r's current state as FP64 in the device layout
+ * (site = (x*ny + y)*nz + z): phi (nsite), rho (nsite), u (3 nsite), each
+ * may be NULL.  f: 19 nsite values, the distributions the reference's
+ * lb->f would hold after lb_propagation(). */
+int ldw_ensemble_snapshot(ldw_ensemble_t * ens, int r, double * phi, double * rho,
+			  double * u);
+int ldw_ensemble_f_read(ldw_ensemble_t * ens, int r, double * f);
+/* phi, ux, uy, uz of replica r as 4 nsite floats into device memory dst */
+int ldw_ensemble_snapshot_device(ldw_ensemble_t * ens, int r, float * dst);
+/* Compensated FP64 sums of replica r: out[0] = total density, out[1] = sum phi */
+int ldw_ensemble_conserved_sums(ldw_ensemble_t * ens, int r, double out[2]);
 
 #ifdef __cplusplus
 }

@@ -132,6 +132,8 @@ def lib():
         "ldw_field_read": [vp, dp], "ldw_field_write": [vp, dp],
         "ldw_hydro_read": [vp, dp, dp, dp], "ldw_hydro_write": [vp, dp, dp, dp],
         "ldw_copy_calibration": [ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+        "ldw_device_alloc": [ctypes.c_longlong, pvp], "ldw_device_free": [vp],
+        "ldw_memcpy": [vp, vp, ctypes.c_longlong, ctypes.c_int],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -153,6 +155,49 @@ def _ck(rc, what):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DeviceArray:
+    """A 1-D array in device memory allocated through libludwig_hip (the
+    library's HIP runtime), with host copies through numpy."""
+
+    def __init__(self, n, dtype=np.float32):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _ck(lib().ldw_device_alloc(self.nbytes, ctypes.byref(p)), "ldw_device_alloc")
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype if dtype is not None else np.asarray(a).dtype)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n, (a.size, self.n)
+        _ck(lib().ldw_memcpy(ctypes.c_void_p(self.ptr), a.ctypes.data_as(ctypes.c_void_p),
+                             self.nbytes, H2D), "ldw_memcpy")
+
+    def download(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        _ck(lib().ldw_memcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr),
+                             self.nbytes, D2H), "ldw_memcpy")
+        return out
+
+    def free(self):
+        if self.ptr and _lib is not None:
+            _lib.ldw_device_free(ctypes.c_void_p(self.ptr))
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def set_precision(bits):

@@ -29,19 +29,20 @@ namespace ldw {
 namespace {
 
 using namespace tile;
+using S = Shape64x4;
 
 template <typename T, int ORDER, int METHOD, bool HAVE_U>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(S::NT)
 force_ch_kernel(Geom g, FeParams fe, int xchunk, int nbz, int nby, int nbx, int remap, const T * __restrict__ phi,
 		const T * __restrict__ u, T * __restrict__ force, T * __restrict__ phi_out,
 		int force_mode, T fz0, T fz1, T fz2) {
 
-  __shared__ Smem<T> s;
+  __shared__ Smem<T, S> s;
 
   int bzi, byi, bxi;
   if (!tile_of_block(nbz, nby, nbx, remap, bzi, byi, bxi)) return;
   const int tz = threadIdx.x, ty = threadIdx.y;
-  const int z0 = bzi*TZ, y0 = byi*TY;
+  const int z0 = bzi*S::TZ, y0 = byi*S::TY;
   const int xb = bxi*xchunk;
   const int xe = min(xb + xchunk, g.nx);
   const int y = y0 + ty, z = z0 + tz;
@@ -50,8 +51,8 @@ force_ch_kernel(Geom g, FeParams fe, int xchunk, int nbz, int nby, int nbx, int 
   // prologue: phi planes xb-2 .. xb+1, Q planes xb-1, xb; registers hold
   // phi plane xb+2 and the velocities of plane xb
   for (int x = xb - 2; x <= xb + 1; x++) load_phi_plane(s, phi, g, x, y0, z0);
-  T pre[NPRE];
-  prefetch_phi_plane(pre, phi, g, xb + 2, y0, z0);
+  T pre[S::NPRE];
+  prefetch_phi_plane<T, S>(pre, phi, g, xb + 2, y0, z0);
   T uc[9];
   if constexpr (HAVE_U) {
     if (active) load_u(uc, u, g, xb, y, z);
@@ -63,7 +64,7 @@ force_ch_kernel(Geom g, FeParams fe, int xchunk, int nbz, int nby, int nbx, int 
   for (int x = xb; x < xe; x++) {
     // A: phi plane x+2 from registers; fetch plane x+3 for the next round
     store_phi_plane(s, pre, x + 2);
-    if (x + 1 < xe) prefetch_phi_plane(pre, phi, g, x + 3, y0, z0);
+    if (x + 1 < xe) prefetch_phi_plane<T, S>(pre, phi, g, x + 3, y0, z0);
     __syncthreads();
     // B: Q plane x+1
     compute_q_plane(s, fe, x + 1);
@@ -77,7 +78,7 @@ force_ch_kernel(Geom g, FeParams fe, int xchunk, int nbz, int nby, int nbx, int 
     if (active) {
       const int64_t site = ((int64_t) x*g.ny + y)*g.nz + z;
       T F[3], pnew;
-      force_ch_site<T, ORDER, METHOD, HAVE_U>(s, fe, x, ty, tz, uc, F, pnew);
+      force_ch_site<T, S, ORDER, METHOD, HAVE_U>(s, fe, x, ty, tz, uc, F, pnew);
       if (force_mode == 0) {
 	force[site] = fz0 + F[0];
 	force[g.nsite + site] = fz1 + F[1];
@@ -101,15 +102,15 @@ force_ch_kernel(Geom g, FeParams fe, int xchunk, int nbz, int nby, int nbx, int 
 template <typename T>
 void launch(const Geom & g, const FeParams & fe, const T * phi, const T * u, T * force,
 	    T * phi_out, int force_mode, const double fzero[3], hipStream_t st) {
-  const int bz = (g.nz + TZ - 1)/TZ;
-  const int by = (g.ny + TY - 1)/TY;
+  const int bz = (g.nz + S::TZ - 1)/S::TZ;
+  const int by = (g.ny + S::TY - 1)/S::TY;
   // enough blocks to fill 256 CUs several times; chunk of x planes per block
   int xchunk = g.nx;
   const int want = tune_min_blocks();
   while (xchunk > 8 && (int64_t) bz*by*((g.nx + xchunk - 1)/xchunk) < want) xchunk = (xchunk + 1)/2;
   const int bx = (g.nx + xchunk - 1)/xchunk;
   const int ntile = bz*by*bx;
-  dim3 grid(((ntile + NXCD - 1)/NXCD)*NXCD), block(TZ, TY, 1);
+  dim3 grid(((ntile + NXCD - 1)/NXCD)*NXCD), block(S::TZ, S::TY, 1);
   const T f0 = T(fzero[0]), f1 = T(fzero[1]), f2 = T(fzero[2]);
 #define L_(O, M, U) hipLaunchKernelGGL((force_ch_kernel<T, O, M, U>), grid, block, 0, st, \
 				       g, fe, xchunk, bz, by, bx, tune_remap(), phi, u, force, phi_out, force_mode, f0, f1, f2)

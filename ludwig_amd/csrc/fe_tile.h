@@ -16,13 +16,21 @@
 namespace ldw {
 namespace tile {
 
-constexpr int TZ = 64;
-constexpr int TY = 4;
-constexpr int NT = TZ*TY;
-constexpr int PZ = TZ + 4, PY = TY + 4;      // phi tile with halo 2
-constexpr int QZ = TZ + 2, QY = TY + 2;      // Q tile with halo 1
-constexpr int PPLANE = PZ*PY;
-constexpr int QPLANE = QZ*QY;
+// Tile shape: TZ x TY sites of a plane, one thread per site (block
+// dimensions (TZ, TY)).  The single-lattice kernels use 64 x 4; the replica
+// ensemble chooses among several shapes at run time (ensemble.hip).
+template <int TZ_, int TY_>
+struct Shape {
+  static constexpr int TZ = TZ_, TY = TY_;
+  static constexpr int NT = TZ*TY;
+  static constexpr int PZ = TZ + 4, PY = TY + 4;      // phi tile with halo 2
+  static constexpr int QZ = TZ + 2, QY = TY + 2;      // Q tile with halo 1
+  static constexpr int PPLANE = PZ*PY;
+  static constexpr int QPLANE = QZ*QY;
+  static constexpr int NPRE = (PPLANE + NT - 1)/NT;   // phi values prefetched per thread
+};
+
+using Shape64x4 = Shape<64, 4>;
 
 // Tile order.  Workgroups are dispatched round-robin over the 8 XCDs (each
 // with its own L2), so physical block k runs on XCD k % 8.  Tiles are
@@ -53,19 +61,19 @@ __device__ __forceinline__ int modn(int i, int n) {
   return (r < 0) ? r + n : r;
 }
 
-template <typename T>
+template <typename T, class S>
 struct Smem {
-  T phi[5][PPLANE];
-  T q[3][5][QPLANE];          // g0, g1, g2, p0, mu
+  T phi[5][S::PPLANE];
+  T q[3][5][S::QPLANE];       // g0, g1, g2, p0, mu
 };
 
-template <typename T>
-__device__ __forceinline__ void load_phi_plane(Smem<T> & s, const T * __restrict__ phi,
+template <typename T, class S>
+__device__ __forceinline__ void load_phi_plane(Smem<T, S> & s, const T * __restrict__ phi,
 					       const Geom & g, int x, int y0, int z0) {
   const int slot = modn(x, 5);
   const int xw = modn(x, g.nx);
-  for (int i = threadIdx.x + threadIdx.y*TZ; i < PPLANE; i += NT) {
-    const int yy = i / PZ, zz = i - yy*PZ;
+  for (int i = threadIdx.x + threadIdx.y*S::TZ; i < S::PPLANE; i += S::NT) {
+    const int yy = i / S::PZ, zz = i - yy*S::PZ;
     const int y = modn(y0 - 2 + yy, g.ny);
     const int z = modn(z0 - 2 + zz, g.nz);
     s.phi[slot][i] = phi[((int64_t) xw*g.ny + y)*g.nz + z];
@@ -75,17 +83,15 @@ __device__ __forceinline__ void load_phi_plane(Smem<T> & s, const T * __restrict
 // Software pipelining: the phi plane needed by the next iteration is
 // fetched into registers one iteration ahead, so its memory latency overlaps
 // the LDS compute of the current plane.
-constexpr int NPRE = (PPLANE + NT - 1)/NT;
-
-template <typename T>
-__device__ __forceinline__ void prefetch_phi_plane(T (&pre)[NPRE], const T * __restrict__ phi,
+template <typename T, class S>
+__device__ __forceinline__ void prefetch_phi_plane(T (&pre)[S::NPRE], const T * __restrict__ phi,
 						   const Geom & g, int x, int y0, int z0) {
   const int xw = modn(x, g.nx);
 #pragma unroll
-  for (int k = 0; k < NPRE; k++) {
-    const int i = threadIdx.x + threadIdx.y*TZ + k*NT;
-    if (i < PPLANE) {
-      const int yy = i / PZ, zz = i - yy*PZ;
+  for (int k = 0; k < S::NPRE; k++) {
+    const int i = threadIdx.x + threadIdx.y*S::TZ + k*S::NT;
+    if (i < S::PPLANE) {
+      const int yy = i / S::PZ, zz = i - yy*S::PZ;
       const int y = modn(y0 - 2 + yy, g.ny);
       const int z = modn(z0 - 2 + zz, g.nz);
       pre[k] = phi[((int64_t) xw*g.ny + y)*g.nz + z];
@@ -93,13 +99,13 @@ __device__ __forceinline__ void prefetch_phi_plane(T (&pre)[NPRE], const T * __r
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void store_phi_plane(Smem<T> & s, const T (&pre)[NPRE], int x) {
+template <typename T, class S>
+__device__ __forceinline__ void store_phi_plane(Smem<T, S> & s, const T (&pre)[S::NPRE], int x) {
   const int slot = modn(x, 5);
 #pragma unroll
-  for (int k = 0; k < NPRE; k++) {
-    const int i = threadIdx.x + threadIdx.y*TZ + k*NT;
-    if (i < PPLANE) s.phi[slot][i] = pre[k];
+  for (int k = 0; k < S::NPRE; k++) {
+    const int i = threadIdx.x + threadIdx.y*S::TZ + k*S::NT;
+    if (i < S::PPLANE) s.phi[slot][i] = pre[k];
   }
 }
 
@@ -132,12 +138,13 @@ __device__ __forceinline__ void load_u(T (&v)[9], const T * __restrict__ u, cons
 }
 
 // Q on plane x at every point of the halo-1 tile
-template <typename T>
-__device__ __forceinline__ void compute_q_plane(Smem<T> & s, const FeParams & fe, int x) {
+template <typename T, class S>
+__device__ __forceinline__ void compute_q_plane(Smem<T, S> & s, const FeParams & fe, int x) {
+  constexpr int PZ = S::PZ, QZ = S::QZ;
   const int sm = modn(x - 1, 5), s0 = modn(x, 5), sp = modn(x + 1, 5);
   const int qs = modn(x, 3);
   const T a = T(fe.a), b = T(fe.b), kappa = T(fe.kappa);
-  for (int i = threadIdx.x + threadIdx.y*TZ; i < QPLANE; i += NT) {
+  for (int i = threadIdx.x + threadIdx.y*S::TZ; i < S::QPLANE; i += S::NT) {
     const int yy = i / QZ, zz = i - yy*QZ;
     const int c = (yy + 1)*PZ + (zz + 1);         // same point in the phi tile
     const T phi = s.phi[s0][c];
@@ -160,7 +167,7 @@ __device__ __forceinline__ void compute_q_plane(Smem<T> & s, const FeParams & fe
 }
 
 // P_ab at a Q point (symmetric.c:410-413)
-template <typename T>
+template <typename T, int QPLANE>
 __device__ __forceinline__ T pab_q(const T (&q)[5][QPLANE], int i, T kappa, int a, int b) {
   const T s = kappa*q[a][i]*q[b][i];
   return (a == b) ? q[3][i] + s : s;
@@ -176,10 +183,11 @@ __device__ __forceinline__ T adv1(T uf, T phi_self, T phi_other, bool west) {
 // Force and Cahn-Hilliard update at site (x, y, z) of the output plane x,
 // from the LDS rings (Q planes x-1, x, x+1; phi planes x-2 .. x+2) and the
 // velocities uc of load_u().  Returns F and the updated phi.
-template <typename T, int ORDER, int METHOD, bool HAVE_U>
-__device__ __forceinline__ void force_ch_site(const Smem<T> & s, const FeParams & fe,
+template <typename T, class S, int ORDER, int METHOD, bool HAVE_U>
+__device__ __forceinline__ void force_ch_site(const Smem<T, S> & s, const FeParams & fe,
 					      int x, int ty, int tz, const T (&uc)[9],
 					      T (&F)[3], T & pnew) {
+  constexpr int PZ = S::PZ, QZ = S::QZ, PPLANE = S::PPLANE, QPLANE = S::QPLANE;
   const T kappa = T(fe.kappa);
   const T mob = T(fe.mobility);
   const T a1 = T(-0.213933), a2 = T(0.927865), a3 = T(0.286067);

@@ -114,6 +114,47 @@ void launch_binary_step_f32(const Geom & g, const FeParams & fe, const LbParams 
 			    float * u_next, float * rho, const float * g_in,
 			    float * g_out, const double fzero[3], hipStream_t s);
 
+// Replica ensemble (ensemble.hip).  Each replica owns a slice of shared
+// pools: its 19 distribution planes start at element off_f of the f pools,
+// its phi and rho planes at off_s, its 3 velocity planes at 3*off_s.
+struct EnsRep {
+  Geom g;
+  int64_t off_f;
+  int64_t off_s;
+  FeParams fe;
+  LbParams prm;
+};
+
+// One work item: tile [y0, y0+TY) x [z0, z0+TZ) of planes [xb, xe) of replica rep
+struct EnsItem {
+  int rep, y0, z0, xb, xe, pad[3];
+};
+
+// tile shapes: 0 -> 64 x 4, 1 -> 32 x 8, 2 -> 16 x 16 (TZ x TY)
+constexpr int ENS_NSHAPE = 3;
+int ens_shape_tz(int shape);
+int ens_shape_ty(int shape);
+// resident blocks per CU of the ensemble kernel (0 on error)
+int ens_blocks_per_cu(int shape, int precision, int order, int method);
+
+// One step of every replica: phi -> phi_out, u_prev -> u_next, g_in -> g_out
+// (pull), rho written.  ctr: two queue counters, zero before the first
+// launch; each launch leaves them zero.  M10 collision (ghost modes relax
+// to zero).  Returns the launch status.
+hipError_t launch_ensemble_step_f64(int shape, int order, int method, int nblocks,
+				    const EnsRep * reps, const EnsItem * items, int nitems,
+				    unsigned int * ctr, const double * phi, double * phi_out,
+				    const double * u_prev, double * u_next, double * rho,
+				    const double * g_in, double * g_out, hipStream_t s);
+hipError_t launch_ensemble_step_f32(int shape, int order, int method, int nblocks,
+				    const EnsRep * reps, const EnsItem * items, int nitems,
+				    unsigned int * ctr, const float * phi, float * phi_out,
+				    const float * u_prev, float * u_next, float * rho,
+				    const float * g_in, float * g_out, hipStream_t s);
+
+// b[i] = (float) a[i] for n values of precision 64 (double) or 32 (float)
+void launch_to_f32(const void * a, int precision, float * b, int64_t n, hipStream_t s);
+
 // Plain device copy of nbytes (multiple of 32) for bandwidth calibration
 void launch_copy(const void * src, void * dst, int64_t nbytes, hipStream_t s);
 

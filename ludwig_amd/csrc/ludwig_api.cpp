@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "../../include/ludwig_hip.h"
+#include "api_internal.h"
 #include "d3q19.h"
 #include "kernels.h"
 
@@ -77,6 +78,15 @@ static int ldw_fail(const char * file, int line, const char * fmt, ...) {
 extern "C" int ldw_error_mode_set(int fatal) { g_fatal = fatal ? 1 : 0; return 0; }
 extern "C" const char * ldw_last_error(void) { return g_errmsg; }
 
+int ldw::api::fail(const char * file, int line, const char * fmt, ...) {
+  char buf[768];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  return ldw_fail(file, line, "%s", buf);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Device, stream, precision, timers                                        */
 /* ------------------------------------------------------------------------ */
@@ -99,6 +109,9 @@ static int device_init(void) {
   return 0;
 }
 
+int ldw::api::device_init(void) { return ::device_init(); }
+hipStream_t ldw::api::stream(void) { return g_stream; }
+
 static int precision_now(void) {
   if (g_precision == 0) {
     const char * e = getenv("LDW_PRECISION");
@@ -118,6 +131,43 @@ extern "C" int ldw_precision(void) { return precision_now(); }
 extern "C" int ldw_device_synchronize(void) {
   if (!g_ready) return 0;
   if (flush_force()) return -1;
+  HIP_CHECK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+/* Device memory for callers that keep their own data on the GPU (the
+ * library's HIP runtime; a second runtime in the same process, such as the
+ * one a framework bundles, cannot share the device with it) */
+extern "C" int ldw_device_alloc(long long bytes, void ** ptr) {
+  if (ptr == nullptr || bytes < 0) FAIL("ldw_device_alloc: bad argument");
+  *ptr = nullptr;
+  if (device_init()) return -1;
+  if (bytes == 0) return 0;
+  HIP_CHECK(hipMalloc(ptr, (size_t) bytes));
+  return 0;
+}
+
+extern "C" int ldw_device_free(void * ptr) {
+  if (ptr == nullptr) return 0;
+  HIP_CHECK(hipStreamSynchronize(g_stream));
+  HIP_CHECK(hipFree(ptr));
+  return 0;
+}
+
+/* Copy on the library's stream, complete on return */
+extern "C" int ldw_memcpy(void * dst, const void * src, long long bytes, int kind) {
+  if (bytes < 0) FAIL("ldw_memcpy: negative size");
+  if (bytes == 0) return 0;
+  if (device_init()) return -1;
+  hipMemcpyKind k;
+  switch (kind) {
+  case tdpMemcpyHostToDevice: k = hipMemcpyHostToDevice; break;
+  case tdpMemcpyDeviceToHost: k = hipMemcpyDeviceToHost; break;
+  case tdpMemcpyDeviceToDevice: k = hipMemcpyDeviceToDevice; break;
+  default: FAIL("ldw_memcpy: kind %d", kind);
+  }
+  if (flush_force()) return -1;
+  HIP_CHECK(hipMemcpyAsync(dst, src, (size_t) bytes, k, g_stream));
   HIP_CHECK(hipStreamSynchronize(g_stream));
   return 0;
 }
@@ -462,22 +512,24 @@ extern "C" int lb_1st_moment(lb_t * lb, int index, lb_dist_enum_t nd, double g[3
 }
 
 /* lb_data.c:809-840, evaluated on the host copy */
-extern "C" int lb_1st_moment_equilib_set(lb_t * lb, int index, double rho, double u[3]) {
-  for (int p = 0; p < 19; p++) {
-    double cs2 = (1.0/3.0);
-    double rcs2 = 1.0/cs2;
-    double udotc = 0.0;
-    double sdotq = 0.0;
-    for (int ia = 0; ia < 3; ia++) {
-      udotc += u[ia]*ldw::CV.v[p][ia];
-      for (int ib = 0; ib < 3; ib++) {
-	double dab = (ia == ib);
-	sdotq += (ldw::CV.v[p][ia]*ldw::CV.v[p][ib] - cs2*dab)*u[ia]*u[ib];
-      }
+double ldw::api::equilibrium_f(int p, double rho, const double u[3]) {
+  double cs2 = (1.0/3.0);
+  double rcs2 = 1.0/cs2;
+  double udotc = 0.0;
+  double sdotq = 0.0;
+  for (int ia = 0; ia < 3; ia++) {
+    udotc += u[ia]*ldw::CV.v[p][ia];
+    for (int ib = 0; ib < 3; ib++) {
+      double dab = (ia == ib);
+      sdotq += (ldw::CV.v[p][ia]*ldw::CV.v[p][ib] - cs2*dab)*u[ia]*u[ib];
     }
-    lb->f[(int64_t) p*lb->nsite + index]
-      = rho*ldw::weight(p)*(1.0 + rcs2*udotc + 0.5*rcs2*rcs2*sdotq);
   }
+  return rho*ldw::weight(p)*(1.0 + rcs2*udotc + 0.5*rcs2*rcs2*sdotq);
+}
+
+extern "C" int lb_1st_moment_equilib_set(lb_t * lb, int index, double rho, double u[3]) {
+  for (int p = 0; p < 19; p++)
+    lb->f[(int64_t) p*lb->nsite + index] = ldw::api::equilibrium_f(p, rho, u);
   return 0;
 }
 
@@ -1344,24 +1396,20 @@ extern "C" int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi, hydro
 /* ------------------------------------------------------------------------ */
 
 /* Relaxation rates as collision.c:1163-1246 and the per-site helpers
- * (collision.c:1255-1420) with a fixed viscosity */
-static LbParams lb_params_of(lb_t * lb, const double fextra[3]) {
+ * (collision.c:1255-1420) with fixed viscosities */
+LbParams ldw::api::lb_params(double rho0, double eta, double eta_nu, int relax,
+			     const double fglobal[3]) {
   LbParams p;
   memset(&p, 0, sizeof(p));
-  physics_t * phys = nullptr;
-  physics_ref(&phys);
   const double cs2 = (1.0/3.0);
-  const double rho0 = phys->rho0;
-  const double eta = phys->eta_shear;
-  const double eta_nu = phys->eta_bulk;
   const double rtau_shear = 1.0/(0.5 + eta/(rho0*cs2));
   p.rtau_shear = rtau_shear;
-  p.relax = (int) lb->nrelax;
-  if (lb->nrelax == LB_RELAXATION_M10) {
+  p.relax = relax;
+  if (relax == LB_RELAXATION_M10) {
     p.rtau_bulk = 1.0/(0.5 + eta_nu/(rho0*cs2));
     for (int m = 10; m < 19; m++) p.rtau_ghost[m] = 1.0;
   }
-  else if (lb->nrelax == LB_RELAXATION_BGK) {
+  else if (relax == LB_RELAXATION_BGK) {
     p.rtau_bulk = 1.0/(0.5 + eta/(rho0*cs2));
     for (int m = 10; m < 19; m++) p.rtau_ghost[m] = rtau_shear;
   }
@@ -1374,14 +1422,23 @@ static LbParams lb_params_of(lb_t * lb, const double fextra[3]) {
     p.rtau_ghost[11] = rg; p.rtau_ghost[12] = rg; p.rtau_ghost[13] = rg;
     p.rtau_ghost[15] = rg; p.rtau_ghost[16] = rg; p.rtau_ghost[17] = rg;
   }
-  // collision.c:1929-1940: force_global = fbody + pulsatile (zero); the
-  // site force is force_global + hydro force
-  for (int a = 0; a < 3; a++) {
-    p.fglobal[a] = phys->fbody[a];
-    if (fextra) p.fglobal[a] = p.fglobal[a] + fextra[a];
-  }
+  for (int a = 0; a < 3; a++) p.fglobal[a] = fglobal ? fglobal[a] : 0.0;
   p.rho0 = rho0;
   return p;
+}
+
+/* The lattice's collision parameters from the physics object */
+static LbParams lb_params_of(lb_t * lb, const double fextra[3]) {
+  physics_t * phys = nullptr;
+  physics_ref(&phys);
+  // collision.c:1929-1940: force_global = fbody + pulsatile (zero); the
+  // site force is force_global + hydro force
+  double fg[3];
+  for (int a = 0; a < 3; a++) {
+    fg[a] = phys->fbody[a];
+    if (fextra) fg[a] = fg[a] + fextra[a];
+  }
+  return ldw::api::lb_params(phys->rho0, phys->eta_shear, phys->eta_bulk, (int) lb->nrelax, fg);
 }
 
 /* collision.c:143-162 */
@@ -1620,6 +1677,13 @@ static void site_state(int seed, int ig, int jg, int kg, unsigned int s[4]) {
   s[1] = ns_uniform(local);
   s[2] = ns_uniform(local);
   s[3] = ns_uniform(local);
+}
+
+double ldw::api::spinodal_value(int seed, int ic, int jc, int kc, double phi0, double amp) {
+  unsigned int s[4];
+  site_state(seed, ic, jc, kc, s);
+  double ran = (1.0/4294967295u)*ns_uniform(s);
+  return phi0 + amp*(ran - 0.5);
 }
 
 extern "C" int field_phi_init_spinodal(field_t * phi, int seed, double phi0, double amp) {
