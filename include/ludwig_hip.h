@@ -450,6 +450,59 @@ int ldw_ensemble_f_read(ldw_ensemble_t * ens, int r, double * f);
 int ldw_ensemble_snapshot_device(ldw_ensemble_t * ens, int r, float * dst);
 /* Compensated FP64 sums of replica r: out[0] = total density, out[1] = sum phi */
 int ldw_ensemble_conserved_sums(ldw_ensemble_t * ens, int r, double out[2]);
+/* Replace replica r's phi and u by src (device memory, 4 nsite floats:
+ * phi, ux, uy, uz) and its distributions by the equilibrium at the current
+ * density and the new velocity (a surrogate jump) */
+int ldw_ensemble_load_device(ldw_ensemble_t * ens, int r, const float * src);
+
+/* ------------------------------------------------------------------------ */
+/* Surrogate network kernels (additive; no counterpart in the reference)    */
+/* ------------------------------------------------------------------------ */
+
+/* 3 x 3 x 3 convolutions with periodic padding on the MFMA units: BF16
+ * operands (raw 16-bit arrays), FP32 accumulation and results.  Device
+ * pointers; activations channels-last, x[p][c], p = ((b nx + x) ny + y) nz + z;
+ * weights w[27][cout][cin] (tap t = 9 (dx+1) + 3 (dy+1) + (dz+1)); channel
+ * counts multiples of 16.  accumulate = 1 adds to the output.
+ *   y  = bias + sum_t w_t x(p + o_t)                      (ldw_conv3d_fwd)
+ *   dx = sum_t w_t^T dy(p - o_t), wd from weights_prep      (ldw_conv3d_bwd_data)
+ *   dw_t = sum_p dy(p) x(p + o_t)^T, cin, cout <= 160        (ldw_conv3d_bwd_weight) */
+int ldw_conv3d_fwd(const void * x, const void * w, const float * bias, float * y,
+		   int batch, int nx, int ny, int nz, int cin, int cout, int accumulate);
+int ldw_conv3d_bwd_data(const void * dy, const void * wd, float * dx, int batch, int nx,
+			int ny, int nz, int cin, int cout, int accumulate);
+int ldw_conv3d_bwd_weight(const void * x, const void * dy, float * dw, int batch, int nx,
+			  int ny, int nz, int cin, int cout, int accumulate);
+/* FP32 master weights w -> BF16 wf (same layout) and wd (flipped and
+ * transposed for ldw_conv3d_bwd_data); either output may be NULL */
+int ldw_conv3d_weights_prep(const float * w, void * wf, void * wd, int cin, int cout);
+
+/* Elementwise and reshaping kernels of the network (device pointers) */
+int ldw_nn_silu_fwd(const float * z, void * a, long long n);
+int ldw_nn_silu_bwd(const float * z, const float * da, float * dz, void * dzb, long long n);
+int ldw_nn_bias_grad(const float * dz, float * db, long long npos, int c, int accumulate);
+int ldw_nn_pool2_fwd(const void * in, void * out, int batch, int nx, int ny, int nz, int c);
+int ldw_nn_pool2_bwd(const float * dout, float * din, int batch, int nx, int ny, int nz, int c,
+		     int accumulate);
+int ldw_nn_up2_fwd(const void * in, void * out, int batch, int nx, int ny, int nz, int c);
+int ldw_nn_up2_bwd(const float * dout, float * din, int batch, int nx, int ny, int nz, int c);
+int ldw_nn_concat(const void * a, int ca, const void * b, int cb, void * out, long long npos);
+int ldw_nn_split(const float * d, int ca, int cb, float * da, float * db, long long npos,
+		 int acc_a, int acc_b);
+int ldw_nn_to_bf16(const float * a, void * b, long long n);
+/* input channels of one sample from 4 SoA planes (phi, ux, uy, uz):
+ * norm = {phi shift, phi scale, u scale, p0, p1, p2} (p: scaled parameters) */
+int ldw_nn_encode(const float * state, long long nsite, const float norm[6], void * out, int c);
+int ldw_nn_target(const float * now, const float * next, long long nsite, float * tgt,
+		  float dphi_scale, float du_scale);
+int ldw_nn_decode(const float * now, const float * y, int cy, long long nsite, float * out,
+		  float dphi_scale, float du_scale);
+/* mean square error of y's first 4 channels against tgt; gradient into dy
+ * (BF16); *loss (may be NULL) = the mean, compensated sum */
+int ldw_nn_mse(const float * y, int cy, const float * tgt, long long npos, void * dy,
+	       double * loss);
+int ldw_nn_adam(float * w, const float * g, float * m, float * v, long long n, float lr,
+		float b1, float b2, float eps, float wd, int step);
 
 #ifdef __cplusplus
 }

@@ -134,6 +134,27 @@ def lib():
         "ldw_copy_calibration": [ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
         "ldw_device_alloc": [ctypes.c_longlong, pvp], "ldw_device_free": [vp],
         "ldw_memcpy": [vp, vp, ctypes.c_longlong, ctypes.c_int],
+        # surrogate network kernels (device pointers)
+        "ldw_conv3d_fwd": [vp, vp, vp, vp] + [ctypes.c_int] * 7,
+        "ldw_conv3d_bwd_data": [vp, vp, vp] + [ctypes.c_int] * 7,
+        "ldw_conv3d_bwd_weight": [vp, vp, vp] + [ctypes.c_int] * 7,
+        "ldw_conv3d_weights_prep": [vp, vp, vp, ctypes.c_int, ctypes.c_int],
+        "ldw_nn_silu_fwd": [vp, vp, ctypes.c_longlong],
+        "ldw_nn_silu_bwd": [vp, vp, vp, vp, ctypes.c_longlong],
+        "ldw_nn_bias_grad": [vp, vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int],
+        "ldw_nn_pool2_fwd": [vp, vp] + [ctypes.c_int] * 5,
+        "ldw_nn_pool2_bwd": [vp, vp] + [ctypes.c_int] * 6,
+        "ldw_nn_up2_fwd": [vp, vp] + [ctypes.c_int] * 5,
+        "ldw_nn_up2_bwd": [vp, vp] + [ctypes.c_int] * 5,
+        "ldw_nn_concat": [vp, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_longlong],
+        "ldw_nn_split": [vp, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_longlong, ctypes.c_int,
+                         ctypes.c_int],
+        "ldw_nn_to_bf16": [vp, vp, ctypes.c_longlong],
+        "ldw_nn_encode": [vp, ctypes.c_longlong, ctypes.POINTER(ctypes.c_float), vp, ctypes.c_int],
+        "ldw_nn_target": [vp, vp, ctypes.c_longlong, vp, ctypes.c_float, ctypes.c_float],
+        "ldw_nn_decode": [vp, vp, ctypes.c_int, ctypes.c_longlong, vp, ctypes.c_float, ctypes.c_float],
+        "ldw_nn_mse": [vp, ctypes.c_int, vp, ctypes.c_longlong, vp, ctypes.POINTER(ctypes.c_double)],
+        "ldw_nn_adam": [vp, vp, vp, vp, ctypes.c_longlong] + [ctypes.c_float] * 5 + [ctypes.c_int],
     }
     for name, args in sig.items():
         f = getattr(L, name)

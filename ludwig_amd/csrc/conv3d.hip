@@ -1,0 +1,294 @@
+// conv3d.hip -- 3 x 3 x 3 convolutions on periodic lattices as implicit
+// GEMMs on the MFMA units (BF16 operands, FP32 accumulation), for the
+// learned surrogate's network.
+//
+// Layout: activations are channels-last, x[p][c] with p = ((b*nx + x)*ny + y)*nz + z
+// over the batch, so the 27 taps of a position are 27 contiguous channel
+// rows.  Padding is periodic (the lattice is periodic), so no halo exists
+// and the adjoint of a convolution is again a convolution.  Channel
+// counts are multiples of 16.
+//
+// Forward, y = b + sum_t W_t x(p + o_t):  GEMM with M = positions,
+//   N = Cout, K = 27 Cin.  A wave owns 32 positions x 16*NT16 output
+//   channels: 2 x NT16 tiles of v_mfma_f32_16x16x16_bf16.  The A fragment
+//   of a lane is 4 consecutive channels of one (shifted) position and the
+//   B fragment 4 consecutive input channels of one output channel, so both
+//   are single 8-byte loads with weights stored as W[t][cout][cin].
+// Backward data, dx = sum_t W_{26-t}^T dy(p + o_t): the same kernel with the
+//   weights flipped and transposed (conv3_weights_prep).
+// Backward weight, dW_t = sum_p dy(p) x(p + o_t)^T:  GEMM with M = Cout,
+//   N = Cin, K = positions; 64-position slices of dy and of the shifted x
+//   are staged through LDS (contiguous rows) and read as fragments;
+//   position chunks give partial sums that a second kernel adds in a fixed
+//   order (deterministic).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace ldw {
+
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int wrapi(int i, int n) {
+  return (i < 0) ? i + n : (i >= n ? i - n : i);
+}
+
+struct Pos { int b, x, y, z; };
+
+__device__ __forceinline__ Pos pos_of(int64_t p, int nx, int ny, int nz) {
+  Pos q;
+  q.z = (int) (p % nz); p /= nz;
+  q.y = (int) (p % ny); p /= ny;
+  q.x = (int) (p % nx); p /= nx;
+  q.b = (int) p;
+  return q;
+}
+
+__device__ __forceinline__ int64_t shifted(const Pos & q, int t, int nx, int ny, int nz) {
+  const int dx = t/9 - 1, dy = (t/3)%3 - 1, dz = t%3 - 1;
+  return (((int64_t) q.b*nx + wrapi(q.x + dx, nx))*ny + wrapi(q.y + dy, ny))*nz + wrapi(q.z + dz, nz);
+}
+
+template <int NT16>
+__global__ void __launch_bounds__(256)
+conv3_fwd_kernel(const uint16_t * __restrict__ x, const uint16_t * __restrict__ w,
+		 const float * __restrict__ bias, float * __restrict__ y, int64_t npos,
+		 int nx, int ny, int nz, int cin, int cout, int accumulate) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, kq = lane >> 4;
+  const int64_t m0 = (int64_t) blockIdx.x*128 + wave*32;
+  const int n0 = blockIdx.y*16*NT16;
+
+  Pos q[2];
+  bool ok[2];
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++) {
+    const int64_t p = m0 + 16*rb + r;
+    ok[rb] = (p < npos);
+    q[rb] = pos_of(ok[rb] ? p : 0, nx, ny, nz);
+  }
+  f4 acc[2][NT16];
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int nb = 0; nb < NT16; nb++) acc[rb][nb] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  for (int t = 0; t < 27; t++) {
+    const int64_t s0 = shifted(q[0], t, nx, ny, nz)*cin + 4*kq;
+    const int64_t s1 = shifted(q[1], t, nx, ny, nz)*cin + 4*kq;
+    const uint16_t * wt = w + ((int64_t) t*cout + n0 + r)*cin + 4*kq;
+    for (int c0 = 0; c0 < cin; c0 += 16) {
+      s4 a0 = ok[0] ? *reinterpret_cast<const s4 *>(x + s0 + c0) : s4{0, 0, 0, 0};
+      s4 a1 = ok[1] ? *reinterpret_cast<const s4 *>(x + s1 + c0) : s4{0, 0, 0, 0};
+#pragma unroll
+      for (int nb = 0; nb < NT16; nb++) {
+	const s4 b = *reinterpret_cast<const s4 *>(wt + (int64_t) 16*nb*cin + c0);
+	acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b, acc[0][nb], 0, 0, 0);
+	acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b, acc[1][nb], 0, 0, 0);
+      }
+    }
+  }
+  // D layout: lane holds rows 4*kq + i of column r
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int nb = 0; nb < NT16; nb++) {
+      const int ch = n0 + 16*nb + r;
+      const float bv = bias ? bias[ch] : 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+	const int64_t p = m0 + 16*rb + 4*kq + i;
+	if (p < npos) {
+	  float v = acc[rb][nb][i] + bv;
+	  if (accumulate) v += y[p*cout + ch];
+	  y[p*cout + ch] = v;
+	}
+      }
+    }
+}
+
+constexpr int WG_P = 64;          // positions per LDS slice
+constexpr int WG_CMAX = 160;      // channel capacity of the LDS rows (host checks)
+
+__global__ void __launch_bounds__(256)
+conv3_wgrad_kernel(const uint16_t * __restrict__ x, const uint16_t * __restrict__ dy,
+		   float * __restrict__ part, int64_t npos, int64_t chunk,
+		   int nx, int ny, int nz, int cin, int cout) {
+  __shared__ uint16_t sy[WG_P][WG_CMAX + 4];
+  __shared__ uint16_t sx[WG_P][WG_CMAX + 4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, kq = lane >> 4;
+  const int t = blockIdx.y;
+  const int64_t pbeg = (int64_t) blockIdx.x*chunk;
+  const int64_t pend = (pbeg + chunk < npos) ? pbeg + chunk : npos;
+  const int nco = cout/16, nci = cin/16;
+  const int npair = nco*nci;
+  // this block's pairs: [16*blockIdx.z, 16*blockIdx.z + 16); wave: 4 of them
+  int pco[4], pci[4];
+  bool has[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int pr = 16*blockIdx.z + 4*wave + j;
+    has[j] = (pr < npair);
+    pco[j] = has[j] ? pr / nci : 0;
+    pci[j] = has[j] ? pr % nci : 0;
+  }
+  f4 acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) acc[j] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  for (int64_t p0 = pbeg; p0 < pend; p0 += WG_P) {
+    // stage 64 rows of dy and of x shifted by tap t (4 channels per load)
+    const int ly = cout/4, lx = cin/4;
+    for (int i = threadIdx.x; i < WG_P*ly; i += 256) {
+      const int row = i / ly, c = 4*(i - row*ly);
+      const int64_t p = p0 + row;
+      uint64_t v = 0;
+      if (p < pend) v = *reinterpret_cast<const uint64_t *>(dy + p*cout + c);
+      *reinterpret_cast<uint64_t *>(&sy[row][c]) = v;
+    }
+    for (int i = threadIdx.x; i < WG_P*lx; i += 256) {
+      const int row = i / lx, c = 4*(i - row*lx);
+      const int64_t p = p0 + row;
+      uint64_t v = 0;
+      if (p < pend) {
+	const Pos q = pos_of(p, nx, ny, nz);
+	v = *reinterpret_cast<const uint64_t *>(x + shifted(q, t, nx, ny, nz)*cin + c);
+      }
+      *reinterpret_cast<uint64_t *>(&sx[row][c]) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < WG_P/16; ks++) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+	if (!has[j]) continue;
+	s4 a, b;
+#pragma unroll
+	for (int i = 0; i < 4; i++) {
+	  a[i] = (short) sy[16*ks + 4*kq + i][16*pco[j] + r];
+	  b[i] = (short) sx[16*ks + 4*kq + i][16*pci[j] + r];
+	}
+	acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // partial[chunk][t][co][ci]
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (!has[j]) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int co = 16*pco[j] + 4*kq + i, ci = 16*pci[j] + r;
+      part[(((int64_t) blockIdx.x*27 + t)*cout + co)*cin + ci] = acc[j][i];
+    }
+  }
+}
+
+__global__ void wgrad_reduce_kernel(const float * __restrict__ part, float * __restrict__ dw,
+				    int nchunk, int64_t n, int accumulate) {
+  for (int64_t i = blockIdx.x*(int64_t) blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x*blockDim.x) {
+    float s = 0.0f;
+    for (int c = 0; c < nchunk; c++) s += part[(int64_t) c*n + i];
+    dw[i] = accumulate ? dw[i] + s : s;
+  }
+}
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t) ((u >> 16) | 0x40u);   // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);                                              // nearest even
+  return (uint16_t) (u >> 16);
+}
+
+// wf[t][co][ci] = bf16(w[t][co][ci]); wd[t][ci][co] = bf16(w[26 - t][co][ci])
+__global__ void weights_prep_kernel(const float * __restrict__ w, uint16_t * __restrict__ wf,
+				    uint16_t * __restrict__ wd, int cin, int cout) {
+  const int64_t n = (int64_t) 27*cout*cin;
+  for (int64_t i = blockIdx.x*(int64_t) blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x*blockDim.x) {
+    const int ci = (int) (i % cin);
+    const int co = (int) ((i / cin) % cout);
+    const int t = (int) (i / ((int64_t) cin*cout));
+    const uint16_t v = to_bf16(w[i]);
+    if (wf) wf[i] = v;
+    if (wd) wd[((int64_t) (26 - t)*cin + ci)*cout + co] = v;
+  }
+}
+
+__global__ void to_bf16_kernel(const float * __restrict__ a, uint16_t * __restrict__ b, int64_t n) {
+  for (int64_t i = blockIdx.x*(int64_t) blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x*blockDim.x)
+    b[i] = to_bf16(a[i]);
+}
+
+int grid_for(int64_t n) {
+  int64_t g = (n + 255)/256;
+  return (int) (g < 8192 ? (g > 0 ? g : 1) : 8192);
+}
+
+}  // namespace
+
+hipError_t launch_conv3_fwd(const uint16_t * x, const uint16_t * w, const float * bias, float * y,
+			    int batch, int nx, int ny, int nz, int cin, int cout, int accumulate,
+			    hipStream_t s) {
+  const int64_t npos = (int64_t) batch*nx*ny*nz;
+  const int nb16 = cout/16;
+  const int nt = (nb16 % 4 == 0) ? 4 : (nb16 % 3 == 0) ? 3 : (nb16 % 2 == 0) ? 2 : 1;
+  dim3 grid((unsigned) ((npos + 127)/128), nb16/nt), block(256);
+  switch (nt) {
+  case 4: hipLaunchKernelGGL(conv3_fwd_kernel<4>, grid, block, 0, s, x, w, bias, y, npos, nx, ny, nz, cin, cout, accumulate); break;
+  case 3: hipLaunchKernelGGL(conv3_fwd_kernel<3>, grid, block, 0, s, x, w, bias, y, npos, nx, ny, nz, cin, cout, accumulate); break;
+  case 2: hipLaunchKernelGGL(conv3_fwd_kernel<2>, grid, block, 0, s, x, w, bias, y, npos, nx, ny, nz, cin, cout, accumulate); break;
+  default: hipLaunchKernelGGL(conv3_fwd_kernel<1>, grid, block, 0, s, x, w, bias, y, npos, nx, ny, nz, cin, cout, accumulate); break;
+  }
+  return hipGetLastError();
+}
+
+int64_t conv3_wgrad_workspace(int batch, int nx, int ny, int nz, int cin, int cout, int * nchunk,
+			      int64_t * chunk) {
+  const int64_t npos = (int64_t) batch*nx*ny*nz;
+  const int groups = (cout/16*(cin/16) + 15)/16;
+  // about 2048 blocks in all; chunks are whole 64-position slices
+  int64_t nc = 2048/(27*groups);
+  if (nc < 1) nc = 1;
+  int64_t ch = (npos + nc - 1)/nc;
+  ch = ((ch + WG_P - 1)/WG_P)*WG_P;
+  nc = (npos + ch - 1)/ch;
+  *nchunk = (int) nc;
+  *chunk = ch;
+  return nc*27*(int64_t) cout*cin;
+}
+
+hipError_t launch_conv3_wgrad(const uint16_t * x, const uint16_t * dy, float * dw, float * work,
+			      int batch, int nx, int ny, int nz, int cin, int cout, int accumulate,
+			      hipStream_t s) {
+  int nchunk;
+  int64_t chunk;
+  conv3_wgrad_workspace(batch, nx, ny, nz, cin, cout, &nchunk, &chunk);
+  const int64_t npos = (int64_t) batch*nx*ny*nz;
+  const int groups = (cout/16*(cin/16) + 15)/16;
+  hipLaunchKernelGGL(conv3_wgrad_kernel, dim3(nchunk, 27, groups), dim3(256), 0, s,
+		     x, dy, work, npos, chunk, nx, ny, nz, cin, cout);
+  const int64_t n = (int64_t) 27*cout*cin;
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(grid_for(n)), dim3(256), 0, s, work, dw, nchunk, n, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_conv3_weights_prep(const float * w, uint16_t * wf, uint16_t * wd, int cin, int cout,
+				     hipStream_t s) {
+  const int64_t n = (int64_t) 27*cout*cin;
+  hipLaunchKernelGGL(weights_prep_kernel, dim3(grid_for(n)), dim3(256), 0, s, w, wf, wd, cin, cout);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_bf16(const float * a, uint16_t * b, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(grid_for(n)), dim3(256), 0, s, a, b, n);
+  return hipGetLastError();
+}
+
+}  // namespace ldw

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "d3q19.h"
 #include "step_tile.h"
 
 namespace ldw {
@@ -78,6 +79,30 @@ template <typename T>
 __global__ void to_f32_kernel(const T * __restrict__ a, float * __restrict__ b, int64_t n) {
   for (int64_t i = blockIdx.x*(int64_t) blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x*blockDim.x)
     b[i] = (float) a[i];
+}
+
+// phi, u of one replica from 4 planes of floats; distributions set to the
+// equilibrium at the site's density and the new velocity (stored shifted by
+// w rho0 when T is float)
+template <typename T>
+__global__ void set_state_kernel(const float * __restrict__ src, int64_t n, T * __restrict__ phi,
+				 T * __restrict__ u, T * __restrict__ f, const T * __restrict__ rho,
+				 double rho0) {
+  for (int64_t s = blockIdx.x*(int64_t) blockDim.x + threadIdx.x; s < n; s += (int64_t) gridDim.x*blockDim.x) {
+    const double ux = src[n + s], uy = src[2*n + s], uz = src[3*n + s];
+    const bool shifted = (sizeof(T) == 4);
+    const double r = (double) rho[s];           // the full density in both storages
+    phi[s] = (T) src[s];
+    u[s] = (T) ux; u[n + s] = (T) uy; u[2*n + s] = (T) uz;
+    const double uu = ux*ux + uy*uy + uz*uz;
+#pragma unroll
+    for (int p = 0; p < NVEL; p++) {
+      const double cu = CV.v[p][0]*ux + CV.v[p][1]*uy + CV.v[p][2]*uz;
+      const double w = weight(p);
+      const double feq = r*w*(1.0 + 3.0*cu + 4.5*cu*cu - 1.5*uu);
+      f[p*n + s] = (T) (shifted ? feq - w*rho0 : feq);
+    }
+  }
 }
 
 template <typename T, class S, int ORDER, int METHOD>
@@ -144,6 +169,17 @@ hipError_t launch_ensemble_step_f32(int shape, int order, int method, int nblock
 				    const float * g_in, float * g_out, hipStream_t s) {
   return launch<float>(shape, order, method, nblocks, reps, items, nitems, ctr, phi, phi_out,
 		       u_prev, u_next, rho, g_in, g_out, s);
+}
+
+void launch_ensemble_set_state(const float * src, int64_t n, int precision, void * phi, void * u,
+			       void * f, const void * rho, double rho0, hipStream_t s) {
+  const int nb = (int) ((n + 255)/256 < 4096 ? (n + 255)/256 : 4096);
+  if (precision == 64)
+    hipLaunchKernelGGL(set_state_kernel<double>, dim3(nb), dim3(256), 0, s, src, n, (double *) phi,
+		       (double *) u, (double *) f, (const double *) rho, rho0);
+  else
+    hipLaunchKernelGGL(set_state_kernel<float>, dim3(nb), dim3(256), 0, s, src, n, (float *) phi,
+		       (float *) u, (float *) f, (const float *) rho, rho0);
 }
 
 void launch_to_f32(const void * a, int precision, float * b, int64_t n, hipStream_t s) {

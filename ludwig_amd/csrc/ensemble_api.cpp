@@ -481,6 +481,20 @@ extern "C" int ldw_ensemble_snapshot_device(ldw_ensemble_t * e, int r, float * d
   return 0;
 }
 
+extern "C" int ldw_ensemble_load_device(ldw_ensemble_t * e, int r, const float * src) {
+  if (check_rep(e, r)) return -1;
+  if (src == nullptr) LDW_FAIL("ldw_ensemble_load_device: null source");
+  const EnsRep & R = e->reps[r];
+  const int64_t n = R.g.nsite;
+  hipStream_t st = ldw::api::stream();
+  const int c = e->cur;
+  ldw::launch_ensemble_set_state(src, n, e->prec, (char *) e->d_phi[c] + R.off_s*e->es,
+				 (char *) e->d_u[c] + 3*R.off_s*e->es, (char *) e->d_f[c] + R.off_f*e->es,
+				 (const char *) e->d_rho + R.off_s*e->es, e->spec[r].rho0, st);
+  LDW_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 extern "C" int ldw_ensemble_conserved_sums(ldw_ensemble_t * e, int r, double out[2]) {
   if (check_rep(e, r)) return -1;
   const EnsRep & R = e->reps[r];

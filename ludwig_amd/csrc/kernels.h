@@ -152,8 +152,68 @@ hipError_t launch_ensemble_step_f32(int shape, int order, int method, int nblock
 				    const float * u_prev, float * u_next, float * rho,
 				    const float * g_in, float * g_out, hipStream_t s);
 
+// Replace one replica's phi and u by src (4 planes of n floats: phi, ux,
+// uy, uz) and its distributions by the equilibrium at (rho, u); FP32
+// storage subtracts w rho0
+void launch_ensemble_set_state(const float * src, int64_t n, int precision, void * phi, void * u,
+			       void * f, const void * rho, double rho0, hipStream_t s);
+
 // b[i] = (float) a[i] for n values of precision 64 (double) or 32 (float)
 void launch_to_f32(const void * a, int precision, float * b, int64_t n, hipStream_t s);
+
+// 3x3x3 periodic convolutions, BF16 operands (raw 16-bit), FP32 results
+// (conv3d.hip).  Channels-last activations; channel counts multiples of 16,
+// at most 160 for the weight gradient.  Weights W[27][cout][cin].
+hipError_t launch_conv3_fwd(const uint16_t * x, const uint16_t * w, const float * bias, float * y,
+			    int batch, int nx, int ny, int nz, int cin, int cout, int accumulate,
+			    hipStream_t s);
+// floats of workspace for launch_conv3_wgrad
+int64_t conv3_wgrad_workspace(int batch, int nx, int ny, int nz, int cin, int cout, int * nchunk,
+			      int64_t * chunk);
+hipError_t launch_conv3_wgrad(const uint16_t * x, const uint16_t * dy, float * dw, float * work,
+			      int batch, int nx, int ny, int nz, int cin, int cout, int accumulate,
+			      hipStream_t s);
+// wf = bf16(w) (same layout), wd[t][ci][co] = bf16(w[26-t][co][ci]) for the
+// backward-data pass; either output may be null
+hipError_t launch_conv3_weights_prep(const float * w, uint16_t * wf, uint16_t * wd, int cin, int cout,
+				     hipStream_t s);
+
+// Surrogate network elementwise kernels (nn_ops.hip)
+hipError_t nn_silu_fwd(const float * z, uint16_t * a, int64_t n, hipStream_t s);
+hipError_t nn_silu_bwd(const float * z, const float * da, float * dz, uint16_t * dzb, int64_t n,
+		       hipStream_t s);
+int64_t nn_bias_grad_workspace(int64_t npos, int c);
+hipError_t nn_bias_grad(const float * dz, float * db, float * work, int64_t npos, int c,
+			int accumulate, hipStream_t s);
+hipError_t nn_pool2_fwd(const uint16_t * in, uint16_t * out, int batch, int nx, int ny, int nz, int c,
+			hipStream_t s);
+hipError_t nn_pool2_bwd(const float * dout, float * din, int batch, int nx, int ny, int nz, int c,
+			int accumulate, hipStream_t s);
+hipError_t nn_up2_fwd(const uint16_t * in, uint16_t * out, int batch, int nx, int ny, int nz, int c,
+		      hipStream_t s);
+hipError_t nn_up2_bwd(const float * dout, float * din, int batch, int nx, int ny, int nz, int c,
+		      hipStream_t s);
+hipError_t nn_concat(const uint16_t * a, int ca, const uint16_t * b, int cb, uint16_t * out,
+		     int64_t npos, hipStream_t s);
+hipError_t nn_split(const float * d, int ca, int cb, float * da, float * db, int64_t npos,
+		    int acc_a, int acc_b, hipStream_t s);
+hipError_t nn_to_bf16(const float * a, uint16_t * b, int64_t n, hipStream_t s);
+// network input scaling: phi -> (phi - phi_shift) phi_scale, u -> u u_scale,
+// prm: the three scalar channels (already scaled)
+struct EncodeNorm {
+  float phi_shift, phi_scale, u_scale;
+  float prm[3];
+};
+hipError_t nn_encode(const float * state, int64_t nsite, const EncodeNorm & e, uint16_t * out, int c,
+		     hipStream_t s);
+hipError_t nn_target(const float * now, const float * next, int64_t nsite, float * tgt,
+		     float dphi_scale, float du_scale, hipStream_t s);
+hipError_t nn_decode(const float * now, const float * y, int cy, int64_t nsite, float * out,
+		     float dphi_scale, float du_scale, hipStream_t s);
+hipError_t nn_mse(const float * y, int cy, const float * tgt, int64_t npos, float * err, uint16_t * dy,
+		  hipStream_t s);
+hipError_t nn_adam(float * w, const float * g, float * m, float * v, int64_t n, float lr_t,
+		   float b1, float b2, float eps, float wd, hipStream_t s);
 
 // Plain device copy of nbytes (multiple of 32) for bandwidth calibration
 void launch_copy(const void * src, void * dst, int64_t nbytes, hipStream_t s);

@@ -1,0 +1,355 @@
+// nn_ops.hip -- the elementwise and reshaping kernels of the surrogate
+// network around the MFMA convolutions (conv3d.hip): activation, 2x
+// average pooling and nearest upsampling on periodic lattices, channel
+// concatenation, bias gradients, field encoding/decoding, the loss and
+// the Adam update.  All arrays are channels-last (position-major), with
+// p = ((b*nx + x)*ny + y)*nz + z; BF16 values are raw 16-bit patterns.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace ldw {
+
+namespace {
+
+__device__ __forceinline__ uint16_t bf16_of(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t) ((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t) (u >> 16);
+}
+
+__device__ __forceinline__ float f32_of(uint16_t h) {
+  return __uint_as_float(((uint32_t) h) << 16);
+}
+
+#define GRID_STRIDE(i, n) \
+  for (int64_t i = blockIdx.x*(int64_t) blockDim.x + threadIdx.x; i < (n); i += (int64_t) gridDim.x*blockDim.x)
+
+int grid_of(int64_t n) {
+  int64_t g = (n + 255)/256;
+  return (int) (g < 16384 ? (g > 0 ? g : 1) : 16384);
+}
+
+// SiLU: a = z sigma(z)
+__global__ void silu_fwd_kernel(const float * __restrict__ z, uint16_t * __restrict__ a, int64_t n) {
+  GRID_STRIDE(i, n) {
+    const float v = z[i];
+    a[i] = bf16_of(v/(1.0f + expf(-v)));
+  }
+}
+
+// dz = da sigma(z) (1 + z (1 - sigma(z)))
+__global__ void silu_bwd_kernel(const float * __restrict__ z, const float * __restrict__ da,
+				float * __restrict__ dz, uint16_t * __restrict__ dzb, int64_t n) {
+  GRID_STRIDE(i, n) {
+    const float v = z[i];
+    const float s = 1.0f/(1.0f + expf(-v));
+    const float g = da[i]*s*(1.0f + v*(1.0f - s));
+    if (dz) dz[i] = g;
+    dzb[i] = bf16_of(g);
+  }
+}
+
+// per-block partial sums of each channel over a slice of positions
+__global__ void bias_grad_kernel(const float * __restrict__ dz, float * __restrict__ part,
+				 int64_t npos, int c, int64_t chunk) {
+  const int64_t p0 = blockIdx.x*chunk;
+  const int64_t p1 = (p0 + chunk < npos) ? p0 + chunk : npos;
+  for (int ch = threadIdx.x; ch < c; ch += blockDim.x) {
+    float s = 0.0f;
+    for (int64_t p = p0; p < p1; p++) s += dz[p*c + ch];
+    part[(int64_t) blockIdx.x*c + ch] = s;
+  }
+}
+
+__global__ void sum_rows_kernel(const float * __restrict__ part, float * __restrict__ out,
+				int nrow, int c, int accumulate) {
+  for (int ch = blockIdx.x*blockDim.x + threadIdx.x; ch < c; ch += gridDim.x*blockDim.x) {
+    float s = 0.0f;
+    for (int r = 0; r < nrow; r++) s += part[(int64_t) r*c + ch];
+    out[ch] = accumulate ? out[ch] + s : s;
+  }
+}
+
+// coarse (nx/2, ny/2, nz/2) = mean of the 8 fine children
+__global__ void pool2_fwd_kernel(const uint16_t * __restrict__ in, uint16_t * __restrict__ out,
+				 int batch, int nx, int ny, int nz, int c) {
+  const int mx = nx/2, my = ny/2, mz = nz/2;
+  const int64_t n = (int64_t) batch*mx*my*mz*c;
+  GRID_STRIDE(i, n) {
+    const int ch = (int) (i % c);
+    int64_t q = i / c;
+    const int z = (int) (q % mz); q /= mz;
+    const int y = (int) (q % my); q /= my;
+    const int x = (int) (q % mx);
+    const int b = (int) (q / mx);
+    float s = 0.0f;
+    for (int d = 0; d < 8; d++) {
+      const int64_t f = (((int64_t) b*nx + 2*x + (d >> 2))*ny + 2*y + ((d >> 1) & 1))*nz + 2*z + (d & 1);
+      s += f32_of(in[f*c + ch]);
+    }
+    out[i] = bf16_of(0.125f*s);
+  }
+}
+
+// fine gradient = coarse gradient / 8 (nx, ny, nz: fine sizes)
+__global__ void pool2_bwd_kernel(const float * __restrict__ dout, float * __restrict__ din,
+				 int batch, int nx, int ny, int nz, int c, int accumulate) {
+  const int64_t n = (int64_t) batch*nx*ny*nz*c;
+  GRID_STRIDE(i, n) {
+    const int ch = (int) (i % c);
+    int64_t q = i / c;
+    const int z = (int) (q % nz); q /= nz;
+    const int y = (int) (q % ny); q /= ny;
+    const int x = (int) (q % nx);
+    const int b = (int) (q / nx);
+    const int64_t cp = (((int64_t) b*(nx/2) + x/2)*(ny/2) + y/2)*(nz/2) + z/2;
+    const float v = 0.125f*dout[cp*c + ch];
+    din[i] = accumulate ? din[i] + v : v;
+  }
+}
+
+// fine (2nx, 2ny, 2nz) = coarse parent (nx, ny, nz: coarse sizes)
+__global__ void up2_fwd_kernel(const uint16_t * __restrict__ in, uint16_t * __restrict__ out,
+			       int batch, int nx, int ny, int nz, int c) {
+  const int fx = 2*nx, fy = 2*ny, fz = 2*nz;
+  const int64_t n = (int64_t) batch*fx*fy*fz*c;
+  GRID_STRIDE(i, n) {
+    const int ch = (int) (i % c);
+    int64_t q = i / c;
+    const int z = (int) (q % fz); q /= fz;
+    const int y = (int) (q % fy); q /= fy;
+    const int x = (int) (q % fx);
+    const int b = (int) (q / fx);
+    const int64_t cp = (((int64_t) b*nx + x/2)*ny + y/2)*nz + z/2;
+    out[i] = in[cp*c + ch];
+  }
+}
+
+// coarse gradient = sum of the 8 fine children's gradients
+__global__ void up2_bwd_kernel(const float * __restrict__ dout, float * __restrict__ din,
+			       int batch, int nx, int ny, int nz, int c) {
+  const int64_t n = (int64_t) batch*nx*ny*nz*c;
+  const int fx = 2*nx, fy = 2*ny, fz = 2*nz;
+  GRID_STRIDE(i, n) {
+    const int ch = (int) (i % c);
+    int64_t q = i / c;
+    const int z = (int) (q % nz); q /= nz;
+    const int y = (int) (q % ny); q /= ny;
+    const int x = (int) (q % nx);
+    const int b = (int) (q / nx);
+    float s = 0.0f;
+    for (int d = 0; d < 8; d++) {
+      const int64_t f = (((int64_t) b*fx + 2*x + (d >> 2))*fy + 2*y + ((d >> 1) & 1))*fz + 2*z + (d & 1);
+      s += dout[f*c + ch];
+    }
+    din[i] = s;
+  }
+}
+
+__global__ void concat_kernel(const uint16_t * __restrict__ a, int ca, const uint16_t * __restrict__ b,
+			      int cb, uint16_t * __restrict__ out, int64_t npos) {
+  const int c = ca + cb;
+  GRID_STRIDE(i, npos*c) {
+    const int64_t p = i / c;
+    const int ch = (int) (i - p*c);
+    out[i] = (ch < ca) ? a[p*ca + ch] : b[p*cb + ch - ca];
+  }
+}
+
+__global__ void split_kernel(const float * __restrict__ d, int ca, int cb, float * __restrict__ da,
+			     float * __restrict__ db, int64_t npos, int acc_a, int acc_b) {
+  const int c = ca + cb;
+  GRID_STRIDE(i, npos*c) {
+    const int64_t p = i / c;
+    const int ch = (int) (i - p*c);
+    const float v = d[i];
+    if (ch < ca) { if (da) { float * o = da + p*ca + ch; *o = acc_a ? *o + v : v; } }
+    else { if (db) { float * o = db + p*cb + ch - ca; *o = acc_b ? *o + v : v; } }
+  }
+}
+
+__global__ void to_bf16_kernel(const float * __restrict__ a, uint16_t * __restrict__ b, int64_t n) {
+  GRID_STRIDE(i, n) b[i] = bf16_of(a[i]);
+}
+
+// network input of one sample: channels (phi, ux, uy, uz, p0, p1, p2) scaled,
+// the rest zero.  state: 4 planes of nsite (device-layout SoA).
+__global__ void encode_kernel(const float * __restrict__ state, int64_t nsite, EncodeNorm e,
+			      uint16_t * __restrict__ out, int c) {
+  GRID_STRIDE(i, nsite*c) {
+    const int64_t s = i / c;
+    const int ch = (int) (i - s*c);
+    float v = 0.0f;
+    if (ch == 0) v = (state[s] - e.phi_shift)*e.phi_scale;
+    else if (ch < 4) v = state[ch*nsite + s]*e.u_scale;
+    else if (ch < 7) v = e.prm[ch - 4];
+    out[i] = bf16_of(v);
+  }
+}
+
+// training target: scaled change of (phi, u) over the jump, channels-last [nsite][4]
+__global__ void target_kernel(const float * __restrict__ now, const float * __restrict__ next,
+			      int64_t nsite, float * __restrict__ tgt, float dphi_scale, float du_scale) {
+  GRID_STRIDE(i, nsite*4) {
+    const int64_t s = i / 4;
+    const int ch = (int) (i - s*4);
+    const float d = next[ch*nsite + s] - now[ch*nsite + s];
+    tgt[i] = d*(ch == 0 ? dphi_scale : du_scale);
+  }
+}
+
+// prediction: state + change, back to 4 SoA planes
+__global__ void decode_kernel(const float * __restrict__ now, const float * __restrict__ y, int cy,
+			      int64_t nsite, float * __restrict__ out, float dphi_scale, float du_scale) {
+  GRID_STRIDE(i, nsite*4) {
+    const int ch = (int) (i / nsite);
+    const int64_t s = i - (int64_t) ch*nsite;
+    out[i] = now[i] + y[s*cy + ch]/(ch == 0 ? dphi_scale : du_scale);
+  }
+}
+
+// mean-square error over the first 4 channels of y: squared errors into
+// err (for a compensated sum), gradient 2 (y - t)/(4 npos) into dy (BF16),
+// zero for the other channels
+__global__ void mse_kernel(const float * __restrict__ y, int cy, const float * __restrict__ tgt,
+			   int64_t npos, float * __restrict__ err, uint16_t * __restrict__ dy) {
+  const float scale = 2.0f/(4.0f*(float) npos);
+  GRID_STRIDE(i, npos*cy) {
+    const int64_t p = i / cy;
+    const int ch = (int) (i - p*cy);
+    float g = 0.0f;
+    if (ch < 4) {
+      const float e = y[i] - tgt[p*4 + ch];
+      err[p*4 + ch] = e*e;
+      g = scale*e;
+    }
+    dy[i] = bf16_of(g);
+  }
+}
+
+// Adam with bias correction folded into lr_t = lr sqrt(1 - b2^t)/(1 - b1^t)
+__global__ void adam_kernel(float * __restrict__ w, const float * __restrict__ g,
+			    float * __restrict__ m, float * __restrict__ v, int64_t n,
+			    float lr_t, float b1, float b2, float eps, float wd) {
+  GRID_STRIDE(i, n) {
+    const float gi = g[i];
+    const float mi = b1*m[i] + (1.0f - b1)*gi;
+    const float vi = b2*v[i] + (1.0f - b2)*gi*gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] = w[i] - lr_t*(mi/(sqrtf(vi) + eps) + wd*w[i]);
+  }
+}
+
+}  // namespace
+
+#define LAUNCH(k, n, ...) do { hipLaunchKernelGGL(k, dim3(grid_of(n)), dim3(256), 0, s, __VA_ARGS__); } while (0)
+
+hipError_t nn_silu_fwd(const float * z, uint16_t * a, int64_t n, hipStream_t s) {
+  LAUNCH(silu_fwd_kernel, n, z, a, n);
+  return hipGetLastError();
+}
+
+hipError_t nn_silu_bwd(const float * z, const float * da, float * dz, uint16_t * dzb, int64_t n,
+		       hipStream_t s) {
+  LAUNCH(silu_bwd_kernel, n, z, da, dz, dzb, n);
+  return hipGetLastError();
+}
+
+int64_t nn_bias_grad_workspace(int64_t npos, int c) {
+  (void) npos;
+  return (int64_t) 1024*c;
+}
+
+hipError_t nn_bias_grad(const float * dz, float * db, float * work, int64_t npos, int c,
+			int accumulate, hipStream_t s) {
+  int64_t nblk = 1024;
+  if (npos < nblk) nblk = npos > 0 ? npos : 1;
+  const int64_t chunk = (npos + nblk - 1)/nblk;
+  nblk = (npos + chunk - 1)/chunk;
+  if (nblk < 1) nblk = 1;
+  hipLaunchKernelGGL(bias_grad_kernel, dim3((unsigned) nblk), dim3(64), 0, s, dz, work, npos, c, chunk);
+  hipLaunchKernelGGL(sum_rows_kernel, dim3((c + 255)/256), dim3(256), 0, s, work, db, (int) nblk, c, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t nn_pool2_fwd(const uint16_t * in, uint16_t * out, int batch, int nx, int ny, int nz, int c,
+			hipStream_t s) {
+  const int64_t n = (int64_t) batch*(nx/2)*(ny/2)*(nz/2)*c;
+  LAUNCH(pool2_fwd_kernel, n, in, out, batch, nx, ny, nz, c);
+  return hipGetLastError();
+}
+
+hipError_t nn_pool2_bwd(const float * dout, float * din, int batch, int nx, int ny, int nz, int c,
+			int accumulate, hipStream_t s) {
+  const int64_t n = (int64_t) batch*nx*ny*nz*c;
+  LAUNCH(pool2_bwd_kernel, n, dout, din, batch, nx, ny, nz, c, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t nn_up2_fwd(const uint16_t * in, uint16_t * out, int batch, int nx, int ny, int nz, int c,
+		      hipStream_t s) {
+  const int64_t n = (int64_t) batch*8*nx*ny*nz*c;
+  LAUNCH(up2_fwd_kernel, n, in, out, batch, nx, ny, nz, c);
+  return hipGetLastError();
+}
+
+hipError_t nn_up2_bwd(const float * dout, float * din, int batch, int nx, int ny, int nz, int c,
+		      hipStream_t s) {
+  const int64_t n = (int64_t) batch*nx*ny*nz*c;
+  LAUNCH(up2_bwd_kernel, n, dout, din, batch, nx, ny, nz, c);
+  return hipGetLastError();
+}
+
+hipError_t nn_concat(const uint16_t * a, int ca, const uint16_t * b, int cb, uint16_t * out,
+		     int64_t npos, hipStream_t s) {
+  LAUNCH(concat_kernel, npos*(ca + cb), a, ca, b, cb, out, npos);
+  return hipGetLastError();
+}
+
+hipError_t nn_split(const float * d, int ca, int cb, float * da, float * db, int64_t npos,
+		    int acc_a, int acc_b, hipStream_t s) {
+  LAUNCH(split_kernel, npos*(ca + cb), d, ca, cb, da, db, npos, acc_a, acc_b);
+  return hipGetLastError();
+}
+
+hipError_t nn_to_bf16(const float * a, uint16_t * b, int64_t n, hipStream_t s) {
+  LAUNCH(to_bf16_kernel, n, a, b, n);
+  return hipGetLastError();
+}
+
+hipError_t nn_encode(const float * state, int64_t nsite, const EncodeNorm & e, uint16_t * out, int c,
+		     hipStream_t s) {
+  LAUNCH(encode_kernel, nsite*c, state, nsite, e, out, c);
+  return hipGetLastError();
+}
+
+hipError_t nn_target(const float * now, const float * next, int64_t nsite, float * tgt,
+		     float dphi_scale, float du_scale, hipStream_t s) {
+  LAUNCH(target_kernel, nsite*4, now, next, nsite, tgt, dphi_scale, du_scale);
+  return hipGetLastError();
+}
+
+hipError_t nn_decode(const float * now, const float * y, int cy, int64_t nsite, float * out,
+		     float dphi_scale, float du_scale, hipStream_t s) {
+  LAUNCH(decode_kernel, nsite*4, now, y, cy, nsite, out, dphi_scale, du_scale);
+  return hipGetLastError();
+}
+
+hipError_t nn_mse(const float * y, int cy, const float * tgt, int64_t npos, float * err, uint16_t * dy,
+		  hipStream_t s) {
+  LAUNCH(mse_kernel, npos*cy, y, cy, tgt, npos, err, dy);
+  return hipGetLastError();
+}
+
+hipError_t nn_adam(float * w, const float * g, float * m, float * v, int64_t n, float lr_t,
+		   float b1, float b2, float eps, float wd, hipStream_t s) {
+  LAUNCH(adam_kernel, n, w, g, m, v, n, lr_t, b1, b2, eps, wd);
+  return hipGetLastError();
+}
+
+}  // namespace ldw

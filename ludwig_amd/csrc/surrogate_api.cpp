@@ -1,0 +1,216 @@
+// surrogate_api.cpp -- C ABI of the surrogate network's device kernels
+// (ldw_conv3d_*, ldw_nn_*; include/ludwig_hip.h).  All arrays are device
+// pointers (from ldw_device_alloc); BF16 arrays are raw 16-bit patterns.
+// Work is ordered on the library's stream; a call that returns a host
+// value (ldw_nn_mse with a loss pointer) completes before it returns.
+// Scratch memory for partial sums is kept by the library and grown on
+// demand.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/ludwig_hip.h"
+#include "api_internal.h"
+#include "kernels.h"
+
+namespace {
+
+struct Scratch {
+  void * p = nullptr;
+  size_t bytes = 0;
+};
+
+Scratch g_wgrad, g_bias, g_err, g_sum;
+
+int scratch(Scratch & s, size_t bytes, void ** out) {
+  if (s.bytes < bytes) {
+    hipStream_t st = ldw::api::stream();
+    LDW_HIP_CHECK(hipStreamSynchronize(st));
+    if (s.p) LDW_HIP_CHECK(hipFree(s.p));
+    s.p = nullptr;
+    s.bytes = 0;
+    LDW_HIP_CHECK(hipMalloc(&s.p, bytes));
+    s.bytes = bytes;
+  }
+  *out = s.p;
+  return 0;
+}
+
+int check_conv(int batch, int nx, int ny, int nz, int cin, int cout) {
+  if (batch < 1 || nx < 1 || ny < 1 || nz < 1) LDW_FAIL("conv3d: bad sizes %d x %d x %d x %d", batch, nx, ny, nz);
+  if (cin < 16 || cout < 16 || cin % 16 || cout % 16)
+    LDW_FAIL("conv3d: channel counts must be multiples of 16 (cin %d, cout %d)", cin, cout);
+  return 0;
+}
+
+}  // namespace
+
+#define READY() do { if (ldw::api::device_init()) return -1; } while (0)
+#define KCHECK(call) LDW_HIP_CHECK(call)
+
+extern "C" int ldw_conv3d_fwd(const void * x, const void * w, const float * bias, float * y,
+			      int batch, int nx, int ny, int nz, int cin, int cout, int accumulate) {
+  READY();
+  if (check_conv(batch, nx, ny, nz, cin, cout)) return -1;
+  KCHECK(ldw::launch_conv3_fwd((const uint16_t *) x, (const uint16_t *) w, bias, y, batch, nx, ny, nz,
+			       cin, cout, accumulate, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_conv3d_bwd_data(const void * dy, const void * wd, float * dx, int batch, int nx,
+				   int ny, int nz, int cin, int cout, int accumulate) {
+  READY();
+  if (check_conv(batch, nx, ny, nz, cin, cout)) return -1;
+  // dx (cin channels) = convolution of dy (cout channels) with the flipped,
+  // transposed weights
+  KCHECK(ldw::launch_conv3_fwd((const uint16_t *) dy, (const uint16_t *) wd, nullptr, dx, batch, nx,
+			       ny, nz, cout, cin, accumulate, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_conv3d_bwd_weight(const void * x, const void * dy, float * dw, int batch, int nx,
+				     int ny, int nz, int cin, int cout, int accumulate) {
+  READY();
+  if (check_conv(batch, nx, ny, nz, cin, cout)) return -1;
+  if (cin > 160 || cout > 160) LDW_FAIL("conv3d weight gradient: at most 160 channels (cin %d, cout %d)", cin, cout);
+  int nchunk;
+  int64_t chunk;
+  const int64_t n = ldw::conv3_wgrad_workspace(batch, nx, ny, nz, cin, cout, &nchunk, &chunk);
+  void * work = nullptr;
+  if (scratch(g_wgrad, (size_t) n*sizeof(float), &work)) return -1;
+  KCHECK(ldw::launch_conv3_wgrad((const uint16_t *) x, (const uint16_t *) dy, dw, (float *) work, batch,
+				 nx, ny, nz, cin, cout, accumulate, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_conv3d_weights_prep(const float * w, void * wf, void * wd, int cin, int cout) {
+  READY();
+  if (cin < 1 || cout < 1) LDW_FAIL("ldw_conv3d_weights_prep: bad channel counts");
+  KCHECK(ldw::launch_conv3_weights_prep(w, (uint16_t *) wf, (uint16_t *) wd, cin, cout, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_silu_fwd(const float * z, void * a, long long n) {
+  READY();
+  KCHECK(ldw::nn_silu_fwd(z, (uint16_t *) a, n, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_silu_bwd(const float * z, const float * da, float * dz, void * dzb, long long n) {
+  READY();
+  KCHECK(ldw::nn_silu_bwd(z, da, dz, (uint16_t *) dzb, n, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_bias_grad(const float * dz, float * db, long long npos, int c, int accumulate) {
+  READY();
+  void * work = nullptr;
+  if (scratch(g_bias, (size_t) ldw::nn_bias_grad_workspace(npos, c)*sizeof(float), &work)) return -1;
+  KCHECK(ldw::nn_bias_grad(dz, db, (float *) work, npos, c, accumulate, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_pool2_fwd(const void * in, void * out, int batch, int nx, int ny, int nz, int c) {
+  READY();
+  if (nx % 2 || ny % 2 || nz % 2) LDW_FAIL("ldw_nn_pool2_fwd: odd lattice %d x %d x %d", nx, ny, nz);
+  KCHECK(ldw::nn_pool2_fwd((const uint16_t *) in, (uint16_t *) out, batch, nx, ny, nz, c, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_pool2_bwd(const float * dout, float * din, int batch, int nx, int ny, int nz, int c,
+				int accumulate) {
+  READY();
+  if (nx % 2 || ny % 2 || nz % 2) LDW_FAIL("ldw_nn_pool2_bwd: odd lattice %d x %d x %d", nx, ny, nz);
+  KCHECK(ldw::nn_pool2_bwd(dout, din, batch, nx, ny, nz, c, accumulate, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_up2_fwd(const void * in, void * out, int batch, int nx, int ny, int nz, int c) {
+  READY();
+  KCHECK(ldw::nn_up2_fwd((const uint16_t *) in, (uint16_t *) out, batch, nx, ny, nz, c, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_up2_bwd(const float * dout, float * din, int batch, int nx, int ny, int nz, int c) {
+  READY();
+  KCHECK(ldw::nn_up2_bwd(dout, din, batch, nx, ny, nz, c, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_concat(const void * a, int ca, const void * b, int cb, void * out, long long npos) {
+  READY();
+  KCHECK(ldw::nn_concat((const uint16_t *) a, ca, (const uint16_t *) b, cb, (uint16_t *) out, npos,
+			ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_split(const float * d, int ca, int cb, float * da, float * db, long long npos,
+			    int acc_a, int acc_b) {
+  READY();
+  KCHECK(ldw::nn_split(d, ca, cb, da, db, npos, acc_a, acc_b, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_to_bf16(const float * a, void * b, long long n) {
+  READY();
+  KCHECK(ldw::nn_to_bf16(a, (uint16_t *) b, n, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_encode(const float * state, long long nsite, const float norm[6], void * out, int c) {
+  READY();
+  if (c < 7) LDW_FAIL("ldw_nn_encode: need at least 7 channels, got %d", c);
+  ldw::EncodeNorm e;
+  e.phi_shift = norm[0]; e.phi_scale = norm[1]; e.u_scale = norm[2];
+  e.prm[0] = norm[3]; e.prm[1] = norm[4]; e.prm[2] = norm[5];
+  KCHECK(ldw::nn_encode(state, nsite, e, (uint16_t *) out, c, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_target(const float * now, const float * next, long long nsite, float * tgt,
+			     float dphi_scale, float du_scale) {
+  READY();
+  KCHECK(ldw::nn_target(now, next, nsite, tgt, dphi_scale, du_scale, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_decode(const float * now, const float * y, int cy, long long nsite, float * out,
+			     float dphi_scale, float du_scale) {
+  READY();
+  if (cy < 4) LDW_FAIL("ldw_nn_decode: need at least 4 output channels, got %d", cy);
+  KCHECK(ldw::nn_decode(now, y, cy, nsite, out, dphi_scale, du_scale, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_mse(const float * y, int cy, const float * tgt, long long npos, void * dy, double * loss) {
+  READY();
+  if (cy < 4) LDW_FAIL("ldw_nn_mse: need at least 4 output channels, got %d", cy);
+  hipStream_t st = ldw::api::stream();
+  void * err = nullptr;
+  if (scratch(g_err, (size_t) 4*npos*sizeof(float), &err)) return -1;
+  KCHECK(ldw::nn_mse(y, cy, tgt, npos, (float *) err, (uint16_t *) dy, st));
+  if (loss) {
+    const int nb = ldw::sum_blocks(4*npos);
+    void * work = nullptr;
+    if (scratch(g_sum, (size_t) (3*nb + 3)*sizeof(double), &work)) return -1;
+    double * w = (double *) work;
+    ldw::launch_klein_sum_f32((const float *) err, 4*npos, w + 3, w, st);
+    LDW_HIP_CHECK(hipGetLastError());
+    double h[3];
+    LDW_HIP_CHECK(hipMemcpyAsync(h, w, 3*sizeof(double), hipMemcpyDeviceToHost, st));
+    LDW_HIP_CHECK(hipStreamSynchronize(st));
+    *loss = h[0]/(4.0*(double) npos);
+  }
+  return 0;
+}
+
+extern "C" int ldw_nn_adam(float * w, const float * g, float * m, float * v, long long n, float lr,
+			   float b1, float b2, float eps, float wd, int step) {
+  READY();
+  if (step < 1) LDW_FAIL("ldw_nn_adam: step %d < 1", step);
+  const double lr_t = lr*std::sqrt(1.0 - std::pow((double) b2, step))/(1.0 - std::pow((double) b1, step));
+  KCHECK(ldw::nn_adam(w, g, m, v, n, (float) lr_t, b1, b2, eps, wd, ldw::api::stream()));
+  return 0;
+}

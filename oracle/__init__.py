@@ -213,3 +213,40 @@ def kahan_sum(a):
 def klein_sum(a):
     a = np.ascontiguousarray(a, dtype=np.float64)
     return lib().ora_klein_sum(a.size, _p(a))
+
+
+# ---------------------------------------------------------------------------
+# FP64 convolution checker (conv_oracle.c); channels-last arrays, periodic
+
+
+def _conv_args(batch, nx, ny, nz, cin, cout):
+    return [ctypes.c_int(v) for v in (batch, nx, ny, nz, cin, cout)]
+
+
+def conv3_fwd(x, w, bias, shape, cin, cout, batch=1):
+    nx, ny, nz = shape
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    b = None if bias is None else np.ascontiguousarray(bias, dtype=np.float64)
+    y = np.zeros(batch * nx * ny * nz * cout)
+    lib().ora_conv3_fwd(_p(x), _p(w), _p(b), _p(y), *_conv_args(batch, nx, ny, nz, cin, cout))
+    return y
+
+
+def conv3_bwd_data(dy, w, shape, cin, cout, batch=1):
+    nx, ny, nz = shape
+    dy = np.ascontiguousarray(dy, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    dx = np.zeros(batch * nx * ny * nz * cin)
+    lib().ora_conv3_bwd_data(_p(dy), _p(w), _p(dx), *_conv_args(batch, nx, ny, nz, cin, cout))
+    return dx
+
+
+def conv3_bwd_weight(x, dy, shape, cin, cout, batch=1):
+    nx, ny, nz = shape
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    dy = np.ascontiguousarray(dy, dtype=np.float64)
+    dw = np.zeros(27 * cout * cin)
+    lib().ora_conv3_bwd_weight(_p(x), _p(dy), _p(dw), *_conv_args(batch, nx, ny, nz, cin, cout))
+    return dw
+

@@ -1,0 +1,169 @@
+"""The MFMA convolutions and the network's elementwise kernels through the
+C ABI, against the FP64 CPU checker (oracle/conv_oracle.c) on the same
+BF16-rounded inputs.
+
+Tolerance: products of two BF16 values are exact in FP32, so the only error
+is FP32 accumulation of K terms: |gpu - exact| <= 2 K 2^-24 sum |a b| per
+output element (K = 27 cin for the data passes, the number of positions for
+the weight gradient), with sum |a b| from the checker applied to |a|, |b|.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import ludwig_amd as L
+import oracle as O
+from bf16util import bf16_bits, bf16_round, bf16_value
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (6, 8, 10)
+EPS32 = 2.0 ** -24
+
+
+def dev(a, dtype):
+    return L.DeviceArray.from_host(np.ascontiguousarray(a), dtype)
+
+
+def vp(d):
+    return ctypes.c_void_p(d.ptr)
+
+
+def within(got, want, bound, what):
+    err = np.abs(got - want)
+    bad = np.flatnonzero(err > bound)
+    if bad.size:
+        i = bad[0]
+        pytest.fail(f"{what}: {bad.size}/{got.size} outside bound, first {i}: {got[i]} vs {want[i]} "
+                    f"(bound {bound[i]:.3e})")
+
+
+@pytest.mark.parametrize("cin,cout,batch", [(16, 16, 1), (32, 48, 2), (48, 64, 1), (16, 96, 2),
+                                            (32, 160, 1), (160, 32, 1)])
+def test_conv_fwd_bwd_against_oracle(cin, cout, batch):
+    lib = L.lib()
+    rng = np.random.default_rng(cin * 1000 + cout)
+    n = batch * int(np.prod(SHAPE))
+    print(f"conv cin={cin} cout={cout} batch={batch} shape={SHAPE}", flush=True)
+    x = bf16_round(rng.standard_normal(n * cin))
+    w32 = (rng.standard_normal(27 * cout * cin) / np.sqrt(27 * cin)).astype(np.float32)
+    w = bf16_round(w32)
+    bias = rng.standard_normal(cout).astype(np.float32)
+    g = bf16_round(rng.standard_normal(n * cout))
+
+    dx_in, dw_in, dg = dev(bf16_bits(x), np.uint16), dev(w32, np.float32), dev(bf16_bits(g), np.uint16)
+    wf, wd = L.DeviceArray(w.size, np.uint16), L.DeviceArray(w.size, np.uint16)
+    db = dev(bias, np.float32)
+    y, dxo, dwo = L.DeviceArray(n * cout, np.float32), L.DeviceArray(n * cin, np.float32), \
+        L.DeviceArray(w.size, np.float32)
+    assert lib.ldw_conv3d_weights_prep(vp(dw_in), vp(wf), vp(wd), cin, cout) == 0
+    assert np.array_equal(wf.download(), bf16_bits(w32))
+    nx, ny, nz = SHAPE
+    assert lib.ldw_conv3d_fwd(vp(dx_in), vp(wf), vp(db), vp(y), batch, nx, ny, nz, cin, cout, 0) == 0
+    assert lib.ldw_conv3d_bwd_data(vp(dg), vp(wd), vp(dxo), batch, nx, ny, nz, cin, cout, 0) == 0
+    assert lib.ldw_conv3d_bwd_weight(vp(dx_in), vp(dg), vp(dwo), batch, nx, ny, nz, cin, cout, 0) == 0
+
+    yo = O.conv3_fwd(x, w, bias.astype(np.float64), SHAPE, cin, cout, batch)
+    ya = O.conv3_fwd(np.abs(x), np.abs(w), np.abs(bias.astype(np.float64)), SHAPE, cin, cout, batch)
+    within(y.download().astype(np.float64), yo, 2 * 27 * cin * EPS32 * ya + 1e-30, "forward")
+    dxr = O.conv3_bwd_data(g, w, SHAPE, cin, cout, batch)
+    dxa = O.conv3_bwd_data(np.abs(g), np.abs(w), SHAPE, cin, cout, batch)
+    within(dxo.download().astype(np.float64), dxr, 2 * 27 * cout * EPS32 * dxa + 1e-30, "backward data")
+    dwr = O.conv3_bwd_weight(x, g, SHAPE, cin, cout, batch)
+    dwa = O.conv3_bwd_weight(np.abs(x), np.abs(g), SHAPE, cin, cout, batch)
+    within(dwo.download().astype(np.float64), dwr, 2 * n * EPS32 * dwa + 1e-30, "backward weight")
+    # accumulate = 1 adds to what is there
+    assert lib.ldw_conv3d_fwd(vp(dx_in), vp(wf), vp(db), vp(y), batch, nx, ny, nz, cin, cout, 1) == 0
+    within(y.download().astype(np.float64), 2 * yo, 4 * 27 * cin * EPS32 * ya + 1e-30, "forward x2")
+
+
+def test_pool_up_concat_split_silu_exact():
+    lib = L.lib()
+    rng = np.random.default_rng(1)
+    b, nx, ny, nz, c = 2, 4, 6, 8, 16
+    n = b * nx * ny * nz
+    a = bf16_round(rng.standard_normal(n * c))
+    da = dev(bf16_bits(a), np.uint16)
+    po = L.DeviceArray(n // 8 * c, np.uint16)
+    assert lib.ldw_nn_pool2_fwd(vp(da), vp(po), b, nx, ny, nz, c) == 0
+    a5 = a.reshape(b, nx // 2, 2, ny // 2, 2, nz // 2, 2, c)
+    want = bf16_round(a5.mean(axis=(2, 4, 6)).astype(np.float32).ravel())
+    got = bf16_value(po.download())
+    # the kernel sums 8 BF16 values in FP32 (exact here) then scales by 1/8
+    assert np.array_equal(got, want)
+    up = L.DeviceArray(n * c, np.uint16)
+    assert lib.ldw_nn_up2_fwd(vp(po), vp(up), b, nx // 2, ny // 2, nz // 2, c) == 0
+    u = bf16_value(up.download()).reshape(b, nx, ny, nz, c)
+    pc = got.reshape(b, nx // 2, ny // 2, nz // 2, c)
+    assert np.array_equal(u, pc.repeat(2, 1).repeat(2, 2).repeat(2, 3))
+    # backward of up: sum of children; of pool: parent / 8
+    g = rng.standard_normal(n * c).astype(np.float32)
+    dg, dc = dev(g, np.float32), L.DeviceArray(n // 8 * c, np.float32)
+    assert lib.ldw_nn_up2_bwd(vp(dg), vp(dc), b, nx // 2, ny // 2, nz // 2, c) == 0
+    g5 = g.astype(np.float64).reshape(b, nx // 2, 2, ny // 2, 2, nz // 2, 2, c)
+    assert np.allclose(dc.download(), g5.sum(axis=(2, 4, 6)).ravel(), rtol=1e-6, atol=1e-6)
+    df = L.DeviceArray(n * c, np.float32)
+    assert lib.ldw_nn_pool2_bwd(vp(dc), vp(df), b, nx, ny, nz, c, 0) == 0
+    dcv = dc.download().reshape(b, nx // 2, ny // 2, nz // 2, c)
+    assert np.array_equal(df.download().reshape(b, nx, ny, nz, c),
+                          (np.float32(0.125) * dcv).repeat(2, 1).repeat(2, 2).repeat(2, 3))
+    # concat / split
+    c2 = 32
+    a2 = bf16_bits(rng.standard_normal(n * c2))
+    d2 = dev(a2, np.uint16)
+    cat = L.DeviceArray(n * (c + c2), np.uint16)
+    assert lib.ldw_nn_concat(vp(da), c, vp(d2), c2, vp(cat), n) == 0
+    assert np.array_equal(cat.download().reshape(n, c + c2),
+                          np.concatenate([bf16_bits(a).reshape(n, c), a2.reshape(n, c2)], axis=1))
+    gg = rng.standard_normal(n * (c + c2)).astype(np.float32)
+    dgg, sa, sb = dev(gg, np.float32), L.DeviceArray(n * c, np.float32), L.DeviceArray(n * c2, np.float32)
+    assert lib.ldw_nn_split(vp(dgg), c, c2, vp(sa), vp(sb), n, 0, 0) == 0
+    assert np.array_equal(sa.download(), gg.reshape(n, -1)[:, :c].ravel())
+    assert np.array_equal(sb.download(), gg.reshape(n, -1)[:, c:].ravel())
+    # SiLU and its derivative against float64
+    z = rng.standard_normal(n * c).astype(np.float32) * 3
+    dz, act = dev(z, np.float32), L.DeviceArray(n * c, np.uint16)
+    assert lib.ldw_nn_silu_fwd(vp(dz), vp(act), n * c) == 0
+    zz = z.astype(np.float64)
+    sig = 1 / (1 + np.exp(-zz))
+    assert np.allclose(bf16_value(act.download()), zz * sig, rtol=2 ** -7, atol=1e-6)
+    dgr = dev(g.astype(np.float32), np.float32)
+    dzo, dzb = L.DeviceArray(n * c, np.float32), L.DeviceArray(n * c, np.uint16)
+    assert lib.ldw_nn_silu_bwd(vp(dz), vp(dgr), vp(dzo), vp(dzb), n * c) == 0
+    assert np.allclose(dzo.download(), g * sig * (1 + zz * (1 - sig)), rtol=1e-5, atol=1e-6)
+    # bias gradient: per-channel sums
+    dbias = L.DeviceArray(c, np.float32)
+    assert lib.ldw_nn_bias_grad(vp(dgr), vp(dbias), n, c, 0) == 0
+    assert np.allclose(dbias.download(), g.astype(np.float64).reshape(n, c).sum(0), rtol=1e-5, atol=1e-4)
+
+
+def test_mse_and_adam():
+    lib = L.lib()
+    rng = np.random.default_rng(2)
+    npos, cy = 1000, 16
+    y = rng.standard_normal(npos * cy).astype(np.float32)
+    t = rng.standard_normal(npos * 4).astype(np.float32)
+    dy_, dt_, dg = dev(y, np.float32), dev(t, np.float32), L.DeviceArray(npos * cy, np.uint16)
+    loss = ctypes.c_double()
+    assert lib.ldw_nn_mse(vp(dy_), cy, vp(dt_), npos, vp(dg), ctypes.byref(loss)) == 0
+    e = y.reshape(npos, cy)[:, :4].astype(np.float64) - t.reshape(npos, 4)
+    assert loss.value == pytest.approx(np.mean(e * e), rel=1e-6)
+    grad = np.zeros((npos, cy))
+    grad[:, :4] = 2 * e / (4 * npos)
+    assert np.allclose(bf16_value(dg.download()), grad.ravel(), rtol=2 ** -8, atol=1e-12)
+    n = 777
+    w, g = rng.standard_normal(n).astype(np.float32), rng.standard_normal(n).astype(np.float32)
+    m, v = np.zeros(n, np.float32), np.zeros(n, np.float32)
+    dw, dgr, dm, dv = (dev(a, np.float32) for a in (w, g, m, v))
+    lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+    wref, mref, vref = w.astype(np.float64), m.astype(np.float64), v.astype(np.float64)
+    for step in (1, 2, 3):
+        assert lib.ldw_nn_adam(vp(dw), vp(dgr), vp(dm), vp(dv), n, ctypes.c_float(lr), ctypes.c_float(b1),
+                               ctypes.c_float(b2), ctypes.c_float(eps), ctypes.c_float(0.0), step) == 0
+        mref = b1 * mref + (1 - b1) * g
+        vref = b2 * vref + (1 - b2) * g.astype(np.float64) ** 2
+        mh, vh = mref / (1 - b1 ** step), vref / (1 - b2 ** step)
+        wref = wref - lr * mh / (np.sqrt(vh) + eps)
+    assert np.allclose(dw.download(), wref, rtol=1e-5, atol=1e-6)
