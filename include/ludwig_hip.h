@@ -498,9 +498,10 @@ int ldw_nn_target(const float * now, const float * next, long long nsite, float 
 int ldw_nn_decode(const float * now, const float * y, int cy, long long nsite, float * out,
 		  float dphi_scale, float du_scale);
 /* mean square error of y's first 4 channels against tgt; gradient into dy
- * (BF16); *loss (may be NULL) = the mean, compensated sum */
+ * (BF16) and dyf (FP32, may be NULL); *loss (may be NULL) = the mean,
+ * compensated sum */
 int ldw_nn_mse(const float * y, int cy, const float * tgt, long long npos, void * dy,
-	       double * loss);
+	       float * dyf, double * loss);
 int ldw_nn_adam(float * w, const float * g, float * m, float * v, long long n, float lr,
 		float b1, float b2, float eps, float wd, int step);
 

@@ -211,7 +211,7 @@ hipError_t nn_target(const float * now, const float * next, int64_t nsite, float
 hipError_t nn_decode(const float * now, const float * y, int cy, int64_t nsite, float * out,
 		     float dphi_scale, float du_scale, hipStream_t s);
 hipError_t nn_mse(const float * y, int cy, const float * tgt, int64_t npos, float * err, uint16_t * dy,
-		  hipStream_t s);
+		  float * dyf, hipStream_t s);
 hipError_t nn_adam(float * w, const float * g, float * m, float * v, int64_t n, float lr_t,
 		   float b1, float b2, float eps, float wd, hipStream_t s);
 

@@ -213,10 +213,11 @@ __global__ void decode_kernel(const float * __restrict__ now, const float * __re
 }
 
 // mean-square error over the first 4 channels of y: squared errors into
-// err (for a compensated sum), gradient 2 (y - t)/(4 npos) into dy (BF16),
-// zero for the other channels
+// err (for a compensated sum), gradient 2 (y - t)/(4 npos) into dy (BF16)
+// and, if given, dyf (FP32); zero for the other channels
 __global__ void mse_kernel(const float * __restrict__ y, int cy, const float * __restrict__ tgt,
-			   int64_t npos, float * __restrict__ err, uint16_t * __restrict__ dy) {
+			   int64_t npos, float * __restrict__ err, uint16_t * __restrict__ dy,
+			   float * __restrict__ dyf) {
   const float scale = 2.0f/(4.0f*(float) npos);
   GRID_STRIDE(i, npos*cy) {
     const int64_t p = i / cy;
@@ -228,6 +229,7 @@ __global__ void mse_kernel(const float * __restrict__ y, int cy, const float * _
       g = scale*e;
     }
     dy[i] = bf16_of(g);
+    if (dyf) dyf[i] = g;
   }
 }
 
@@ -341,8 +343,8 @@ hipError_t nn_decode(const float * now, const float * y, int cy, int64_t nsite, 
 }
 
 hipError_t nn_mse(const float * y, int cy, const float * tgt, int64_t npos, float * err, uint16_t * dy,
-		  hipStream_t s) {
-  LAUNCH(mse_kernel, npos*cy, y, cy, tgt, npos, err, dy);
+		  float * dyf, hipStream_t s) {
+  LAUNCH(mse_kernel, npos*cy, y, cy, tgt, npos, err, dy, dyf);
   return hipGetLastError();
 }
 

@@ -184,13 +184,14 @@ extern "C" int ldw_nn_decode(const float * now, const float * y, int cy, long lo
   return 0;
 }
 
-extern "C" int ldw_nn_mse(const float * y, int cy, const float * tgt, long long npos, void * dy, double * loss) {
+extern "C" int ldw_nn_mse(const float * y, int cy, const float * tgt, long long npos, void * dy,
+			  float * dyf, double * loss) {
   READY();
   if (cy < 4) LDW_FAIL("ldw_nn_mse: need at least 4 output channels, got %d", cy);
   hipStream_t st = ldw::api::stream();
   void * err = nullptr;
   if (scratch(g_err, (size_t) 4*npos*sizeof(float), &err)) return -1;
-  KCHECK(ldw::nn_mse(y, cy, tgt, npos, (float *) err, (uint16_t *) dy, st));
+  KCHECK(ldw::nn_mse(y, cy, tgt, npos, (float *) err, (uint16_t *) dy, dyf, st));
   if (loss) {
     const int nb = ldw::sum_blocks(4*npos);
     void * work = nullptr;

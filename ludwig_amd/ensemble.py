@@ -92,6 +92,7 @@ def _bind():
         "ldw_ensemble_f_read": [vp, ctypes.c_int, vp],
         "ldw_ensemble_snapshot_device": [vp, ctypes.c_int, vp],
         "ldw_ensemble_conserved_sums": [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+        "ldw_ensemble_load_device": [vp, ctypes.c_int, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -167,6 +168,12 @@ class Ensemble:
         """phi, ux, uy, uz of replica r as 4 * nsite floats at device address dst_ptr."""
         _ck(_bind().ldw_ensemble_snapshot_device(self.h, r, ctypes.c_void_p(dst_ptr)),
             "ldw_ensemble_snapshot_device")
+
+    def load_device(self, r, src_ptr):
+        """Replace replica r's phi, u by 4 * nsite floats at device address
+        src_ptr; distributions become the local equilibrium."""
+        _ck(_bind().ldw_ensemble_load_device(self.h, r, ctypes.c_void_p(src_ptr)),
+            "ldw_ensemble_load_device")
 
     def conserved_sums(self, r):
         out = (ctypes.c_double * 2)()

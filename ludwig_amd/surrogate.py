@@ -1,0 +1,431 @@
+"""Learned surrogate for the binary-fluid step: a 4-level 3D convolutional
+U-Net that advances (phi, u) by a fixed number of lattice Boltzmann steps.
+
+Every operation of training and inference runs on the GPU through
+libludwig_hip.so's own kernels: BF16 MFMA implicit-GEMM convolutions with
+periodic padding (forward, backward data, backward weight), SiLU, 2x average
+pooling, nearest upsampling, channel concatenation, the loss and Adam (FP32
+master weights, BF16 copies for the MFMA passes).  This module only orders
+the kernel calls and owns the device buffers.
+
+Network (channels-last, periodic lattices whose sides are multiples of 8):
+
+    input 16 ch: phi, ux, uy, uz, (phi0, kappa, M) scaled, 9 zero channels
+    e0a 16->32, e0b 32->32            | level 0 (n)
+    pool, e1a 32->48, e1b 48->48      | level 1 (n/2)
+    pool, e2a 48->64, e2b 64->64      | level 2 (n/4)
+    pool, b0 64->96, b1 96->96        | level 3 (n/8)
+    up, concat(96 + 64), d2 160->64   | level 2
+    up, concat(64 + 48), d1 112->48   | level 1
+    up, concat(48 + 32), d0 80->32    | level 0
+    out 32->16 (no activation): channels 0..3 = scaled change of phi, u
+
+All convolutions are 3x3x3 with bias; SiLU after every one but `out`.
+1.26 M parameters.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import DeviceArray, _ck, lib
+
+C = (32, 48, 64, 96)
+CIN = 16
+COUT = 16
+
+LAYERS = [("e0a", CIN, C[0]), ("e0b", C[0], C[0]),
+          ("e1a", C[0], C[1]), ("e1b", C[1], C[1]),
+          ("e2a", C[1], C[2]), ("e2b", C[2], C[2]),
+          ("b0", C[2], C[3]), ("b1", C[3], C[3]),
+          ("d2", C[3] + C[2], C[2]), ("d1", C[2] + C[1], C[1]), ("d0", C[1] + C[0], C[0]),
+          ("out", C[0], COUT)]
+LEVEL = {"e0a": 0, "e0b": 0, "e1a": 1, "e1b": 1, "e2a": 2, "e2b": 2, "b0": 3, "b1": 3,
+         "d2": 2, "d1": 1, "d0": 0, "out": 0}
+
+
+def _v(d):
+    return ctypes.c_void_p(d.ptr if isinstance(d, DeviceArray) else d)
+
+
+def _off(d, nbytes):
+    """Device address `nbytes` into buffer d."""
+    return ctypes.c_void_p(d.ptr + int(nbytes))
+
+
+class UNet:
+    """Parameters, optimiser state and the forward/backward passes."""
+
+    def __init__(self, seed=0, out_scale=0.1):
+        self.layout = {}
+        off = 0
+        for name, ci, co in LAYERS:
+            self.layout[name] = (off, ci, co)
+            off += 27 * co * ci + co
+        self.nparam = off
+        rng = np.random.default_rng(seed)
+        w = np.zeros(off, dtype=np.float32)
+        for name, ci, co in LAYERS:
+            o = self.layout[name][0]
+            std = math.sqrt(2.0 / (27 * ci))
+            if name == "out":
+                std *= out_scale
+            w[o:o + 27 * co * ci] = rng.standard_normal(27 * co * ci) * std
+            if name == "e0a":
+                # the 9 padding input channels never carry signal
+                wl = w[o:o + 27 * co * ci].reshape(27, co, ci)
+                wl[:, :, 7:] = 0.0
+        self.w = DeviceArray.from_host(w, np.float32)
+        self.g = DeviceArray(off, np.float32)
+        self.m = DeviceArray.from_host(np.zeros(off, np.float32), np.float32)
+        self.v = DeviceArray.from_host(np.zeros(off, np.float32), np.float32)
+        self.wf, self.wd = {}, {}
+        for name, ci, co in LAYERS:
+            self.wf[name] = DeviceArray(27 * co * ci, np.uint16)
+            self.wd[name] = DeviceArray(27 * co * ci, np.uint16)
+        self.step_count = 0
+        self._bufs = {}
+        self.prepare()
+
+    # -- parameters -------------------------------------------------------
+    def wptr(self, name, arr=None):
+        o = self.layout[name][0]
+        return _off(arr or self.w, 4 * o)
+
+    def bptr(self, name, arr=None):
+        o, ci, co = self.layout[name]
+        return _off(arr or self.w, 4 * (o + 27 * co * ci))
+
+    def prepare(self):
+        """BF16 weight copies for the forward and backward-data passes."""
+        L = lib()
+        for name, ci, co in LAYERS:
+            _ck(L.ldw_conv3d_weights_prep(self.wptr(name), _v(self.wf[name]), _v(self.wd[name]), ci, co),
+                "ldw_conv3d_weights_prep")
+
+    def params(self):
+        return self.w.download()
+
+    def set_params(self, w):
+        self.w.upload(np.asarray(w, dtype=np.float32))
+        self.prepare()
+
+    def grads(self):
+        return self.g.download()
+
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+        self.step_count += 1
+        _ck(lib().ldw_nn_adam(_v(self.w), _v(self.g), _v(self.m), _v(self.v), self.nparam, lr, b1, b2,
+                              eps, wd, self.step_count), "ldw_nn_adam")
+        self.prepare()
+
+    # -- buffers ----------------------------------------------------------
+    def buffers(self, batch, shape):
+        key = (batch,) + tuple(shape)
+        if key in self._bufs:
+            return self._bufs[key]
+        nx, ny, nz = shape
+        if nx % 8 or ny % 8 or nz % 8:
+            raise ValueError(f"lattice sides must be multiples of 8, got {shape}")
+        P = [batch * (nx >> l) * (ny >> l) * (nz >> l) for l in range(4)]
+        b = {"P": P, "shape": [(nx >> l, ny >> l, nz >> l) for l in range(4)]}
+        for name, ci, co in LAYERS:
+            p = P[LEVEL[name]]
+            b["z_" + name] = DeviceArray(p * co, np.float32)
+            if name != "out":
+                b["a_" + name] = DeviceArray(p * co, np.uint16)
+                b["da_" + name] = DeviceArray(p * co, np.float32)
+            b["dz_" + name] = DeviceArray(p * co, np.float32)
+            b["dzb_" + name] = DeviceArray(p * co, np.uint16)
+        for l, c in ((1, C[0]), (2, C[1]), (3, C[2])):
+            b[f"pool{l}"] = DeviceArray(P[l] * c, np.uint16)
+            b[f"dpool{l}"] = DeviceArray(P[l] * c, np.float32)
+        for l, c_up, c_skip in ((2, C[3], C[2]), (1, C[2], C[1]), (0, C[1], C[0])):
+            b[f"up{l}"] = DeviceArray(P[l] * c_up, np.uint16)
+            b[f"dup{l}"] = DeviceArray(P[l] * c_up, np.float32)
+            b[f"cat{l}"] = DeviceArray(P[l] * (c_up + c_skip), np.uint16)
+            b[f"dcat{l}"] = DeviceArray(P[l] * (c_up + c_skip), np.float32)
+        self._bufs[key] = b
+        return b
+
+    # -- passes -----------------------------------------------------------
+    def _conv(self, b, name, x, act=True):
+        L = lib()
+        _, ci, co = self.layout[name]
+        l = LEVEL[name]
+        nx, ny, nz = b["shape"][l]
+        batch = b["batch"]
+        _ck(L.ldw_conv3d_fwd(_v(x), _v(self.wf[name]), self.bptr(name), _v(b["z_" + name]), batch,
+                             nx, ny, nz, ci, co, 0), f"conv {name}")
+        if act:
+            _ck(L.ldw_nn_silu_fwd(_v(b["z_" + name]), _v(b["a_" + name]), b["P"][l] * co), f"silu {name}")
+
+    def forward(self, x, batch, shape):
+        """x: device BF16 [batch*nsite][16].  Returns the FP32 output buffer
+        [batch*nsite][16] (channels 0..3 used)."""
+        L = lib()
+        b = self.buffers(batch, shape)
+        b["batch"] = batch
+        b["x"] = x
+        sh = b["shape"]
+        self._conv(b, "e0a", x)
+        self._conv(b, "e0b", b["a_e0a"])
+        for l, prev, na, nb in ((1, "e0b", "e1a", "e1b"), (2, "e1b", "e2a", "e2b"), (3, "e2b", "b0", "b1")):
+            c = self.layout[prev][2]
+            _ck(L.ldw_nn_pool2_fwd(_v(b["a_" + prev]), _v(b[f"pool{l}"]), batch, *sh[l - 1], c), "pool")
+            self._conv(b, na, b[f"pool{l}"])
+            self._conv(b, nb, b["a_" + na])
+        for l, low, skip, name in ((2, "b1", "e2b", "d2"), (1, "d2", "e1b", "d1"), (0, "d1", "e0b", "d0")):
+            cu, cs = self.layout[low][2], self.layout[skip][2]
+            _ck(L.ldw_nn_up2_fwd(_v(b["a_" + low]), _v(b[f"up{l}"]), batch, *sh[l + 1], cu), "up")
+            _ck(L.ldw_nn_concat(_v(b[f"up{l}"]), cu, _v(b["a_" + skip]), cs, _v(b[f"cat{l}"]), b["P"][l]),
+                "concat")
+            self._conv(b, name, b[f"cat{l}"])
+        self._conv(b, "out", b["a_d0"], act=False)
+        self._last = b
+        return b["z_out"]
+
+    def _conv_back(self, b, name, x, dzb, dz, dx=None, acc_dx=0):
+        """Weight and bias gradients of layer `name`; input gradient into dx."""
+        L = lib()
+        _, ci, co = self.layout[name]
+        l = LEVEL[name]
+        nx, ny, nz = b["shape"][l]
+        batch = b["batch"]
+        _ck(L.ldw_conv3d_bwd_weight(_v(x), _v(dzb), self.wptr(name, self.g), batch, nx, ny, nz, ci, co, 0),
+            f"wgrad {name}")
+        _ck(L.ldw_nn_bias_grad(_v(dz), self.bptr(name, self.g), b["P"][l], co, 0), f"bias grad {name}")
+        if dx is not None:
+            _ck(L.ldw_conv3d_bwd_data(_v(dzb), _v(self.wd[name]), _v(dx), batch, nx, ny, nz, ci, co, acc_dx),
+                f"dgrad {name}")
+
+    def _act_back(self, b, name):
+        L = lib()
+        co = self.layout[name][2]
+        _ck(L.ldw_nn_silu_bwd(_v(b["z_" + name]), _v(b["da_" + name]), _v(b["dz_" + name]),
+                              _v(b["dzb_" + name]), b["P"][LEVEL[name]] * co), f"silu back {name}")
+
+    def backward(self, dyb, dyf):
+        """Gradients of the loss into self.g, from the output gradient dyb
+        (BF16) and dyf (FP32), both [batch*nsite][16]."""
+        L = lib()
+        b = self._last
+        batch, sh, P = b["batch"], b["shape"], b["P"]
+        self._conv_back(b, "out", b["a_d0"], dyb, dyf, b["da_d0"])
+        for l, low, skip, name in ((0, "d1", "e0b", "d0"), (1, "d2", "e1b", "d1"), (2, "b1", "e2b", "d2")):
+            cu, cs = self.layout[low][2], self.layout[skip][2]
+            self._act_back(b, name)
+            self._conv_back(b, name, b[f"cat{l}"], b["dzb_" + name], b["dz_" + name], b[f"dcat{l}"])
+            # the skip branch's gradient is written first; pooling adds to it later
+            _ck(L.ldw_nn_split(_v(b[f"dcat{l}"]), cu, cs, _v(b[f"dup{l}"]), _v(b["da_" + skip]), P[l], 0, 0),
+                "split")
+            _ck(L.ldw_nn_up2_bwd(_v(b[f"dup{l}"]), _v(b["da_" + low]), batch, *sh[l + 1], cu), "up back")
+        for l, prev, na, nb in ((3, "e2b", "b0", "b1"), (2, "e1b", "e2a", "e2b"), (1, "e0b", "e1a", "e1b")):
+            self._act_back(b, nb)
+            self._conv_back(b, nb, b["a_" + na], b["dzb_" + nb], b["dz_" + nb], b["da_" + na])
+            self._act_back(b, na)
+            self._conv_back(b, na, b[f"pool{l}"], b["dzb_" + na], b["dz_" + na], b[f"dpool{l}"])
+            c = self.layout[prev][2]
+            _ck(L.ldw_nn_pool2_bwd(_v(b[f"dpool{l}"]), _v(b["da_" + prev]), batch, *sh[l - 1], c, 1),
+                "pool back")
+        self._act_back(b, "e0b")
+        self._conv_back(b, "e0b", b["a_e0a"], b["dzb_e0b"], b["dz_e0b"], b["da_e0a"])
+        self._act_back(b, "e0a")
+        self._conv_back(b, "e0a", b["x"], b["dzb_e0a"], b["dz_e0a"], None)
+
+
+class Normaliser:
+    """Scales of the network's inputs and outputs, from training data."""
+
+    def __init__(self, u_scale, dphi_scale, du_scale, phi_shift=0.0, phi_scale=1.0):
+        self.u_scale, self.dphi_scale, self.du_scale = float(u_scale), float(dphi_scale), float(du_scale)
+        self.phi_shift, self.phi_scale = float(phi_shift), float(phi_scale)
+
+    @staticmethod
+    def params(spec):
+        """The three scalar input channels of a replica (phi0, kappa, M)."""
+        return (spec.phi0 / 0.3, (spec.kappa - 0.05) / 0.03, (spec.mobility - 0.3) / 0.2)
+
+    def norm(self, spec):
+        return (ctypes.c_float * 6)(self.phi_shift, self.phi_scale, self.u_scale, *self.params(spec))
+
+    def to_dict(self):
+        return dict(u_scale=self.u_scale, dphi_scale=self.dphi_scale, du_scale=self.du_scale,
+                    phi_shift=self.phi_shift, phi_scale=self.phi_scale)
+
+
+def encode(state, nsite, spec, normaliser, out, offset_sites=0):
+    """Network input of one sample (4 SoA planes of floats at `state`) into
+    the BF16 batch buffer `out`, at sample offset `offset_sites`."""
+    _ck(lib().ldw_nn_encode(_v(state), nsite, normaliser.norm(spec), _off(out, 2 * CIN * offset_sites), CIN),
+        "ldw_nn_encode")
+
+
+def target(now, nxt, nsite, normaliser, out, offset_sites=0):
+    _ck(lib().ldw_nn_target(_v(now), _v(nxt), nsite, _off(out, 4 * 4 * offset_sites),
+                            normaliser.dphi_scale, normaliser.du_scale), "ldw_nn_target")
+
+
+def decode(now, y, nsite, normaliser, out, offset_sites=0):
+    _ck(lib().ldw_nn_decode(_v(now), _off(y, 4 * COUT * offset_sites), COUT, nsite, _v(out),
+                            normaliser.dphi_scale, normaliser.du_scale), "ldw_nn_decode")
+
+
+def loss_and_grad(y, tgt, npos, dyb, dyf):
+    loss = ctypes.c_double()
+    _ck(lib().ldw_nn_mse(_v(y), COUT, _v(tgt), npos, _v(dyb), _v(dyf), ctypes.byref(loss)), "ldw_nn_mse")
+    return loss.value
+
+
+# ---------------------------------------------------------------------------
+# Data, training and jumps
+
+
+class Dataset:
+    """(phi, u) snapshots of every replica of an ensemble, kept on the
+    device: snaps[r][k] is replica r at step k*every, 4 SoA planes of floats."""
+
+    def __init__(self, ensemble, nsnap, every, replicas=None):
+        self.specs = ensemble.specs
+        self.shapes = ensemble.shapes
+        self.every = every
+        self.replicas = list(range(len(self.specs))) if replicas is None else list(replicas)
+        self.snaps = {r: [] for r in self.replicas}
+        self._take(ensemble)
+        for k in range(nsnap):
+            ensemble.step(every)
+            self._take(ensemble)
+
+    def _take(self, e):
+        for r in self.replicas:
+            d = DeviceArray(4 * int(np.prod(self.shapes[r])), np.float32)
+            e.snapshot_device(r, d.ptr)
+            self.snaps[r].append(d)
+
+    def pairs(self, lag=1):
+        return [(r, k) for r in self.replicas for k in range(len(self.snaps[r]) - lag)]
+
+    def normaliser(self, sample=16, lag=1, seed=0):
+        """Scales from a sample of pairs: u to unit RMS, and the changes of
+        phi and u over one jump to unit RMS."""
+        rng = np.random.default_rng(seed)
+        pr = self.pairs(lag)
+        pick = [pr[i] for i in rng.choice(len(pr), min(sample, len(pr)), replace=False)]
+        us, dps, dus = [], [], []
+        for r, k in pick:
+            a = self.snaps[r][k].download().reshape(4, -1)
+            b = self.snaps[r][k + lag].download().reshape(4, -1)
+            us.append(np.sqrt(np.mean(a[1:] ** 2)))
+            dps.append(np.sqrt(np.mean((b[0] - a[0]) ** 2)))
+            dus.append(np.sqrt(np.mean((b[1:] - a[1:]) ** 2)))
+        rms = lambda v: max(float(np.mean(v)), 1e-12)
+        return Normaliser(1.0 / rms(us), 1.0 / rms(dps), 1.0 / rms(dus))
+
+    def free(self):
+        for r in self.snaps:
+            for d in self.snaps[r]:
+                d.free()
+        self.snaps = {}
+
+
+class Trainer:
+    """Adam on the GPU over batches of same-size pairs (snapshot k -> k + lag)."""
+
+    def __init__(self, net, normaliser, batch=4):
+        self.net, self.norm, self.batch = net, normaliser, batch
+        self._io = {}
+
+    def _bufs(self, shape, batch):
+        key = (batch,) + tuple(shape)
+        if key not in self._io:
+            n = batch * int(np.prod(shape))
+            self._io[key] = dict(x=DeviceArray(n * CIN, np.uint16), t=DeviceArray(n * 4, np.float32),
+                                 dyb=DeviceArray(n * COUT, np.uint16), dyf=DeviceArray(n * COUT, np.float32))
+        return self._io[key]
+
+    def groups(self, data, pairs):
+        by = {}
+        for r, k in pairs:
+            by.setdefault(tuple(data.shapes[r]), []).append((r, k))
+        return by
+
+    def step(self, data, items, lr, lag=1, train=True):
+        """One optimiser step (or, train=False, only the loss) on `items`,
+        a list of (replica, snapshot) of one lattice size."""
+        shape = tuple(data.shapes[items[0][0]])
+        ns = int(np.prod(shape))
+        io = self._bufs(shape, len(items))
+        for j, (r, k) in enumerate(items):
+            encode(data.snaps[r][k], ns, data.specs[r], self.norm, io["x"], j * ns)
+            target(data.snaps[r][k], data.snaps[r][k + lag], ns, self.norm, io["t"], j * ns)
+        y = self.net.forward(io["x"], len(items), shape)
+        loss = loss_and_grad(y, io["t"], len(items) * ns, io["dyb"], io["dyf"])
+        if train:
+            self.net.backward(io["dyb"], io["dyf"])
+            self.net.adam(lr)
+        return loss
+
+    def fit(self, data, pairs, iters, lr=1e-3, lag=1, seed=0, lr_final=None, warmup=50, log=None,
+            max_seconds=None):
+        """`iters` Adam steps (fewer if `max_seconds` of wall time run out;
+        the loss read-back each step keeps the host in step with the GPU)."""
+        import time
+        t_start = time.perf_counter()
+        rng = np.random.default_rng(seed)
+        groups = [g for g in self.groups(data, pairs).values()]
+        weights = np.array([len(g) for g in groups], dtype=float)
+        weights /= weights.sum()
+        losses = []
+        for it in range(iters):
+            g = groups[rng.choice(len(groups), p=weights)]
+            items = [g[i] for i in rng.choice(len(g), min(self.batch, len(g)), replace=False)]
+            lr_it = lr if lr_final is None else lr * (lr_final / lr) ** (it / max(iters - 1, 1))
+            lr_it *= min(1.0, (it + 1) / max(warmup, 1))
+            losses.append(self.step(data, items, lr_it, lag))
+            if log and (it % 100 == 0 or it == iters - 1):
+                log(f"iter {it} loss {losses[-1]:.4e} ({time.perf_counter() - t_start:.1f} s)")
+            if max_seconds is not None and time.perf_counter() - t_start > max_seconds:
+                if log:
+                    log(f"stopped after {it + 1} iterations: time budget {max_seconds} s")
+                break
+        return losses
+
+    def evaluate(self, data, pairs, lag=1):
+        return float(np.mean([self.step(data, [p], 0.0, lag, train=False) for p in pairs]))
+
+
+class Jumper:
+    """Surrogate jumps of whole ensembles: every replica's (phi, u) is
+    replaced by the network's prediction, distributions by the local
+    equilibrium; plain lattice Boltzmann steps then correct it."""
+
+    def __init__(self, net, normaliser):
+        self.net, self.norm = net, normaliser
+        self._io = {}
+
+    def _bufs(self, shape, batch):
+        key = (batch,) + tuple(shape)
+        if key not in self._io:
+            ns = int(np.prod(shape))
+            self._io[key] = dict(x=DeviceArray(batch * ns * CIN, np.uint16),
+                                 now=[DeviceArray(4 * ns, np.float32) for _ in range(batch)],
+                                 out=DeviceArray(4 * ns, np.float32))
+        return self._io[key]
+
+    def jump(self, ensemble, replicas=None):
+        """One network jump of the listed replicas (default: all), batched by size."""
+        reps = list(range(len(ensemble.specs))) if replicas is None else list(replicas)
+        by = {}
+        for r in reps:
+            by.setdefault(tuple(ensemble.shapes[r]), []).append(r)
+        for shape, rs in by.items():
+            ns = int(np.prod(shape))
+            io = self._bufs(shape, len(rs))
+            for j, r in enumerate(rs):
+                ensemble.snapshot_device(r, io["now"][j].ptr)
+                encode(io["now"][j], ns, ensemble.specs[r], self.norm, io["x"], j * ns)
+            y = self.net.forward(io["x"], len(rs), shape)
+            for j, r in enumerate(rs):
+                decode(io["now"][j], y, ns, self.norm, io["out"], j * ns)
+                ensemble.load_device(r, io["out"].ptr)

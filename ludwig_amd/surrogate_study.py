@@ -1,0 +1,167 @@
+"""Surrogate + lattice Boltzmann against plain lattice Boltzmann.
+
+Protocol:
+1. An uneven-size replica ensemble (random phi0, kappa, M per replica) is
+   run with snapshots of (phi, u) every `every` steps: the training data.
+2. Replicas are split by seed, 80 % for training and 20 % held out.
+3. The U-Net is trained on the GPU to map the snapshot at step t to the
+   one at t + every.
+4. On the held-out replicas of the evaluation size, from the initial
+   state, three runs reach the same time T = jumps * (every + corr):
+     plain LB                  T steps;
+     surrogate + LB            `jumps` times: one network jump (+ every
+                               steps), then `corr` LB steps;
+     identity + LB (baseline)  the same without the network, i.e. jumps
+                               that leave the state unchanged.
+   Each is judged by the domain length L(T) from the structure factor
+   (ludwig_amd.analysis) and the pointwise phi error against plain LB, and
+   timed on the GPU (all held-out replicas advance together).
+
+Run as a script for the full protocol; prints one JSON object.
+"""
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import lib
+from .analysis import domain_length
+from .ensemble import Ensemble, draw_replicas
+from .surrogate import Dataset, Jumper, Trainer, UNet
+
+
+def split_by_seed(specs, frac=0.2, seed=0):
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(len(specs))
+    nval = max(1, int(round(frac * len(specs))))
+    return sorted(order[nval:].tolist()), sorted(order[:nval].tolist())
+
+
+def _sync():
+    lib().ldw_device_synchronize()
+
+
+def evaluate(net, norm, specs, every, jumps, corr, log=print):
+    """Plain LB, surrogate + LB and identity + LB on `specs` from t = 0."""
+    T = jumps * (every + corr)
+    out = {"T": T, "replicas": len(specs)}
+    plain = Ensemble(specs, precision=32, fatal=True)
+    plain.tune(reps=2)
+    plain.step(0)                    # captures the step graphs
+    _sync()
+    t0 = time.perf_counter()
+    plain.step(T)
+    _sync()
+    out["plain_s"] = time.perf_counter() - t0
+    ref = [plain.snapshot(r)[0] for r in range(len(specs))]
+    plain.close()
+
+    def jumped(with_net):
+        e = Ensemble(specs, precision=32, fatal=True)
+        e.tune(reps=2)
+        e.step(0)
+        jumper = Jumper(net, norm)
+        if with_net:                 # warm the buffers and kernels once
+            e2 = Ensemble(specs, precision=32, fatal=True)
+            jumper.jump(e2)
+            e2.close()
+        _sync()
+        t0 = time.perf_counter()
+        for j in range(jumps):
+            if with_net:
+                jumper.jump(e)
+            e.step(corr)
+        _sync()
+        dt = time.perf_counter() - t0
+        phis = [e.snapshot(r)[0] for r in range(len(specs))]
+        e.close()
+        return phis, dt
+
+    sur, out["surrogate_s"] = jumped(True)
+    ide, out["identity_s"] = jumped(False)
+    rows = []
+    for r, s in enumerate(specs):
+        shape = (s.nx, s.ny, s.nz)
+        Lr = domain_length(ref[r], shape)
+        row = {"replica": r, "L_plain": Lr}
+        for name, phis in (("surrogate", sur), ("identity", ide)):
+            Ls = domain_length(phis[r], shape)
+            dev = ref[r] - ref[r].mean()
+            row[f"L_{name}"] = Ls
+            row[f"L_err_{name}"] = abs(Ls - Lr) / Lr
+            row[f"phi_rms_err_{name}"] = float(np.sqrt(np.mean((phis[r] - ref[r]) ** 2))
+                                               / max(np.sqrt(np.mean(dev ** 2)), 1e-30))
+        rows.append(row)
+    out["per_replica"] = rows
+    for k in ("L_err_surrogate", "L_err_identity", "phi_rms_err_surrogate", "phi_rms_err_identity"):
+        out[k + "_mean"] = float(np.mean([row[k] for row in rows]))
+        out[k + "_max"] = float(np.max([row[k] for row in rows]))
+    out["speedup"] = out["plain_s"] / out["surrogate_s"]
+    log(f"evaluation: {json.dumps({k: v for k, v in out.items() if k != 'per_replica'})}")
+    return out
+
+
+def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=64, iters=2000,
+        batch=4, lr=1e-3, jumps=8, corr=20, seed=2026, max_eval=None, train_seconds=None, log=print):
+    specs = draw_replicas(nrep, seed=seed, sizes=sizes)
+    train_idx, val_idx = split_by_seed(specs, 0.2, seed)
+    t0 = time.perf_counter()
+    ens = Ensemble(specs, precision=32, fatal=True)
+    ens.tune(reps=2)
+    data = Dataset(ens, steps // every, every)
+    ens.close()
+    _sync()
+    t_data = time.perf_counter() - t0
+    log(f"data: {nrep} replicas, {steps} steps, snapshots every {every}: {t_data:.1f} s")
+    norm = data.normaliser(sample=min(32, len(data.pairs())))
+    net = UNet(seed=seed)
+    tr = Trainer(net, norm, batch=batch)
+    tpairs = [(r, k) for (r, k) in data.pairs() if r in set(train_idx)]
+    vpairs = [(r, k) for (r, k) in data.pairs() if r in set(val_idx)]
+    t0 = time.perf_counter()
+    losses = tr.fit(data, tpairs, iters, lr=lr, lr_final=lr * 0.1, log=log, max_seconds=train_seconds)
+    _sync()
+    t_train = time.perf_counter() - t0
+    vsub = vpairs if len(vpairs) <= 64 else [vpairs[i] for i in
+                                              np.random.default_rng(1).choice(len(vpairs), 64, replace=False)]
+    val_loss = tr.evaluate(data, vsub)
+    log(f"training: {len(losses)} iterations in {t_train:.1f} s, last loss {np.mean(losses[-20:]):.4e}, "
+        f"held-out loss {val_loss:.4e} (a zero-change prediction scores ~1)")
+    data.free()
+    ev_specs = [specs[i] for i in val_idx if specs[i].nx == eval_size] or [specs[i] for i in val_idx]
+    if max_eval:
+        ev_specs = ev_specs[:max_eval]
+    result = {"replicas": nrep, "steps": steps, "every": every, "data_s": t_data,
+              "train_iters": len(losses), "train_s": t_train, "train_loss_last": float(np.mean(losses[-20:])),
+              "heldout_loss": val_loss, "normaliser": norm.to_dict()}
+    if ev_specs:
+        result["evaluation"] = evaluate(net, norm, ev_specs, every, jumps, corr, log=log)
+    return result
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--every", type=int, default=250)
+    ap.add_argument("--eval-size", type=int, default=64)
+    ap.add_argument("--iters", type=int, default=2000)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--jumps", type=int, default=8)
+    ap.add_argument("--corr", type=int, default=20)
+    ap.add_argument("--max-eval", type=int, default=0)
+    ap.add_argument("--train-seconds", type=float, default=0.0)
+    ap.add_argument("--seed", type=int, default=2026)
+    a = ap.parse_args()
+    res = run(a.replicas, a.steps, a.every, eval_size=a.eval_size, iters=a.iters, batch=a.batch, lr=a.lr,
+              jumps=a.jumps, corr=a.corr, seed=a.seed, max_eval=a.max_eval or None,
+              train_seconds=a.train_seconds or None,
+              log=lambda m: print(m, flush=True))
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -147,12 +147,14 @@ def test_mse_and_adam():
     t = rng.standard_normal(npos * 4).astype(np.float32)
     dy_, dt_, dg = dev(y, np.float32), dev(t, np.float32), L.DeviceArray(npos * cy, np.uint16)
     loss = ctypes.c_double()
-    assert lib.ldw_nn_mse(vp(dy_), cy, vp(dt_), npos, vp(dg), ctypes.byref(loss)) == 0
+    dgf = L.DeviceArray(npos * cy, np.float32)
+    assert lib.ldw_nn_mse(vp(dy_), cy, vp(dt_), npos, vp(dg), vp(dgf), ctypes.byref(loss)) == 0
     e = y.reshape(npos, cy)[:, :4].astype(np.float64) - t.reshape(npos, 4)
     assert loss.value == pytest.approx(np.mean(e * e), rel=1e-6)
     grad = np.zeros((npos, cy))
     grad[:, :4] = 2 * e / (4 * npos)
     assert np.allclose(bf16_value(dg.download()), grad.ravel(), rtol=2 ** -8, atol=1e-12)
+    assert np.allclose(dgf.download(), grad.ravel(), rtol=1e-6, atol=1e-12)
     n = 777
     w, g = rng.standard_normal(n).astype(np.float32), rng.standard_normal(n).astype(np.float32)
     m, v = np.zeros(n, np.float32), np.zeros(n, np.float32)
