@@ -495,8 +495,10 @@ int ldw_nn_to_bf16(const float * a, void * b, long long n);
 int ldw_nn_encode(const float * state, long long nsite, const float norm[6], void * out, int c);
 int ldw_nn_target(const float * now, const float * next, long long nsite, float * tgt,
 		  float dphi_scale, float du_scale);
+/* now + y / scale back to 4 SoA planes, clamped to |phi| <= phi_max and
+ * |u_a| <= u_max */
 int ldw_nn_decode(const float * now, const float * y, int cy, long long nsite, float * out,
-		  float dphi_scale, float du_scale);
+		  float dphi_scale, float du_scale, float phi_max, float u_max);
 /* mean square error of y's first 4 channels against tgt; gradient into dy
  * (BF16) and dyf (FP32, may be NULL); *loss (may be NULL) = the mean,
  * compensated sum */

@@ -209,7 +209,7 @@ hipError_t nn_encode(const float * state, int64_t nsite, const EncodeNorm & e, u
 hipError_t nn_target(const float * now, const float * next, int64_t nsite, float * tgt,
 		     float dphi_scale, float du_scale, hipStream_t s);
 hipError_t nn_decode(const float * now, const float * y, int cy, int64_t nsite, float * out,
-		     float dphi_scale, float du_scale, hipStream_t s);
+		     float dphi_scale, float du_scale, float phi_max, float u_max, hipStream_t s);
 hipError_t nn_mse(const float * y, int cy, const float * tgt, int64_t npos, float * err, uint16_t * dy,
 		  float * dyf, hipStream_t s);
 hipError_t nn_adam(float * w, const float * g, float * m, float * v, int64_t n, float lr_t,

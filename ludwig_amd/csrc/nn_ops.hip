@@ -202,13 +202,17 @@ __global__ void target_kernel(const float * __restrict__ now, const float * __re
   }
 }
 
-// prediction: state + change, back to 4 SoA planes
+// prediction: state + change, back to 4 SoA planes, clamped to
+// |phi| <= phi_max and |u_a| <= u_max (a guard against a wild jump)
 __global__ void decode_kernel(const float * __restrict__ now, const float * __restrict__ y, int cy,
-			      int64_t nsite, float * __restrict__ out, float dphi_scale, float du_scale) {
+			      int64_t nsite, float * __restrict__ out, float dphi_scale, float du_scale,
+			      float phi_max, float u_max) {
   GRID_STRIDE(i, nsite*4) {
     const int ch = (int) (i / nsite);
     const int64_t s = i - (int64_t) ch*nsite;
-    out[i] = now[i] + y[s*cy + ch]/(ch == 0 ? dphi_scale : du_scale);
+    const float v = now[i] + y[s*cy + ch]/(ch == 0 ? dphi_scale : du_scale);
+    const float m = (ch == 0) ? phi_max : u_max;
+    out[i] = fminf(fmaxf(v, -m), m);
   }
 }
 
@@ -337,8 +341,8 @@ hipError_t nn_target(const float * now, const float * next, int64_t nsite, float
 }
 
 hipError_t nn_decode(const float * now, const float * y, int cy, int64_t nsite, float * out,
-		     float dphi_scale, float du_scale, hipStream_t s) {
-  LAUNCH(decode_kernel, nsite*4, now, y, cy, nsite, out, dphi_scale, du_scale);
+		     float dphi_scale, float du_scale, float phi_max, float u_max, hipStream_t s) {
+  LAUNCH(decode_kernel, nsite*4, now, y, cy, nsite, out, dphi_scale, du_scale, phi_max, u_max);
   return hipGetLastError();
 }
 

@@ -177,10 +177,11 @@ extern "C" int ldw_nn_target(const float * now, const float * next, long long ns
 }
 
 extern "C" int ldw_nn_decode(const float * now, const float * y, int cy, long long nsite, float * out,
-			     float dphi_scale, float du_scale) {
+			     float dphi_scale, float du_scale, float phi_max, float u_max) {
   READY();
   if (cy < 4) LDW_FAIL("ldw_nn_decode: need at least 4 output channels, got %d", cy);
-  KCHECK(ldw::nn_decode(now, y, cy, nsite, out, dphi_scale, du_scale, ldw::api::stream()));
+  if (!(phi_max > 0.0f) || !(u_max > 0.0f)) LDW_FAIL("ldw_nn_decode: clamps must be positive");
+  KCHECK(ldw::nn_decode(now, y, cy, nsite, out, dphi_scale, du_scale, phi_max, u_max, ldw::api::stream()));
   return 0;
 }
 

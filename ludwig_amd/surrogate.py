@@ -267,9 +267,12 @@ def target(now, nxt, nsite, normaliser, out, offset_sites=0):
                             normaliser.dphi_scale, normaliser.du_scale), "ldw_nn_target")
 
 
+PHI_MAX, U_MAX = 1.5, 0.1      # clamps on a jump's result (|phi| ~ 1, |u| << 0.1)
+
+
 def decode(now, y, nsite, normaliser, out, offset_sites=0):
     _ck(lib().ldw_nn_decode(_v(now), _off(y, 4 * COUT * offset_sites), COUT, nsite, _v(out),
-                            normaliser.dphi_scale, normaliser.du_scale), "ldw_nn_decode")
+                            normaliser.dphi_scale, normaliser.du_scale, PHI_MAX, U_MAX), "ldw_nn_decode")
 
 
 def loss_and_grad(y, tgt, npos, dyb, dyf):
@@ -303,14 +306,15 @@ class Dataset:
             e.snapshot_device(r, d.ptr)
             self.snaps[r].append(d)
 
-    def pairs(self, lag=1):
-        return [(r, k) for r in self.replicas for k in range(len(self.snaps[r]) - lag)]
+    def pairs(self, lag=1, kmin=0):
+        """(replica, snapshot) of every pair k -> k + lag with k >= kmin."""
+        return [(r, k) for r in self.replicas for k in range(kmin, len(self.snaps[r]) - lag)]
 
-    def normaliser(self, sample=16, lag=1, seed=0):
+    def normaliser(self, sample=16, lag=1, seed=0, kmin=0):
         """Scales from a sample of pairs: u to unit RMS, and the changes of
-        phi and u over one jump to unit RMS."""
+        phi and u over one jump to unit RMS (medians over the sample)."""
         rng = np.random.default_rng(seed)
-        pr = self.pairs(lag)
+        pr = self.pairs(lag, kmin)
         pick = [pr[i] for i in rng.choice(len(pr), min(sample, len(pr)), replace=False)]
         us, dps, dus = [], [], []
         for r, k in pick:
@@ -319,7 +323,7 @@ class Dataset:
             us.append(np.sqrt(np.mean(a[1:] ** 2)))
             dps.append(np.sqrt(np.mean((b[0] - a[0]) ** 2)))
             dus.append(np.sqrt(np.mean((b[1:] - a[1:]) ** 2)))
-        rms = lambda v: max(float(np.mean(v)), 1e-12)
+        rms = lambda v: max(float(np.median(v)), 1e-12)
         return Normaliser(1.0 / rms(us), 1.0 / rms(dps), 1.0 / rms(dus))
 
     def free(self):

@@ -5,9 +5,10 @@ Protocol:
    run with snapshots of (phi, u) every `every` steps: the training data.
 2. Replicas are split by seed, 80 % for training and 20 % held out.
 3. The U-Net is trained on the GPU to map the snapshot at step t to the
-   one at t + every.
-4. On the held-out replicas of the evaluation size, from the initial
-   state, three runs reach the same time T = jumps * (every + corr):
+   one at t + every, for t >= t_start.
+4. On the held-out replicas of the evaluation size, from the lattice
+   Boltzmann state at t_start (domains have formed; both runs get there by
+   plain LB), three runs advance by the same time T = jumps * (every + corr):
      plain LB                  T steps;
      surrogate + LB            `jumps` times: one network jump (+ every
                                steps), then `corr` LB steps;
@@ -43,13 +44,14 @@ def _sync():
     lib().ldw_device_synchronize()
 
 
-def evaluate(net, norm, specs, every, jumps, corr, log=print):
-    """Plain LB, surrogate + LB and identity + LB on `specs` from t = 0."""
+def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
+    """Plain LB, surrogate + LB and identity + LB on `specs` from the LB
+    state at t_start."""
     T = jumps * (every + corr)
-    out = {"T": T, "replicas": len(specs)}
+    out = {"T": T, "t_start": t_start, "replicas": len(specs)}
     plain = Ensemble(specs, precision=32, fatal=True)
     plain.tune(reps=2)
-    plain.step(0)                    # captures the step graphs
+    plain.step(t_start)              # also captures the step graphs
     _sync()
     t0 = time.perf_counter()
     plain.step(T)
@@ -61,10 +63,11 @@ def evaluate(net, norm, specs, every, jumps, corr, log=print):
     def jumped(with_net):
         e = Ensemble(specs, precision=32, fatal=True)
         e.tune(reps=2)
-        e.step(0)
+        e.step(t_start)
         jumper = Jumper(net, norm)
         if with_net:                 # warm the buffers and kernels once
             e2 = Ensemble(specs, precision=32, fatal=True)
+            e2.step(1)
             jumper.jump(e2)
             e2.close()
         _sync()
@@ -104,7 +107,8 @@ def evaluate(net, norm, specs, every, jumps, corr, log=print):
 
 
 def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=64, iters=2000,
-        batch=4, lr=1e-3, jumps=8, corr=20, seed=2026, max_eval=None, train_seconds=None, log=print):
+        batch=4, lr=1e-3, jumps=8, corr=20, seed=2026, max_eval=None, train_seconds=None,
+        t_start=500, log=print):
     specs = draw_replicas(nrep, seed=seed, sizes=sizes)
     train_idx, val_idx = split_by_seed(specs, 0.2, seed)
     t0 = time.perf_counter()
@@ -115,11 +119,12 @@ def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=64, i
     _sync()
     t_data = time.perf_counter() - t0
     log(f"data: {nrep} replicas, {steps} steps, snapshots every {every}: {t_data:.1f} s")
-    norm = data.normaliser(sample=min(32, len(data.pairs())))
+    kmin = (t_start + every - 1) // every
+    norm = data.normaliser(sample=min(32, len(data.pairs(kmin=kmin))), kmin=kmin)
     net = UNet(seed=seed)
     tr = Trainer(net, norm, batch=batch)
-    tpairs = [(r, k) for (r, k) in data.pairs() if r in set(train_idx)]
-    vpairs = [(r, k) for (r, k) in data.pairs() if r in set(val_idx)]
+    tpairs = [(r, k) for (r, k) in data.pairs(kmin=kmin) if r in set(train_idx)]
+    vpairs = [(r, k) for (r, k) in data.pairs(kmin=kmin) if r in set(val_idx)]
     t0 = time.perf_counter()
     losses = tr.fit(data, tpairs, iters, lr=lr, lr_final=lr * 0.1, log=log, max_seconds=train_seconds)
     _sync()
@@ -137,7 +142,8 @@ def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=64, i
               "train_iters": len(losses), "train_s": t_train, "train_loss_last": float(np.mean(losses[-20:])),
               "heldout_loss": val_loss, "normaliser": norm.to_dict()}
     if ev_specs:
-        result["evaluation"] = evaluate(net, norm, ev_specs, every, jumps, corr, log=log)
+        result["evaluation"] = evaluate(net, norm, ev_specs, every, jumps, corr, t_start=kmin * every,
+                                        log=log)
     return result
 
 
@@ -154,11 +160,12 @@ def main():
     ap.add_argument("--corr", type=int, default=20)
     ap.add_argument("--max-eval", type=int, default=0)
     ap.add_argument("--train-seconds", type=float, default=0.0)
+    ap.add_argument("--t-start", type=int, default=500)
     ap.add_argument("--seed", type=int, default=2026)
     a = ap.parse_args()
     res = run(a.replicas, a.steps, a.every, eval_size=a.eval_size, iters=a.iters, batch=a.batch, lr=a.lr,
               jumps=a.jumps, corr=a.corr, seed=a.seed, max_eval=a.max_eval or None,
-              train_seconds=a.train_seconds or None,
+              train_seconds=a.train_seconds or None, t_start=a.t_start,
               log=lambda m: print(m, flush=True))
     print(json.dumps(res))
 

@@ -14,10 +14,10 @@ pytestmark = pytest.mark.gpu
 
 def test_small_surrogate_study():
     print("surrogate study: 10 replicas of 16^3 / 32^3, 400 steps", flush=True)
-    res = run(nrep=10, steps=400, every=50, sizes=(16, 32), eval_size=32, iters=300, batch=2,
-              lr=2e-3, jumps=3, corr=10, seed=5, log=lambda m: print(m, flush=True))
+    res = run(nrep=10, steps=500, every=50, sizes=(16, 32), eval_size=32, iters=400, batch=2,
+              lr=1e-3, jumps=3, corr=10, seed=5, t_start=100, log=lambda m: print(m, flush=True))
     assert np.isfinite(res["train_loss_last"])
     assert res["heldout_loss"] < 0.6, res["heldout_loss"]
     ev = res["evaluation"]
-    assert ev["T"] == 3 * 60
+    assert ev["T"] == 3 * 60 and ev["t_start"] == 100
     assert ev["phi_rms_err_surrogate_mean"] < ev["phi_rms_err_identity_mean"], ev
