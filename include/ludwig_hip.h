@@ -499,13 +499,17 @@ int ldw_nn_target(const float * now, const float * next, long long nsite, float 
  * |u_a| <= u_max */
 int ldw_nn_decode(const float * now, const float * y, int cy, long long nsite, float * out,
 		  float dphi_scale, float du_scale, float phi_max, float u_max);
-/* mean square error of y's first 4 channels against tgt; gradient into dy
- * (BF16) and dyf (FP32, may be NULL); *loss (may be NULL) = the mean,
- * compensated sum */
+/* mean of w_c (y_c - tgt_c)^2 over y's first 4 channels (weights may be
+ * NULL: all 1); gradient into dy (BF16) and dyf (FP32, may be NULL);
+ * *loss (may be NULL) = the mean, compensated sum */
 int ldw_nn_mse(const float * y, int cy, const float * tgt, long long npos, void * dy,
-	       float * dyf, double * loss);
+	       float * dyf, const float weights[4], double * loss);
+/* Adam step `step` (1-based) on FP32 weights; the gradient is multiplied
+ * by gscale first (global-norm clipping) */
 int ldw_nn_adam(float * w, const float * g, float * m, float * v, long long n, float lr,
-		float b1, float b2, float eps, float wd, int step);
+		float b1, float b2, float eps, float wd, float gscale, int step);
+/* *out = sum a[i]^2 in FP64 (fixed order), complete on return */
+int ldw_nn_sumsq(const float * a, long long n, double * out);
 
 #ifdef __cplusplus
 }

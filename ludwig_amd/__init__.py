@@ -153,8 +153,10 @@ def lib():
         "ldw_nn_encode": [vp, ctypes.c_longlong, ctypes.POINTER(ctypes.c_float), vp, ctypes.c_int],
         "ldw_nn_target": [vp, vp, ctypes.c_longlong, vp, ctypes.c_float, ctypes.c_float],
         "ldw_nn_decode": [vp, vp, ctypes.c_int, ctypes.c_longlong, vp] + [ctypes.c_float] * 4,
-        "ldw_nn_mse": [vp, ctypes.c_int, vp, ctypes.c_longlong, vp, vp, ctypes.POINTER(ctypes.c_double)],
-        "ldw_nn_adam": [vp, vp, vp, vp, ctypes.c_longlong] + [ctypes.c_float] * 5 + [ctypes.c_int],
+        "ldw_nn_mse": [vp, ctypes.c_int, vp, ctypes.c_longlong, vp, vp, ctypes.POINTER(ctypes.c_float),
+                       ctypes.POINTER(ctypes.c_double)],
+        "ldw_nn_adam": [vp, vp, vp, vp, ctypes.c_longlong] + [ctypes.c_float] * 6 + [ctypes.c_int],
+        "ldw_nn_sumsq": [vp, ctypes.c_longlong, ctypes.POINTER(ctypes.c_double)],
     }
     for name, args in sig.items():
         f = getattr(L, name)

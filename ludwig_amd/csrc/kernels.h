@@ -210,10 +210,13 @@ hipError_t nn_target(const float * now, const float * next, int64_t nsite, float
 		     float dphi_scale, float du_scale, hipStream_t s);
 hipError_t nn_decode(const float * now, const float * y, int cy, int64_t nsite, float * out,
 		     float dphi_scale, float du_scale, float phi_max, float u_max, hipStream_t s);
+struct ChannelWeights { float w[4]; };
 hipError_t nn_mse(const float * y, int cy, const float * tgt, int64_t npos, float * err, uint16_t * dy,
-		  float * dyf, hipStream_t s);
+		  float * dyf, const ChannelWeights & w, hipStream_t s);
 hipError_t nn_adam(float * w, const float * g, float * m, float * v, int64_t n, float lr_t,
-		   float b1, float b2, float eps, float wd, hipStream_t s);
+		   float b1, float b2, float eps, float wd, float gscale, hipStream_t s);
+// sum of a[i]^2 in FP64 into work[0]; work holds 1 + 512 doubles
+hipError_t nn_sumsq(const float * a, int64_t n, double * work, hipStream_t s);
 
 // Plain device copy of nbytes (multiple of 32) for bandwidth calibration
 void launch_copy(const void * src, void * dst, int64_t nbytes, hipStream_t s);

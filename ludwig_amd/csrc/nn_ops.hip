@@ -216,12 +216,12 @@ __global__ void decode_kernel(const float * __restrict__ now, const float * __re
   }
 }
 
-// mean-square error over the first 4 channels of y: squared errors into
-// err (for a compensated sum), gradient 2 (y - t)/(4 npos) into dy (BF16)
-// and, if given, dyf (FP32); zero for the other channels
+// weighted mean-square error over the first 4 channels of y: w_c (y - t)^2
+// into err (for a compensated sum), gradient 2 w_c (y - t)/(4 npos) into dy
+// (BF16) and, if given, dyf (FP32); zero for the other channels
 __global__ void mse_kernel(const float * __restrict__ y, int cy, const float * __restrict__ tgt,
 			   int64_t npos, float * __restrict__ err, uint16_t * __restrict__ dy,
-			   float * __restrict__ dyf) {
+			   float * __restrict__ dyf, ChannelWeights w) {
   const float scale = 2.0f/(4.0f*(float) npos);
   GRID_STRIDE(i, npos*cy) {
     const int64_t p = i / cy;
@@ -229,25 +229,48 @@ __global__ void mse_kernel(const float * __restrict__ y, int cy, const float * _
     float g = 0.0f;
     if (ch < 4) {
       const float e = y[i] - tgt[p*4 + ch];
-      err[p*4 + ch] = e*e;
-      g = scale*e;
+      err[p*4 + ch] = w.w[ch]*e*e;
+      g = scale*w.w[ch]*e;
     }
     dy[i] = bf16_of(g);
     if (dyf) dyf[i] = g;
   }
 }
 
-// Adam with bias correction folded into lr_t = lr sqrt(1 - b2^t)/(1 - b1^t)
+// Adam with bias correction folded into lr_t = lr sqrt(1 - b2^t)/(1 - b1^t);
+// the gradient is first multiplied by gscale (global-norm clipping)
 __global__ void adam_kernel(float * __restrict__ w, const float * __restrict__ g,
 			    float * __restrict__ m, float * __restrict__ v, int64_t n,
-			    float lr_t, float b1, float b2, float eps, float wd) {
+			    float lr_t, float b1, float b2, float eps, float wd, float gscale) {
   GRID_STRIDE(i, n) {
-    const float gi = g[i];
+    const float gi = g[i]*gscale;
     const float mi = b1*m[i] + (1.0f - b1)*gi;
     const float vi = b2*v[i] + (1.0f - b2)*gi*gi;
     m[i] = mi;
     v[i] = vi;
     w[i] = w[i] - lr_t*(mi/(sqrtf(vi) + eps) + wd*w[i]);
+  }
+}
+
+// per-block sums of squares in FP64 (fixed reduction order)
+__global__ void sumsq_kernel(const float * __restrict__ a, int64_t n, double * __restrict__ part) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  GRID_STRIDE(i, n) { const double v = a[i]; s += v*v; }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int) threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void sum_parts_kernel(const double * __restrict__ part, int nb, double * __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s += part[b];
+    out[0] = s;
   }
 }
 
@@ -347,14 +370,22 @@ hipError_t nn_decode(const float * now, const float * y, int cy, int64_t nsite, 
 }
 
 hipError_t nn_mse(const float * y, int cy, const float * tgt, int64_t npos, float * err, uint16_t * dy,
-		  float * dyf, hipStream_t s) {
-  LAUNCH(mse_kernel, npos*cy, y, cy, tgt, npos, err, dy, dyf);
+		  float * dyf, const ChannelWeights & w, hipStream_t s) {
+  LAUNCH(mse_kernel, npos*cy, y, cy, tgt, npos, err, dy, dyf, w);
   return hipGetLastError();
 }
 
 hipError_t nn_adam(float * w, const float * g, float * m, float * v, int64_t n, float lr_t,
-		   float b1, float b2, float eps, float wd, hipStream_t s) {
-  LAUNCH(adam_kernel, n, w, g, m, v, n, lr_t, b1, b2, eps, wd);
+		   float b1, float b2, float eps, float wd, float gscale, hipStream_t s) {
+  LAUNCH(adam_kernel, n, w, g, m, v, n, lr_t, b1, b2, eps, wd, gscale);
+  return hipGetLastError();
+}
+
+constexpr int SUMSQ_BLOCKS = 512;
+
+hipError_t nn_sumsq(const float * a, int64_t n, double * work, hipStream_t s) {
+  hipLaunchKernelGGL(sumsq_kernel, dim3(SUMSQ_BLOCKS), dim3(256), 0, s, a, n, work + 1);
+  hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(64), 0, s, work + 1, SUMSQ_BLOCKS, work);
   return hipGetLastError();
 }
 

@@ -22,7 +22,7 @@ struct Scratch {
   size_t bytes = 0;
 };
 
-Scratch g_wgrad, g_bias, g_err, g_sum;
+Scratch g_wgrad, g_bias, g_err, g_sum, g_sumsq;
 
 int scratch(Scratch & s, size_t bytes, void ** out) {
   if (s.bytes < bytes) {
@@ -186,13 +186,15 @@ extern "C" int ldw_nn_decode(const float * now, const float * y, int cy, long lo
 }
 
 extern "C" int ldw_nn_mse(const float * y, int cy, const float * tgt, long long npos, void * dy,
-			  float * dyf, double * loss) {
+			  float * dyf, const float weights[4], double * loss) {
   READY();
   if (cy < 4) LDW_FAIL("ldw_nn_mse: need at least 4 output channels, got %d", cy);
+  ldw::ChannelWeights cw;
+  for (int c = 0; c < 4; c++) cw.w[c] = weights ? weights[c] : 1.0f;
   hipStream_t st = ldw::api::stream();
   void * err = nullptr;
   if (scratch(g_err, (size_t) 4*npos*sizeof(float), &err)) return -1;
-  KCHECK(ldw::nn_mse(y, cy, tgt, npos, (float *) err, (uint16_t *) dy, dyf, st));
+  KCHECK(ldw::nn_mse(y, cy, tgt, npos, (float *) err, (uint16_t *) dy, dyf, cw, st));
   if (loss) {
     const int nb = ldw::sum_blocks(4*npos);
     void * work = nullptr;
@@ -209,10 +211,22 @@ extern "C" int ldw_nn_mse(const float * y, int cy, const float * tgt, long long 
 }
 
 extern "C" int ldw_nn_adam(float * w, const float * g, float * m, float * v, long long n, float lr,
-			   float b1, float b2, float eps, float wd, int step) {
+			   float b1, float b2, float eps, float wd, float gscale, int step) {
   READY();
   if (step < 1) LDW_FAIL("ldw_nn_adam: step %d < 1", step);
   const double lr_t = lr*std::sqrt(1.0 - std::pow((double) b2, step))/(1.0 - std::pow((double) b1, step));
-  KCHECK(ldw::nn_adam(w, g, m, v, n, (float) lr_t, b1, b2, eps, wd, ldw::api::stream()));
+  KCHECK(ldw::nn_adam(w, g, m, v, n, (float) lr_t, b1, b2, eps, wd, gscale, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_sumsq(const float * a, long long n, double * out) {
+  READY();
+  if (out == nullptr) LDW_FAIL("ldw_nn_sumsq: null output");
+  hipStream_t st = ldw::api::stream();
+  void * work = nullptr;
+  if (scratch(g_sumsq, 513*sizeof(double), &work)) return -1;
+  KCHECK(ldw::nn_sumsq(a, n, (double *) work, st));
+  LDW_HIP_CHECK(hipMemcpyAsync(out, work, sizeof(double), hipMemcpyDeviceToHost, st));
+  LDW_HIP_CHECK(hipStreamSynchronize(st));
   return 0;
 }

@@ -114,10 +114,23 @@ class UNet:
     def grads(self):
         return self.g.download()
 
-    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+    def grad_norm(self):
+        out = ctypes.c_double()
+        _ck(lib().ldw_nn_sumsq(_v(self.g), self.nparam, ctypes.byref(out)), "ldw_nn_sumsq")
+        return math.sqrt(out.value)
+
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0, clip=None):
+        """One Adam step; with `clip`, the gradient is first scaled to a
+        global norm of at most `clip`."""
         self.step_count += 1
+        gscale = 1.0
+        if clip is not None:
+            gn = self.grad_norm()
+            if not math.isfinite(gn):
+                raise FloatingPointError("non-finite gradient")
+            gscale = min(1.0, clip / max(gn, 1e-30))
         _ck(lib().ldw_nn_adam(_v(self.w), _v(self.g), _v(self.m), _v(self.v), self.nparam, lr, b1, b2,
-                              eps, wd, self.step_count), "ldw_nn_adam")
+                              eps, wd, gscale, self.step_count), "ldw_nn_adam")
         self.prepare()
 
     # -- buffers ----------------------------------------------------------
@@ -275,9 +288,10 @@ def decode(now, y, nsite, normaliser, out, offset_sites=0):
                             normaliser.dphi_scale, normaliser.du_scale, PHI_MAX, U_MAX), "ldw_nn_decode")
 
 
-def loss_and_grad(y, tgt, npos, dyb, dyf):
+def loss_and_grad(y, tgt, npos, dyb, dyf, weights=(1.0, 1.0, 1.0, 1.0)):
     loss = ctypes.c_double()
-    _ck(lib().ldw_nn_mse(_v(y), COUT, _v(tgt), npos, _v(dyb), _v(dyf), ctypes.byref(loss)), "ldw_nn_mse")
+    w = (ctypes.c_float * 4)(*weights)
+    _ck(lib().ldw_nn_mse(_v(y), COUT, _v(tgt), npos, _v(dyb), _v(dyf), w, ctypes.byref(loss)), "ldw_nn_mse")
     return loss.value
 
 
@@ -336,8 +350,12 @@ class Dataset:
 class Trainer:
     """Adam on the GPU over batches of same-size pairs (snapshot k -> k + lag)."""
 
-    def __init__(self, net, normaliser, batch=4):
+    def __init__(self, net, normaliser, batch=4, weights=(1.0, 0.1, 0.1, 0.1), clip=1.0):
+        """weights: loss weights of (phi, ux, uy, uz) changes -- the velocity
+        over a whole jump is nearly decorrelated, so phi leads; clip: global
+        gradient-norm bound."""
         self.net, self.norm, self.batch = net, normaliser, batch
+        self.weights, self.clip = tuple(weights), clip
         self._io = {}
 
     def _bufs(self, shape, batch):
@@ -354,20 +372,25 @@ class Trainer:
             by.setdefault(tuple(data.shapes[r]), []).append((r, k))
         return by
 
-    def step(self, data, items, lr, lag=1, train=True):
+    def step(self, data, items, lr, lag=1, train=True, zero=False):
         """One optimiser step (or, train=False, only the loss) on `items`,
-        a list of (replica, snapshot) of one lattice size."""
+        a list of (replica, snapshot) of one lattice size.  zero=True scores
+        the zero-change prediction instead of the network."""
         shape = tuple(data.shapes[items[0][0]])
         ns = int(np.prod(shape))
         io = self._bufs(shape, len(items))
         for j, (r, k) in enumerate(items):
             encode(data.snaps[r][k], ns, data.specs[r], self.norm, io["x"], j * ns)
             target(data.snaps[r][k], data.snaps[r][k + lag], ns, self.norm, io["t"], j * ns)
+        if zero:
+            if "zero" not in io:
+                io["zero"] = DeviceArray.from_host(np.zeros(len(items) * ns * COUT, np.float32), np.float32)
+            return loss_and_grad(io["zero"], io["t"], len(items) * ns, io["dyb"], io["dyf"], self.weights)
         y = self.net.forward(io["x"], len(items), shape)
-        loss = loss_and_grad(y, io["t"], len(items) * ns, io["dyb"], io["dyf"])
+        loss = loss_and_grad(y, io["t"], len(items) * ns, io["dyb"], io["dyf"], self.weights)
         if train:
             self.net.backward(io["dyb"], io["dyf"])
-            self.net.adam(lr)
+            self.net.adam(lr, clip=self.clip)
         return loss
 
     def fit(self, data, pairs, iters, lr=1e-3, lag=1, seed=0, lr_final=None, warmup=50, log=None,
@@ -395,8 +418,8 @@ class Trainer:
                 break
         return losses
 
-    def evaluate(self, data, pairs, lag=1):
-        return float(np.mean([self.step(data, [p], 0.0, lag, train=False) for p in pairs]))
+    def evaluate(self, data, pairs, lag=1, zero=False):
+        return float(np.mean([self.step(data, [p], 0.0, lag, train=False, zero=zero) for p in pairs]))
 
 
 class Jumper:

@@ -107,7 +107,7 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
 
 
 def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=64, iters=2000,
-        batch=4, lr=1e-3, jumps=8, corr=20, seed=2026, max_eval=None, train_seconds=None,
+        batch=4, lr=5e-4, jumps=8, corr=20, seed=2026, max_eval=None, train_seconds=None,
         t_start=500, log=print):
     specs = draw_replicas(nrep, seed=seed, sizes=sizes)
     train_idx, val_idx = split_by_seed(specs, 0.2, seed)
@@ -132,15 +132,17 @@ def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=64, i
     vsub = vpairs if len(vpairs) <= 64 else [vpairs[i] for i in
                                               np.random.default_rng(1).choice(len(vpairs), 64, replace=False)]
     val_loss = tr.evaluate(data, vsub)
+    zero_loss = tr.evaluate(data, vsub, zero=True)
     log(f"training: {len(losses)} iterations in {t_train:.1f} s, last loss {np.mean(losses[-20:]):.4e}, "
-        f"held-out loss {val_loss:.4e} (a zero-change prediction scores ~1)")
+        f"held-out loss {val_loss:.4e} (zero-change prediction {zero_loss:.4e})")
     data.free()
     ev_specs = [specs[i] for i in val_idx if specs[i].nx == eval_size] or [specs[i] for i in val_idx]
     if max_eval:
         ev_specs = ev_specs[:max_eval]
     result = {"replicas": nrep, "steps": steps, "every": every, "data_s": t_data,
               "train_iters": len(losses), "train_s": t_train, "train_loss_last": float(np.mean(losses[-20:])),
-              "heldout_loss": val_loss, "normaliser": norm.to_dict()}
+              "heldout_loss": val_loss, "heldout_zero_change_loss": zero_loss,
+              "normaliser": norm.to_dict()}
     if ev_specs:
         result["evaluation"] = evaluate(net, norm, ev_specs, every, jumps, corr, t_start=kmin * every,
                                         log=log)
@@ -155,7 +157,7 @@ def main():
     ap.add_argument("--eval-size", type=int, default=64)
     ap.add_argument("--iters", type=int, default=2000)
     ap.add_argument("--batch", type=int, default=4)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--jumps", type=int, default=8)
     ap.add_argument("--corr", type=int, default=20)
     ap.add_argument("--max-eval", type=int, default=0)

@@ -148,7 +148,7 @@ def test_mse_and_adam():
     dy_, dt_, dg = dev(y, np.float32), dev(t, np.float32), L.DeviceArray(npos * cy, np.uint16)
     loss = ctypes.c_double()
     dgf = L.DeviceArray(npos * cy, np.float32)
-    assert lib.ldw_nn_mse(vp(dy_), cy, vp(dt_), npos, vp(dg), vp(dgf), ctypes.byref(loss)) == 0
+    assert lib.ldw_nn_mse(vp(dy_), cy, vp(dt_), npos, vp(dg), vp(dgf), None, ctypes.byref(loss)) == 0
     e = y.reshape(npos, cy)[:, :4].astype(np.float64) - t.reshape(npos, 4)
     assert loss.value == pytest.approx(np.mean(e * e), rel=1e-6)
     grad = np.zeros((npos, cy))
@@ -162,10 +162,18 @@ def test_mse_and_adam():
     lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
     wref, mref, vref = w.astype(np.float64), m.astype(np.float64), v.astype(np.float64)
     for step in (1, 2, 3):
-        assert lib.ldw_nn_adam(vp(dw), vp(dgr), vp(dm), vp(dv), n, ctypes.c_float(lr), ctypes.c_float(b1),
-                               ctypes.c_float(b2), ctypes.c_float(eps), ctypes.c_float(0.0), step) == 0
+        assert lib.ldw_nn_adam(vp(dw), vp(dgr), vp(dm), vp(dv), n, lr, b1, b2, eps, 0.0, 1.0, step) == 0
         mref = b1 * mref + (1 - b1) * g
         vref = b2 * vref + (1 - b2) * g.astype(np.float64) ** 2
         mh, vh = mref / (1 - b1 ** step), vref / (1 - b2 ** step)
         wref = wref - lr * mh / (np.sqrt(vh) + eps)
     assert np.allclose(dw.download(), wref, rtol=1e-5, atol=1e-6)
+    # weighted loss and the sum of squares used for gradient clipping
+    wts = (ctypes.c_float * 4)(1.0, 0.5, 0.25, 0.0)
+    assert lib.ldw_nn_mse(vp(dy_), cy, vp(dt_), npos, vp(dg), vp(dgf), wts, ctypes.byref(loss)) == 0
+    wv = np.array([1.0, 0.5, 0.25, 0.0])
+    assert loss.value == pytest.approx(np.mean(wv * e * e), rel=1e-6)
+    assert np.allclose(dgf.download().reshape(npos, cy)[:, :4], 2 * wv * e / (4 * npos), rtol=1e-6, atol=1e-12)
+    ss = ctypes.c_double()
+    assert lib.ldw_nn_sumsq(vp(dgr), n, ctypes.byref(ss)) == 0
+    assert ss.value == pytest.approx(float(np.sum(g.astype(np.float64) ** 2)), rel=1e-12)
