@@ -24,6 +24,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "kernels.h"
 
@@ -110,6 +111,132 @@ conv3_fwd_kernel(const uint16_t * __restrict__ x, const uint16_t * __restrict__ 
 	}
       }
     }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-tiled forward pass for lattices at least one tile in size.
+//
+// A block owns a 4 x 4 x 16 (x, y, z) tile of one sample and 16*NT output
+// channels; wave w owns the 4 z rows (x0 + w, y0 + j, z0..z0+15), j < 4,
+// i.e. 4 row blocks of 16 positions.  Input channels go in chunks of 32:
+// the chunk's 6 x 6 x 18 halo tile is staged in LDS once (periodic wrap on
+// load), so the 27 taps read their A fragments from LDS instead of
+// re-fetching them; the weights of 3 taps at a time are staged in LDS too,
+// shared by the 4 waves, and the next 3 taps' weights are fetched into
+// registers while the current ones are used.  v_mfma_f32_16x16x32_bf16:
+// a lane's A fragment is 8 consecutive channels of one position (16 bytes),
+// its B fragment 8 consecutive input channels of one output channel.
+
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+
+constexpr int LT_X = 4, LT_Y = 4, LT_Z = 16;
+constexpr int LH_X = LT_X + 2, LH_Y = LT_Y + 2, LH_Z = LT_Z + 2;
+constexpr int LH_N = LH_X*LH_Y*LH_Z;          // 648 halo positions
+constexpr int LK = 32;                        // input channels per chunk
+constexpr int LNT_MAX = 6;                    // output channel blocks per block
+
+template <int NT>
+__global__ void __launch_bounds__(256)
+conv3_fwd_lds_kernel(const uint16_t * __restrict__ x, const uint16_t * __restrict__ w,
+		     const float * __restrict__ bias, float * __restrict__ y,
+		     int nx, int ny, int nz, int cin, int cout, int accumulate,
+		     int ntx, int nty, int ntz) {
+  __shared__ uint4 sa[LH_N*(LK/8)];           // halo tile: [pos][4 x 8 channels]   41.5 KB
+  __shared__ uint4 sb[3][16*NT][LK/8];        // 3 taps x 16 NT outputs x 32 inputs
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, q = lane >> 4;
+  int t = blockIdx.x;
+  const int tz = t % ntz; t /= ntz;
+  const int ty = t % nty; t /= nty;
+  const int tx = t % ntx;
+  const int b = t / ntx;
+  const int x0 = tx*LT_X, y0 = ty*LT_Y, z0 = tz*LT_Z;
+  const int n0 = blockIdx.y*16*NT;
+  constexpr int NB = 3*16*NT*(LK/8);           // uint4 per weight stage
+  constexpr int NBT = (NB + 255)/256;          // per thread
+
+  f4 acc[4][NT];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int n = 0; n < NT; n++) acc[j][n] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  for (int c0 = 0; c0 < cin; c0 += LK) {
+    const int kc = (cin - c0 < LK) ? cin - c0 : LK;          // 16 or 32
+    for (int i = threadIdx.x; i < LH_N*4; i += 256) {
+      const int h = i >> 2, part = i & 3;
+      const int hz = h % LH_Z, hy = (h / LH_Z) % LH_Y, hx = h / (LH_Z*LH_Y);
+      uint4 v = {0u, 0u, 0u, 0u};
+      if (8*part < kc) {
+	const int gx = wrapi(x0 + hx - 1, nx), gy = wrapi(y0 + hy - 1, ny), gz = wrapi(z0 + hz - 1, nz);
+	v = *reinterpret_cast<const uint4 *>(x + ((((int64_t) b*nx + gx)*ny + gy)*nz + gz)*cin + c0 + 8*part);
+      }
+      sa[i] = v;
+    }
+    // weights of taps 3s .. 3s+2: [tap][n][part]
+    uint4 pre[NBT];
+    auto fetch = [&](int st) {
+#pragma unroll
+      for (int k = 0; k < NBT; k++) {
+	const int i = threadIdx.x + 256*k;
+	uint4 v = {0u, 0u, 0u, 0u};
+	if (i < NB) {
+	  const int part = i & 3, n = (i >> 2) % (16*NT), tp = (i >> 2)/(16*NT);
+	  if (8*part < kc)
+	    v = *reinterpret_cast<const uint4 *>(w + ((int64_t) (3*st + tp)*cout + n0 + n)*cin + c0 + 8*part);
+	}
+	pre[k] = v;
+      }
+    };
+    fetch(0);
+    for (int st = 0; st < 9; st++) {
+#pragma unroll
+      for (int k = 0; k < NBT; k++) {
+	const int i = threadIdx.x + 256*k;
+	if (i < NB) (&sb[0][0][0])[i] = pre[k];
+      }
+      __syncthreads();                        // halo (first stage) and weights in place
+      if (st + 1 < 9) fetch(st + 1);
+      const int dx = st/3, dy = st%3;
+#pragma unroll
+      for (int dz = 0; dz < 3; dz++) {
+	s8 bv[NT];
+#pragma unroll
+	for (int n = 0; n < NT; n++) bv[n] = *reinterpret_cast<const s8 *>(&sb[dz][16*n + r][q]);
+#pragma unroll
+	for (int j = 0; j < 4; j++) {
+	  const int h = ((wave + dx)*LH_Y + (j + dy))*LH_Z + (r + dz);
+	  const s8 a = *reinterpret_cast<const s8 *>(&sa[h*4 + q]);
+#pragma unroll
+	  for (int n = 0; n < NT; n++)
+	    acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a),
+								 __builtin_bit_cast(b8, bv[n]), acc[j][n], 0, 0, 0);
+	}
+      }
+      __syncthreads();                        // everyone done with this stage's weights
+    }
+  }
+  const int gx = x0 + wave;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int gy = y0 + j;
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+      const int ch = n0 + 16*n + r;
+      const float bv = bias ? bias[ch] : 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+	const int gz = z0 + 4*q + i;
+	if (gx < nx && gy < ny && gz < nz) {
+	  const int64_t p = (((int64_t) b*nx + gx)*ny + gy)*nz + gz;
+	  float v = acc[j][n][i] + bv;
+	  if (accumulate) v += y[p*cout + ch];
+	  y[p*cout + ch] = v;
+	}
+      }
+    }
+  }
 }
 
 constexpr int WG_P = 64;          // positions per LDS slice
@@ -226,6 +353,16 @@ __global__ void to_bf16_kernel(const float * __restrict__ a, uint16_t * __restri
     b[i] = to_bf16(a[i]);
 }
 
+// LDW_CONV_LDS=0 selects the fragment-direct kernel everywhere (for A/B timing)
+int conv_lds_enabled() {
+  static int on = -1;
+  if (on < 0) {
+    const char * e = getenv("LDW_CONV_LDS");
+    on = (e && atoi(e) == 0) ? 0 : 1;
+  }
+  return on;
+}
+
 int grid_for(int64_t n) {
   int64_t g = (n + 255)/256;
   return (int) (g < 8192 ? (g > 0 ? g : 1) : 8192);
@@ -238,6 +375,24 @@ hipError_t launch_conv3_fwd(const uint16_t * x, const uint16_t * w, const float 
 			    hipStream_t s) {
   const int64_t npos = (int64_t) batch*nx*ny*nz;
   const int nb16 = cout/16;
+  if (nz >= LT_Z && ny >= LT_Y && nx >= LT_X && conv_lds_enabled()) {
+    int nt = 1;
+    for (int c = LNT_MAX; c >= 1; c--) if (nb16 % c == 0) { nt = c; break; }
+    const int ntx = (nx + LT_X - 1)/LT_X, nty = (ny + LT_Y - 1)/LT_Y, ntz = (nz + LT_Z - 1)/LT_Z;
+    dim3 grid((unsigned) ((int64_t) batch*ntx*nty*ntz), nb16/nt), block(256);
+#define LDS_(NT) hipLaunchKernelGGL(conv3_fwd_lds_kernel<NT>, grid, block, 0, s, x, w, bias, y, nx, ny, nz, \
+				    cin, cout, accumulate, ntx, nty, ntz)
+    switch (nt) {
+    case 6: LDS_(6); break;
+    case 5: LDS_(5); break;
+    case 4: LDS_(4); break;
+    case 3: LDS_(3); break;
+    case 2: LDS_(2); break;
+    default: LDS_(1); break;
+    }
+#undef LDS_
+    return hipGetLastError();
+  }
   const int nt = (nb16 % 4 == 0) ? 4 : (nb16 % 3 == 0) ? 3 : (nb16 % 2 == 0) ? 2 : 1;
   dim3 grid((unsigned) ((npos + 127)/128), nb16/nt), block(256);
   switch (nt) {

@@ -40,9 +40,15 @@ def within(got, want, bound, what):
                     f"(bound {bound[i]:.3e})")
 
 
-@pytest.mark.parametrize("cin,cout,batch", [(16, 16, 1), (32, 48, 2), (48, 64, 1), (16, 96, 2),
-                                            (32, 160, 1), (160, 32, 1)])
-def test_conv_fwd_bwd_against_oracle(cin, cout, batch):
+# lattices below 4 x 4 x 16 take the fragment-direct kernel, the rest the
+# LDS-tiled one (ragged tiles and 16-channel chunks included)
+@pytest.mark.parametrize("cin,cout,batch,shape", [
+    (16, 16, 1, SHAPE), (32, 48, 2, SHAPE), (48, 64, 1, SHAPE), (16, 96, 2, SHAPE),
+    (32, 160, 1, SHAPE), (160, 32, 1, SHAPE),
+    (32, 96, 2, (8, 4, 16)), (48, 64, 1, (5, 6, 20)), (16, 16, 1, (4, 4, 32)),
+    (80, 112, 1, (6, 5, 17)), (160, 64, 1, (4, 8, 16))])
+def test_conv_fwd_bwd_against_oracle(cin, cout, batch, shape):
+    SHAPE = shape
     lib = L.lib()
     rng = np.random.default_rng(cin * 1000 + cout)
     n = batch * int(np.prod(SHAPE))
