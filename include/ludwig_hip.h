@@ -469,6 +469,10 @@ int ldw_ensemble_load_device(ldw_ensemble_t * ens, int r, const float * src);
  *   dw_t = sum_p dy(p) x(p + o_t)^T, cin, cout <= 160        (ldw_conv3d_bwd_weight) */
 int ldw_conv3d_fwd(const void * x, const void * w, const float * bias, float * y,
 		   int batch, int nx, int ny, int nz, int cin, int cout, int accumulate);
+/* the forward pass with SiLU fused: z (may be NULL) = bias + conv,
+ * a = bf16(SiLU(z)) */
+int ldw_conv3d_fwd_silu(const void * x, const void * w, const float * bias, float * z, void * a,
+			int batch, int nx, int ny, int nz, int cin, int cout);
 int ldw_conv3d_bwd_data(const void * dy, const void * wd, float * dx, int batch, int nx,
 			int ny, int nz, int cin, int cout, int accumulate);
 int ldw_conv3d_bwd_weight(const void * x, const void * dy, float * dw, int batch, int nx,

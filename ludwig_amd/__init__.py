@@ -136,6 +136,7 @@ def lib():
         "ldw_memcpy": [vp, vp, ctypes.c_longlong, ctypes.c_int],
         # surrogate network kernels (device pointers)
         "ldw_conv3d_fwd": [vp, vp, vp, vp] + [ctypes.c_int] * 7,
+        "ldw_conv3d_fwd_silu": [vp, vp, vp, vp, vp] + [ctypes.c_int] * 6,
         "ldw_conv3d_bwd_data": [vp, vp, vp] + [ctypes.c_int] * 7,
         "ldw_conv3d_bwd_weight": [vp, vp, vp] + [ctypes.c_int] * 7,
         "ldw_conv3d_weights_prep": [vp, vp, vp, ctypes.c_int, ctypes.c_int],

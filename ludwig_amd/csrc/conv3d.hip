@@ -41,6 +41,23 @@ __device__ __forceinline__ int wrapi(int i, int n) {
 
 struct Pos { int b, x, y, z; };
 
+__device__ __forceinline__ uint16_t silu_bf16(float v) {
+  uint32_t u = __float_as_uint(v/(1.0f + expf(-v)));
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t) ((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t) (u >> 16);
+}
+
+// epilogue: y = acc + bias (+ y) if y is given; a = bf16(SiLU(that)) if a is given
+__device__ __forceinline__ void store_out(float * __restrict__ y, uint16_t * __restrict__ a, int64_t idx,
+					  float v, int accumulate) {
+  if (y) {
+    if (accumulate) v += y[idx];
+    y[idx] = v;
+  }
+  if (a) a[idx] = silu_bf16(v);
+}
+
 __device__ __forceinline__ Pos pos_of(int64_t p, int nx, int ny, int nz) {
   Pos q;
   q.z = (int) (p % nz); p /= nz;
@@ -58,8 +75,8 @@ __device__ __forceinline__ int64_t shifted(const Pos & q, int t, int nx, int ny,
 template <int NT16>
 __global__ void __launch_bounds__(256)
 conv3_fwd_kernel(const uint16_t * __restrict__ x, const uint16_t * __restrict__ w,
-		 const float * __restrict__ bias, float * __restrict__ y, int64_t npos,
-		 int nx, int ny, int nz, int cin, int cout, int accumulate) {
+		 const float * __restrict__ bias, float * __restrict__ y, uint16_t * __restrict__ act,
+		 int64_t npos, int nx, int ny, int nz, int cin, int cout, int accumulate) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 15, kq = lane >> 4;
   const int64_t m0 = (int64_t) blockIdx.x*128 + wave*32;
@@ -104,11 +121,7 @@ conv3_fwd_kernel(const uint16_t * __restrict__ x, const uint16_t * __restrict__ 
 #pragma unroll
       for (int i = 0; i < 4; i++) {
 	const int64_t p = m0 + 16*rb + 4*kq + i;
-	if (p < npos) {
-	  float v = acc[rb][nb][i] + bv;
-	  if (accumulate) v += y[p*cout + ch];
-	  y[p*cout + ch] = v;
-	}
+	if (p < npos) store_out(y, act, p*cout + ch, acc[rb][nb][i] + bv, accumulate);
       }
     }
 }
@@ -139,7 +152,7 @@ constexpr int LNT_MAX = 6;                    // output channel blocks per block
 template <int NT>
 __global__ void __launch_bounds__(256)
 conv3_fwd_lds_kernel(const uint16_t * __restrict__ x, const uint16_t * __restrict__ w,
-		     const float * __restrict__ bias, float * __restrict__ y,
+		     const float * __restrict__ bias, float * __restrict__ y, uint16_t * __restrict__ act,
 		     int nx, int ny, int nz, int cin, int cout, int accumulate,
 		     int ntx, int nty, int ntz) {
   __shared__ uint4 sa[LH_N*(LK/8)];           // halo tile: [pos][4 x 8 channels]   41.5 KB
@@ -230,9 +243,7 @@ conv3_fwd_lds_kernel(const uint16_t * __restrict__ x, const uint16_t * __restric
 	const int gz = z0 + 4*q + i;
 	if (gx < nx && gy < ny && gz < nz) {
 	  const int64_t p = (((int64_t) b*nx + gx)*ny + gy)*nz + gz;
-	  float v = acc[j][n][i] + bv;
-	  if (accumulate) v += y[p*cout + ch];
-	  y[p*cout + ch] = v;
+	  store_out(y, act, p*cout + ch, acc[j][n][i] + bv, accumulate);
 	}
       }
     }
@@ -371,8 +382,8 @@ int grid_for(int64_t n) {
 }  // namespace
 
 hipError_t launch_conv3_fwd(const uint16_t * x, const uint16_t * w, const float * bias, float * y,
-			    int batch, int nx, int ny, int nz, int cin, int cout, int accumulate,
-			    hipStream_t s) {
+			    uint16_t * act, int batch, int nx, int ny, int nz, int cin, int cout,
+			    int accumulate, hipStream_t s) {
   const int64_t npos = (int64_t) batch*nx*ny*nz;
   const int nb16 = cout/16;
   if (nz >= LT_Z && ny >= LT_Y && nx >= LT_X && conv_lds_enabled()) {
@@ -380,7 +391,7 @@ hipError_t launch_conv3_fwd(const uint16_t * x, const uint16_t * w, const float 
     for (int c = LNT_MAX; c >= 1; c--) if (nb16 % c == 0) { nt = c; break; }
     const int ntx = (nx + LT_X - 1)/LT_X, nty = (ny + LT_Y - 1)/LT_Y, ntz = (nz + LT_Z - 1)/LT_Z;
     dim3 grid((unsigned) ((int64_t) batch*ntx*nty*ntz), nb16/nt), block(256);
-#define LDS_(NT) hipLaunchKernelGGL(conv3_fwd_lds_kernel<NT>, grid, block, 0, s, x, w, bias, y, nx, ny, nz, \
+#define LDS_(NT) hipLaunchKernelGGL(conv3_fwd_lds_kernel<NT>, grid, block, 0, s, x, w, bias, y, act, nx, ny, nz, \
 				    cin, cout, accumulate, ntx, nty, ntz)
     switch (nt) {
     case 6: LDS_(6); break;
@@ -396,10 +407,10 @@ hipError_t launch_conv3_fwd(const uint16_t * x, const uint16_t * w, const float 
   const int nt = (nb16 % 4 == 0) ? 4 : (nb16 % 3 == 0) ? 3 : (nb16 % 2 == 0) ? 2 : 1;
   dim3 grid((unsigned) ((npos + 127)/128), nb16/nt), block(256);
   switch (nt) {
-  case 4: hipLaunchKernelGGL(conv3_fwd_kernel<4>, grid, block, 0, s, x, w, bias, y, npos, nx, ny, nz, cin, cout, accumulate); break;
-  case 3: hipLaunchKernelGGL(conv3_fwd_kernel<3>, grid, block, 0, s, x, w, bias, y, npos, nx, ny, nz, cin, cout, accumulate); break;
-  case 2: hipLaunchKernelGGL(conv3_fwd_kernel<2>, grid, block, 0, s, x, w, bias, y, npos, nx, ny, nz, cin, cout, accumulate); break;
-  default: hipLaunchKernelGGL(conv3_fwd_kernel<1>, grid, block, 0, s, x, w, bias, y, npos, nx, ny, nz, cin, cout, accumulate); break;
+  case 4: hipLaunchKernelGGL(conv3_fwd_kernel<4>, grid, block, 0, s, x, w, bias, y, act, npos, nx, ny, nz, cin, cout, accumulate); break;
+  case 3: hipLaunchKernelGGL(conv3_fwd_kernel<3>, grid, block, 0, s, x, w, bias, y, act, npos, nx, ny, nz, cin, cout, accumulate); break;
+  case 2: hipLaunchKernelGGL(conv3_fwd_kernel<2>, grid, block, 0, s, x, w, bias, y, act, npos, nx, ny, nz, cin, cout, accumulate); break;
+  default: hipLaunchKernelGGL(conv3_fwd_kernel<1>, grid, block, 0, s, x, w, bias, y, act, npos, nx, ny, nz, cin, cout, accumulate); break;
   }
   return hipGetLastError();
 }

@@ -164,9 +164,11 @@ void launch_to_f32(const void * a, int precision, float * b, int64_t n, hipStrea
 // 3x3x3 periodic convolutions, BF16 operands (raw 16-bit), FP32 results
 // (conv3d.hip).  Channels-last activations; channel counts multiples of 16,
 // at most 160 for the weight gradient.  Weights W[27][cout][cin].
+// y (may be null) = bias + conv (+ y when accumulate); act (may be null) =
+// bf16(SiLU(that value)) -- the activation fused into the epilogue
 hipError_t launch_conv3_fwd(const uint16_t * x, const uint16_t * w, const float * bias, float * y,
-			    int batch, int nx, int ny, int nz, int cin, int cout, int accumulate,
-			    hipStream_t s);
+			    uint16_t * act, int batch, int nx, int ny, int nz, int cin, int cout,
+			    int accumulate, hipStream_t s);
 // floats of workspace for launch_conv3_wgrad
 int64_t conv3_wgrad_workspace(int batch, int nx, int ny, int nz, int cin, int cout, int * nchunk,
 			      int64_t * chunk);

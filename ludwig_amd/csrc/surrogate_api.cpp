@@ -54,8 +54,19 @@ extern "C" int ldw_conv3d_fwd(const void * x, const void * w, const float * bias
 			      int batch, int nx, int ny, int nz, int cin, int cout, int accumulate) {
   READY();
   if (check_conv(batch, nx, ny, nz, cin, cout)) return -1;
-  KCHECK(ldw::launch_conv3_fwd((const uint16_t *) x, (const uint16_t *) w, bias, y, batch, nx, ny, nz,
-			       cin, cout, accumulate, ldw::api::stream()));
+  if (y == nullptr) LDW_FAIL("ldw_conv3d_fwd: null output");
+  KCHECK(ldw::launch_conv3_fwd((const uint16_t *) x, (const uint16_t *) w, bias, y, nullptr, batch, nx,
+			       ny, nz, cin, cout, accumulate, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_conv3d_fwd_silu(const void * x, const void * w, const float * bias, float * z, void * a,
+				   int batch, int nx, int ny, int nz, int cin, int cout) {
+  READY();
+  if (check_conv(batch, nx, ny, nz, cin, cout)) return -1;
+  if (a == nullptr) LDW_FAIL("ldw_conv3d_fwd_silu: null activation output");
+  KCHECK(ldw::launch_conv3_fwd((const uint16_t *) x, (const uint16_t *) w, bias, z, (uint16_t *) a, batch,
+			       nx, ny, nz, cin, cout, 0, ldw::api::stream()));
   return 0;
 }
 
@@ -65,8 +76,8 @@ extern "C" int ldw_conv3d_bwd_data(const void * dy, const void * wd, float * dx,
   if (check_conv(batch, nx, ny, nz, cin, cout)) return -1;
   // dx (cin channels) = convolution of dy (cout channels) with the flipped,
   // transposed weights
-  KCHECK(ldw::launch_conv3_fwd((const uint16_t *) dy, (const uint16_t *) wd, nullptr, dx, batch, nx,
-			       ny, nz, cout, cin, accumulate, ldw::api::stream()));
+  KCHECK(ldw::launch_conv3_fwd((const uint16_t *) dy, (const uint16_t *) wd, nullptr, dx, nullptr, batch,
+			       nx, ny, nz, cout, cin, accumulate, ldw::api::stream()));
   return 0;
 }
 

@@ -164,21 +164,28 @@ class UNet:
 
     # -- passes -----------------------------------------------------------
     def _conv(self, b, name, x, act=True):
+        """Convolution with SiLU fused into its epilogue; the pre-activation
+        is kept only when a backward pass will follow."""
         L = lib()
         _, ci, co = self.layout[name]
         l = LEVEL[name]
         nx, ny, nz = b["shape"][l]
         batch = b["batch"]
-        _ck(L.ldw_conv3d_fwd(_v(x), _v(self.wf[name]), self.bptr(name), _v(b["z_" + name]), batch,
-                             nx, ny, nz, ci, co, 0), f"conv {name}")
         if act:
-            _ck(L.ldw_nn_silu_fwd(_v(b["z_" + name]), _v(b["a_" + name]), b["P"][l] * co), f"silu {name}")
+            z = _v(b["z_" + name]) if b["keep"] else None
+            _ck(L.ldw_conv3d_fwd_silu(_v(x), _v(self.wf[name]), self.bptr(name), z, _v(b["a_" + name]),
+                                      batch, nx, ny, nz, ci, co), f"conv {name}")
+        else:
+            _ck(L.ldw_conv3d_fwd(_v(x), _v(self.wf[name]), self.bptr(name), _v(b["z_" + name]), batch,
+                                 nx, ny, nz, ci, co, 0), f"conv {name}")
 
-    def forward(self, x, batch, shape):
+    def forward(self, x, batch, shape, keep=True):
         """x: device BF16 [batch*nsite][16].  Returns the FP32 output buffer
-        [batch*nsite][16] (channels 0..3 used)."""
+        [batch*nsite][16] (channels 0..3 used).  keep=False (inference)
+        skips storing what only the backward pass needs."""
         L = lib()
         b = self.buffers(batch, shape)
+        b["keep"] = keep
         b["batch"] = batch
         b["x"] = x
         sh = b["shape"]
@@ -224,6 +231,8 @@ class UNet:
         (BF16) and dyf (FP32), both [batch*nsite][16]."""
         L = lib()
         b = self._last
+        if not b["keep"]:
+            raise RuntimeError("backward after an inference-only forward pass")
         batch, sh, P = b["batch"], b["shape"], b["P"]
         self._conv_back(b, "out", b["a_d0"], dyb, dyf, b["da_d0"])
         for l, low, skip, name in ((0, "d1", "e0b", "d0"), (1, "d2", "e1b", "d1"), (2, "b1", "e2b", "d2")):
@@ -452,7 +461,7 @@ class Jumper:
             for j, r in enumerate(rs):
                 ensemble.snapshot_device(r, io["now"][j].ptr)
                 encode(io["now"][j], ns, ensemble.specs[r], self.norm, io["x"], j * ns)
-            y = self.net.forward(io["x"], len(rs), shape)
+            y = self.net.forward(io["x"], len(rs), shape, keep=False)
             for j, r in enumerate(rs):
                 decode(io["now"][j], y, ns, self.norm, io["out"], j * ns)
                 ensemble.load_device(r, io["out"].ptr)

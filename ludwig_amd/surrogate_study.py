@@ -15,8 +15,10 @@ Protocol:
      identity + LB (baseline)  the same without the network, i.e. jumps
                                that leave the state unchanged.
    Each is judged by the domain length L(T) from the structure factor
-   (ludwig_amd.analysis) and the pointwise phi error against plain LB, and
-   timed on the GPU (all held-out replicas advance together).
+   (ludwig_amd.analysis), per replica and for the ensemble mean, and the
+   pointwise phi error against plain LB, and timed on the GPU (all held-out
+   replicas advance together).  A control run of plain LB in FP64 storage
+   shows how far two faithful runs drift apart on this chaotic problem.
 
 Run as a script for the full protocol; prints one JSON object.
 """
@@ -59,6 +61,12 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
     out["plain_s"] = time.perf_counter() - t0
     ref = [plain.snapshot(r)[0] for r in range(len(specs))]
     plain.close()
+    # control: the same plain run in FP64 storage -- how far two faithful
+    # lattice Boltzmann runs drift apart on this chaotic problem
+    ctl = Ensemble(specs, precision=64, fatal=True)
+    ctl.step(t_start + T)
+    ref64 = [ctl.snapshot(r)[0] for r in range(len(specs))]
+    ctl.close()
 
     def jumped(with_net):
         e = Ensemble(specs, precision=32, fatal=True)
@@ -89,7 +97,7 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
         shape = (s.nx, s.ny, s.nz)
         Lr = domain_length(ref[r], shape)
         row = {"replica": r, "L_plain": Lr}
-        for name, phis in (("surrogate", sur), ("identity", ide)):
+        for name, phis in (("surrogate", sur), ("identity", ide), ("fp64_control", ref64)):
             Ls = domain_length(phis[r], shape)
             dev = ref[r] - ref[r].mean()
             row[f"L_{name}"] = Ls
@@ -98,9 +106,14 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
                                                / max(np.sqrt(np.mean(dev ** 2)), 1e-30))
         rows.append(row)
     out["per_replica"] = rows
-    for k in ("L_err_surrogate", "L_err_identity", "phi_rms_err_surrogate", "phi_rms_err_identity"):
+    for k in ("L_err_surrogate", "L_err_identity", "L_err_fp64_control", "phi_rms_err_surrogate",
+              "phi_rms_err_identity", "phi_rms_err_fp64_control"):
         out[k + "_mean"] = float(np.mean([row[k] for row in rows]))
         out[k + "_max"] = float(np.max([row[k] for row in rows]))
+    # error of the ensemble-mean domain length
+    Lm = float(np.mean([row["L_plain"] for row in rows]))
+    for name in ("surrogate", "identity", "fp64_control"):
+        out[f"meanL_err_{name}"] = abs(float(np.mean([row[f"L_{name}"] for row in rows])) - Lm) / Lm
     out["speedup"] = out["plain_s"] / out["surrogate_s"]
     log(f"evaluation: {json.dumps({k: v for k, v in out.items() if k != 'per_replica'})}")
     return out
@@ -108,8 +121,8 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
 
 def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=64, iters=2000,
         batch=4, lr=5e-4, jumps=8, corr=20, seed=2026, max_eval=None, train_seconds=None,
-        t_start=500, log=print):
-    specs = draw_replicas(nrep, seed=seed, sizes=sizes)
+        t_start=500, replica_kw=None, log=print):
+    specs = draw_replicas(nrep, seed=seed, sizes=sizes, **(replica_kw or {}))
     train_idx, val_idx = split_by_seed(specs, 0.2, seed)
     t0 = time.perf_counter()
     ens = Ensemble(specs, precision=32, fatal=True)

@@ -80,6 +80,16 @@ def test_conv_fwd_bwd_against_oracle(cin, cout, batch, shape):
     dwr = O.conv3_bwd_weight(x, g, SHAPE, cin, cout, batch)
     dwa = O.conv3_bwd_weight(np.abs(x), np.abs(g), SHAPE, cin, cout, batch)
     within(dwo.download().astype(np.float64), dwr, 2 * n * EPS32 * dwa + 1e-30, "backward weight")
+    # SiLU fused into the epilogue: the same pre-activation and, bit for bit,
+    # what the standalone SiLU kernel makes of it
+    z2, a2, a1 = L.DeviceArray(n * cout, np.float32), L.DeviceArray(n * cout, np.uint16), \
+        L.DeviceArray(n * cout, np.uint16)
+    assert lib.ldw_conv3d_fwd_silu(vp(dx_in), vp(wf), vp(db), vp(z2), vp(a2), batch, nx, ny, nz, cin, cout) == 0
+    assert np.array_equal(z2.download(), y.download())
+    assert lib.ldw_nn_silu_fwd(vp(y), vp(a1), n * cout) == 0
+    assert np.array_equal(a2.download(), a1.download())
+    assert lib.ldw_conv3d_fwd_silu(vp(dx_in), vp(wf), vp(db), None, vp(a2), batch, nx, ny, nz, cin, cout) == 0
+    assert np.array_equal(a2.download(), a1.download())
     # accumulate = 1 adds to what is there
     assert lib.ldw_conv3d_fwd(vp(dx_in), vp(wf), vp(db), vp(y), batch, nx, ny, nz, cin, cout, 1) == 0
     within(y.download().astype(np.float64), 2 * yo, 4 * 27 * cin * EPS32 * ya + 1e-30, "forward x2")
