@@ -514,6 +514,8 @@ int ldw_nn_adam(float * w, const float * g, float * m, float * v, long long n, f
 		float b1, float b2, float eps, float wd, float gscale, int step);
 /* *out = sum a[i]^2 in FP64 (fixed order), complete on return */
 int ldw_nn_sumsq(const float * a, long long n, double * out);
+/* a[i] *= f */
+int ldw_nn_scale(float * a, long long n, float f);
 
 #ifdef __cplusplus
 }

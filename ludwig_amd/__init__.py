@@ -158,6 +158,7 @@ def lib():
                        ctypes.POINTER(ctypes.c_double)],
         "ldw_nn_adam": [vp, vp, vp, vp, ctypes.c_longlong] + [ctypes.c_float] * 6 + [ctypes.c_int],
         "ldw_nn_sumsq": [vp, ctypes.c_longlong, ctypes.POINTER(ctypes.c_double)],
+        "ldw_nn_scale": [vp, ctypes.c_longlong, ctypes.c_float],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -217,6 +217,7 @@ hipError_t nn_mse(const float * y, int cy, const float * tgt, int64_t npos, floa
 		  float * dyf, const ChannelWeights & w, hipStream_t s);
 hipError_t nn_adam(float * w, const float * g, float * m, float * v, int64_t n, float lr_t,
 		   float b1, float b2, float eps, float wd, float gscale, hipStream_t s);
+hipError_t nn_scale(float * a, int64_t n, float f, hipStream_t s);
 // sum of a[i]^2 in FP64 into work[0]; work holds 1 + 512 doubles
 hipError_t nn_sumsq(const float * a, int64_t n, double * work, hipStream_t s);
 

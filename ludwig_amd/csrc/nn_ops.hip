@@ -252,6 +252,10 @@ __global__ void adam_kernel(float * __restrict__ w, const float * __restrict__ g
   }
 }
 
+__global__ void scale_kernel(float * __restrict__ a, int64_t n, float s) {
+  GRID_STRIDE(i, n) a[i] *= s;
+}
+
 // per-block sums of squares in FP64 (fixed reduction order)
 __global__ void sumsq_kernel(const float * __restrict__ a, int64_t n, double * __restrict__ part) {
   __shared__ double sh[256];
@@ -378,6 +382,11 @@ hipError_t nn_mse(const float * y, int cy, const float * tgt, int64_t npos, floa
 hipError_t nn_adam(float * w, const float * g, float * m, float * v, int64_t n, float lr_t,
 		   float b1, float b2, float eps, float wd, float gscale, hipStream_t s) {
   LAUNCH(adam_kernel, n, w, g, m, v, n, lr_t, b1, b2, eps, wd, gscale);
+  return hipGetLastError();
+}
+
+hipError_t nn_scale(float * a, int64_t n, float f, hipStream_t s) {
+  LAUNCH(scale_kernel, n, a, n, f);
   return hipGetLastError();
 }
 

@@ -230,6 +230,12 @@ extern "C" int ldw_nn_adam(float * w, const float * g, float * m, float * v, lon
   return 0;
 }
 
+extern "C" int ldw_nn_scale(float * a, long long n, float f) {
+  READY();
+  KCHECK(ldw::nn_scale(a, n, f, ldw::api::stream()));
+  return 0;
+}
+
 extern "C" int ldw_nn_sumsq(const float * a, long long n, double * out) {
   READY();
   if (out == nullptr) LDW_FAIL("ldw_nn_sumsq: null output");

@@ -114,6 +114,19 @@ class UNet:
     def grads(self):
         return self.g.download()
 
+    def save(self, path):
+        """Weights and optimiser state (numpy .npz)."""
+        np.savez(path, w=self.w.download(), m=self.m.download(), v=self.v.download(),
+                 step=np.array(self.step_count))
+
+    def load(self, path):
+        d = np.load(path)
+        if d["w"].size != self.nparam:
+            raise ValueError(f"{path}: {d['w'].size} parameters, the network has {self.nparam}")
+        self.w.upload(d["w"]); self.m.upload(d["m"]); self.v.upload(d["v"])
+        self.step_count = int(d["step"])
+        self.prepare()
+
     def grad_norm(self):
         out = ctypes.c_double()
         _ck(lib().ldw_nn_sumsq(_v(self.g), self.nparam, ctypes.byref(out)), "ldw_nn_sumsq")
@@ -434,10 +447,17 @@ class Trainer:
 class Jumper:
     """Surrogate jumps of whole ensembles: every replica's (phi, u) is
     replaced by the network's prediction, distributions by the local
-    equilibrium; plain lattice Boltzmann steps then correct it."""
+    equilibrium; plain lattice Boltzmann steps then correct it.
 
-    def __init__(self, net, normaliser):
-        self.net, self.norm = net, normaliser
+    u_mode: "pred" takes the predicted velocity; "rescale" rescales it to
+    the root-mean-square speed before the jump (a mean-square-trained
+    prediction shrinks the part of u it cannot predict); "keep" keeps the
+    velocity from before the jump."""
+
+    def __init__(self, net, normaliser, u_mode="pred"):
+        if u_mode not in ("pred", "rescale", "keep"):
+            raise ValueError(f"u_mode {u_mode!r}")
+        self.net, self.norm, self.u_mode = net, normaliser, u_mode
         self._io = {}
 
     def _bufs(self, shape, batch):
@@ -464,4 +484,14 @@ class Jumper:
             y = self.net.forward(io["x"], len(rs), shape, keep=False)
             for j, r in enumerate(rs):
                 decode(io["now"][j], y, ns, self.norm, io["out"], j * ns)
+                if self.u_mode == "keep":
+                    _ck(lib().ldw_memcpy(_off(io["out"], 4 * ns), _off(io["now"][j], 4 * ns), 12 * ns, 3),
+                        "ldw_memcpy")
+                elif self.u_mode == "rescale":
+                    a, b = ctypes.c_double(), ctypes.c_double()
+                    _ck(lib().ldw_nn_sumsq(_off(io["now"][j], 4 * ns), 3 * ns, ctypes.byref(a)), "sumsq")
+                    _ck(lib().ldw_nn_sumsq(_off(io["out"], 4 * ns), 3 * ns, ctypes.byref(b)), "sumsq")
+                    if b.value > 0.0:
+                        _ck(lib().ldw_nn_scale(_off(io["out"], 4 * ns), 3 * ns, math.sqrt(a.value / b.value)),
+                            "ldw_nn_scale")
                 ensemble.load_device(r, io["out"].ptr)

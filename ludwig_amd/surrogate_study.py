@@ -68,11 +68,11 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
     ref64 = [ctl.snapshot(r)[0] for r in range(len(specs))]
     ctl.close()
 
-    def jumped(with_net):
+    def jumped(with_net, u_mode="pred"):
         e = Ensemble(specs, precision=32, fatal=True)
         e.tune(reps=2)
         e.step(t_start)
-        jumper = Jumper(net, norm)
+        jumper = Jumper(net, norm, u_mode)
         if with_net:                 # warm the buffers and kernels once
             e2 = Ensemble(specs, precision=32, fatal=True)
             e2.step(1)
@@ -92,12 +92,15 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
 
     sur, out["surrogate_s"] = jumped(True)
     ide, out["identity_s"] = jumped(False)
+    sur_r, out["surrogate_u_rescaled_s"] = jumped(True, "rescale")
+    sur_k, out["surrogate_u_kept_s"] = jumped(True, "keep")
     rows = []
     for r, s in enumerate(specs):
         shape = (s.nx, s.ny, s.nz)
         Lr = domain_length(ref[r], shape)
         row = {"replica": r, "L_plain": Lr}
-        for name, phis in (("surrogate", sur), ("identity", ide), ("fp64_control", ref64)):
+        for name, phis in (("surrogate", sur), ("surrogate_u_rescaled", sur_r), ("surrogate_u_kept", sur_k),
+                           ("identity", ide), ("fp64_control", ref64)):
             Ls = domain_length(phis[r], shape)
             dev = ref[r] - ref[r].mean()
             row[f"L_{name}"] = Ls
@@ -106,22 +109,23 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
                                                / max(np.sqrt(np.mean(dev ** 2)), 1e-30))
         rows.append(row)
     out["per_replica"] = rows
-    for k in ("L_err_surrogate", "L_err_identity", "L_err_fp64_control", "phi_rms_err_surrogate",
-              "phi_rms_err_identity", "phi_rms_err_fp64_control"):
+    names = ("surrogate", "surrogate_u_rescaled", "surrogate_u_kept", "identity", "fp64_control")
+    for k in [f"L_err_{n}" for n in names] + [f"phi_rms_err_{n}" for n in names]:
         out[k + "_mean"] = float(np.mean([row[k] for row in rows]))
         out[k + "_max"] = float(np.max([row[k] for row in rows]))
     # error of the ensemble-mean domain length
     Lm = float(np.mean([row["L_plain"] for row in rows]))
-    for name in ("surrogate", "identity", "fp64_control"):
+    for name in names:
         out[f"meanL_err_{name}"] = abs(float(np.mean([row[f"L_{name}"] for row in rows])) - Lm) / Lm
     out["speedup"] = out["plain_s"] / out["surrogate_s"]
+    out["speedup_u_rescaled"] = out["plain_s"] / out["surrogate_u_rescaled_s"]
     log(f"evaluation: {json.dumps({k: v for k, v in out.items() if k != 'per_replica'})}")
     return out
 
 
 def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=64, iters=2000,
         batch=4, lr=5e-4, jumps=8, corr=20, seed=2026, max_eval=None, train_seconds=None,
-        t_start=500, replica_kw=None, log=print):
+        t_start=500, replica_kw=None, init_from=None, save_to=None, log=print):
     specs = draw_replicas(nrep, seed=seed, sizes=sizes, **(replica_kw or {}))
     train_idx, val_idx = split_by_seed(specs, 0.2, seed)
     t0 = time.perf_counter()
@@ -135,6 +139,9 @@ def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=64, i
     kmin = (t_start + every - 1) // every
     norm = data.normaliser(sample=min(32, len(data.pairs(kmin=kmin))), kmin=kmin)
     net = UNet(seed=seed)
+    if init_from:
+        net.load(init_from)
+        log(f"weights from {init_from} (after {net.step_count} steps)")
     tr = Trainer(net, norm, batch=batch)
     tpairs = [(r, k) for (r, k) in data.pairs(kmin=kmin) if r in set(train_idx)]
     vpairs = [(r, k) for (r, k) in data.pairs(kmin=kmin) if r in set(val_idx)]
@@ -149,6 +156,8 @@ def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=64, i
     log(f"training: {len(losses)} iterations in {t_train:.1f} s, last loss {np.mean(losses[-20:]):.4e}, "
         f"held-out loss {val_loss:.4e} (zero-change prediction {zero_loss:.4e})")
     data.free()
+    if save_to:
+        net.save(save_to)
     ev_specs = [specs[i] for i in val_idx if specs[i].nx == eval_size] or [specs[i] for i in val_idx]
     if max_eval:
         ev_specs = ev_specs[:max_eval]
@@ -176,11 +185,14 @@ def main():
     ap.add_argument("--max-eval", type=int, default=0)
     ap.add_argument("--train-seconds", type=float, default=0.0)
     ap.add_argument("--t-start", type=int, default=500)
+    ap.add_argument("--init-from", default=None, help="weights (.npz) to start from")
+    ap.add_argument("--save-to", default=None, help="write the trained weights (.npz) here")
     ap.add_argument("--seed", type=int, default=2026)
     a = ap.parse_args()
     res = run(a.replicas, a.steps, a.every, eval_size=a.eval_size, iters=a.iters, batch=a.batch, lr=a.lr,
               jumps=a.jumps, corr=a.corr, seed=a.seed, max_eval=a.max_eval or None,
-              train_seconds=a.train_seconds or None, t_start=a.t_start,
+              train_seconds=a.train_seconds or None, t_start=a.t_start, init_from=a.init_from,
+              save_to=a.save_to,
               log=lambda m: print(m, flush=True))
     print(json.dumps(res))
 
