@@ -429,7 +429,12 @@ class Trainer:
         for it in range(iters):
             g = groups[rng.choice(len(groups), p=weights)]
             items = [g[i] for i in rng.choice(len(g), min(self.batch, len(g)), replace=False)]
-            lr_it = lr if lr_final is None else lr * (lr_final / lr) ** (it / max(iters - 1, 1))
+            # exponential decay over the iterations or, with a time budget,
+            # over the budget (so a run stopped by time ends at lr_final)
+            frac = it / max(iters - 1, 1)
+            if max_seconds is not None:
+                frac = max(frac, (time.perf_counter() - t_start) / max_seconds)
+            lr_it = lr if lr_final is None else lr * (lr_final / lr) ** min(frac, 1.0)
             lr_it *= min(1.0, (it + 1) / max(warmup, 1))
             losses.append(self.step(data, items, lr_it, lag))
             if log and (it % 100 == 0 or it == iters - 1):
