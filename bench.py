@@ -18,7 +18,8 @@ Prints one JSON line:
                compiled from the reference sources) on the host cores, or the
                CPU oracle when that binary is absent
   extra        256^3 FP64 and FP32-shifted throughput, per-kernel times; the
-               256-replica uneven-size ensemble (FP32) throughput
+               256-replica uneven-size ensemble (FP32) throughput; the
+               surrogate U-Net's BF16 MFMA inference rate (TFLOP/s)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
 """
@@ -120,6 +121,48 @@ def run_ensemble(nrep, steps, warmup, precision=32, seed=2026):
             "hbm_frac_step": mlups * 1e6 * BYTES_SITE[precision]["step"] / (HBM_PEAK_GBS * 1e9),
             "tuned": cfg, "tuner_trials": rows.tolist(), "init_s": t_init,
             "mass_first_last": mass}
+
+
+BF16_PEAK_TFLOPS = 2500.0      # MI355X dense BF16 MFMA
+
+
+def run_surrogate_conv(n=64, batch=8, reps=5):
+    """Inference of the surrogate U-Net on batch x n^3: MFMA FLOPs of its 12
+    convolutions / wall time, and the largest layer (d0, 80 -> 32 channels at
+    full resolution) timed alone."""
+    import ctypes
+    import numpy as np
+    import ludwig_amd as L
+    from ludwig_amd.surrogate import LAYERS, LEVEL, UNet
+    net = UNet(seed=1)
+    shape = (n, n, n)
+    npos = batch * n ** 3
+    x = L.DeviceArray.from_host(np.zeros(npos * 16, np.uint16), np.uint16)
+    net.forward(x, batch, shape, keep=False)
+    L.lib().ldw_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.forward(x, batch, shape, keep=False)
+    L.lib().ldw_device_synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    flops = sum(2 * 27 * ci * co * npos / 8 ** LEVEL[name] for name, ci, co in LAYERS)
+    b = net._bufs[(batch,) + shape]
+    lib = L.lib()
+    args = (ctypes.c_void_p(b["cat0"].ptr), ctypes.c_void_p(net.wf["d0"].ptr), net.bptr("d0"), None,
+            ctypes.c_void_p(b["a_d0"].ptr), batch, n, n, n, 80, 32)
+    lib.ldw_conv3d_fwd_silu(*args)
+    lib.ldw_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps * 4):
+        lib.ldw_conv3d_fwd_silu(*args)
+    lib.ldw_device_synchronize()
+    dl = (time.perf_counter() - t0) / (reps * 4)
+    fl = 2 * 27 * 80 * 32 * npos
+    x.free()
+    return {"lattice": n, "batch": batch, "forward_ms": 1e3 * dt, "forward_tflops": flops / dt / 1e12,
+            "forward_frac_bf16_peak": flops / dt / 1e12 / BF16_PEAK_TFLOPS,
+            "d0_layer_ms": 1e3 * dl, "d0_layer_tflops": fl / dl / 1e12,
+            "d0_layer_frac_bf16_peak": fl / dl / 1e12 / BF16_PEAK_TFLOPS}
 
 
 def cpu_baseline(n=128, steps=10):
@@ -226,6 +269,9 @@ def main():
         r = run_ensemble(256, max(20, args.steps // 4), 5)
         extra["ensemble_256_replicas_fp32"] = r
         print(f"# ensemble: {r}", file=sys.stderr, flush=True)
+        r = run_surrogate_conv()
+        extra["surrogate_unet_inference_bf16_mfma"] = r
+        print(f"# surrogate conv: {r}", file=sys.stderr, flush=True)
 
     nsite = args.size ** 3
     roof = None
