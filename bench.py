@@ -230,7 +230,23 @@ def main():
     ap.add_argument("--precision", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 20 launches)")
+    ap.add_argument("--profile-ensemble", type=int, default=0,
+                    help="only run this many steps of the 256-replica ensemble (for rocprofv3)")
+    ap.add_argument("--profile-surrogate", type=int, default=0,
+                    help="only run this many U-Net inferences on 8 x 64^3 (for rocprofv3)")
     args = ap.parse_args()
+    if args.profile_ensemble:
+        from ludwig_amd.ensemble import Ensemble, draw_replicas
+        e = Ensemble(draw_replicas(256, seed=2026), precision=32, fatal=True)
+        e.tune(reps=2)
+        e.step(args.profile_ensemble)
+        e.close()
+        print(f"profile run: ensemble of 256 replicas, {args.profile_ensemble} steps")
+        return
+    if args.profile_surrogate:
+        r = run_surrogate_conv(reps=args.profile_surrogate)
+        print(f"profile run: surrogate inference {r}")
+        return
     if args.copy_calibration:
         import ctypes
         import ludwig_amd as L
