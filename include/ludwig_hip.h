@@ -491,6 +491,10 @@ int ldw_nn_pool2_bwd(const float * dout, float * din, int batch, int nx, int ny,
 int ldw_nn_up2_fwd(const void * in, void * out, int batch, int nx, int ny, int nz, int c);
 int ldw_nn_up2_bwd(const float * dout, float * din, int batch, int nx, int ny, int nz, int c);
 int ldw_nn_concat(const void * a, int ca, const void * b, int cb, void * out, long long npos);
+/* out (fine lattice fx x fy x fz) = concat(nearest 2x upsampling of coarse
+ * (cu channels), skip (cs channels)); cu, cs multiples of 8 */
+int ldw_nn_up2_concat(const void * coarse, int cu, const void * skip, int cs, void * out,
+		      int batch, int fx, int fy, int fz);
 int ldw_nn_split(const float * d, int ca, int cb, float * da, float * db, long long npos,
 		 int acc_a, int acc_b);
 int ldw_nn_to_bf16(const float * a, void * b, long long n);

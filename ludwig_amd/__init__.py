@@ -148,6 +148,7 @@ def lib():
         "ldw_nn_up2_fwd": [vp, vp] + [ctypes.c_int] * 5,
         "ldw_nn_up2_bwd": [vp, vp] + [ctypes.c_int] * 5,
         "ldw_nn_concat": [vp, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_longlong],
+        "ldw_nn_up2_concat": [vp, ctypes.c_int, vp, ctypes.c_int, vp] + [ctypes.c_int] * 4,
         "ldw_nn_split": [vp, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_longlong, ctypes.c_int,
                          ctypes.c_int],
         "ldw_nn_to_bf16": [vp, vp, ctypes.c_longlong],

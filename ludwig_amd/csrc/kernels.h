@@ -197,6 +197,9 @@ hipError_t nn_up2_bwd(const float * dout, float * din, int batch, int nx, int ny
 		      hipStream_t s);
 hipError_t nn_concat(const uint16_t * a, int ca, const uint16_t * b, int cb, uint16_t * out,
 		     int64_t npos, hipStream_t s);
+// out (fine fx x fy x fz) = concat(up2(coarse), skip); cu, cs multiples of 8
+hipError_t nn_up2_concat(const uint16_t * coarse, int cu, const uint16_t * skip, int cs, uint16_t * out,
+			int batch, int fx, int fy, int fz, hipStream_t s);
 hipError_t nn_split(const float * d, int ca, int cb, float * da, float * db, int64_t npos,
 		    int acc_a, int acc_b, hipStream_t s);
 hipError_t nn_to_bf16(const float * a, uint16_t * b, int64_t n, hipStream_t s);

@@ -150,6 +150,34 @@ __global__ void up2_bwd_kernel(const float * __restrict__ dout, float * __restri
   }
 }
 
+// fine output [P_f][cu + cs] = (coarse parent's cu channels, skip's cs
+// channels): nearest upsampling and concatenation in one pass, 8 channels
+// (16 bytes) per thread; cu and cs are multiples of 8
+__global__ void up2_concat_kernel(const uint16_t * __restrict__ coarse, int cu,
+				  const uint16_t * __restrict__ skip, int cs, uint16_t * __restrict__ out,
+				  int batch, int fx, int fy, int fz) {
+  const int c8 = (cu + cs)/8;
+  const int64_t n = (int64_t) batch*fx*fy*fz*c8;
+  GRID_STRIDE(i, n) {
+    const int k = (int) (i % c8);
+    const int64_t p = i / c8;
+    int64_t q = p;
+    const int z = (int) (q % fz); q /= fz;
+    const int y = (int) (q % fy); q /= fy;
+    const int x = (int) (q % fx);
+    const int b = (int) (q / fx);
+    uint4 v;
+    if (8*k < cu) {
+      const int64_t cp = (((int64_t) b*(fx/2) + x/2)*(fy/2) + y/2)*(fz/2) + z/2;
+      v = *reinterpret_cast<const uint4 *>(coarse + cp*cu + 8*k);
+    }
+    else {
+      v = *reinterpret_cast<const uint4 *>(skip + p*cs + 8*k - cu);
+    }
+    *reinterpret_cast<uint4 *>(out + p*(cu + cs) + 8*k) = v;
+  }
+}
+
 __global__ void concat_kernel(const uint16_t * __restrict__ a, int ca, const uint16_t * __restrict__ b,
 			      int cb, uint16_t * __restrict__ out, int64_t npos) {
   const int c = ca + cb;
@@ -335,6 +363,13 @@ hipError_t nn_up2_bwd(const float * dout, float * din, int batch, int nx, int ny
 		      hipStream_t s) {
   const int64_t n = (int64_t) batch*nx*ny*nz*c;
   LAUNCH(up2_bwd_kernel, n, dout, din, batch, nx, ny, nz, c);
+  return hipGetLastError();
+}
+
+hipError_t nn_up2_concat(const uint16_t * coarse, int cu, const uint16_t * skip, int cs, uint16_t * out,
+			int batch, int fx, int fy, int fz, hipStream_t s) {
+  const int64_t n = (int64_t) batch*fx*fy*fz*((cu + cs)/8);
+  LAUNCH(up2_concat_kernel, n, coarse, cu, skip, cs, out, batch, fx, fy, fz);
   return hipGetLastError();
 }
 

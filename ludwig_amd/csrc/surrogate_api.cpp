@@ -157,6 +157,16 @@ extern "C" int ldw_nn_concat(const void * a, int ca, const void * b, int cb, voi
   return 0;
 }
 
+extern "C" int ldw_nn_up2_concat(const void * coarse, int cu, const void * skip, int cs, void * out,
+				 int batch, int fx, int fy, int fz) {
+  READY();
+  if (cu % 8 || cs % 8) LDW_FAIL("ldw_nn_up2_concat: channel counts must be multiples of 8 (%d, %d)", cu, cs);
+  if (fx % 2 || fy % 2 || fz % 2) LDW_FAIL("ldw_nn_up2_concat: odd fine lattice %d x %d x %d", fx, fy, fz);
+  KCHECK(ldw::nn_up2_concat((const uint16_t *) coarse, cu, (const uint16_t *) skip, cs, (uint16_t *) out,
+			    batch, fx, fy, fz, ldw::api::stream()));
+  return 0;
+}
+
 extern "C" int ldw_nn_split(const float * d, int ca, int cb, float * da, float * db, long long npos,
 			    int acc_a, int acc_b) {
   READY();

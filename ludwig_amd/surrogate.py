@@ -168,7 +168,6 @@ class UNet:
             b[f"pool{l}"] = DeviceArray(P[l] * c, np.uint16)
             b[f"dpool{l}"] = DeviceArray(P[l] * c, np.float32)
         for l, c_up, c_skip in ((2, C[3], C[2]), (1, C[2], C[1]), (0, C[1], C[0])):
-            b[f"up{l}"] = DeviceArray(P[l] * c_up, np.uint16)
             b[f"dup{l}"] = DeviceArray(P[l] * c_up, np.float32)
             b[f"cat{l}"] = DeviceArray(P[l] * (c_up + c_skip), np.uint16)
             b[f"dcat{l}"] = DeviceArray(P[l] * (c_up + c_skip), np.float32)
@@ -211,9 +210,8 @@ class UNet:
             self._conv(b, nb, b["a_" + na])
         for l, low, skip, name in ((2, "b1", "e2b", "d2"), (1, "d2", "e1b", "d1"), (0, "d1", "e0b", "d0")):
             cu, cs = self.layout[low][2], self.layout[skip][2]
-            _ck(L.ldw_nn_up2_fwd(_v(b["a_" + low]), _v(b[f"up{l}"]), batch, *sh[l + 1], cu), "up")
-            _ck(L.ldw_nn_concat(_v(b[f"up{l}"]), cu, _v(b["a_" + skip]), cs, _v(b[f"cat{l}"]), b["P"][l]),
-                "concat")
+            _ck(L.ldw_nn_up2_concat(_v(b["a_" + low]), cu, _v(b["a_" + skip]), cs, _v(b[f"cat{l}"]), batch,
+                                    *sh[l]), "up + concat")
             self._conv(b, name, b[f"cat{l}"])
         self._conv(b, "out", b["a_d0"], act=False)
         self._last = b

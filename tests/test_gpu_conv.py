@@ -125,6 +125,12 @@ def test_pool_up_concat_split_silu_exact():
     dcv = dc.download().reshape(b, nx // 2, ny // 2, nz // 2, c)
     assert np.array_equal(df.download().reshape(b, nx, ny, nz, c),
                           (np.float32(0.125) * dcv).repeat(2, 1).repeat(2, 2).repeat(2, 3))
+    # nearest upsampling and concatenation in one pass
+    skip = bf16_bits(rng.standard_normal(n * 32))
+    dsk, uc = dev(skip, np.uint16), L.DeviceArray(n * (c + 32), np.uint16)
+    assert lib.ldw_nn_up2_concat(vp(po), c, vp(dsk), 32, vp(uc), b, nx, ny, nz) == 0
+    want_uc = np.concatenate([bf16_bits(u.ravel()).reshape(n, c), skip.reshape(n, 32)], axis=1)
+    assert np.array_equal(uc.download().reshape(n, c + 32), want_uc)
     # concat / split
     c2 = 32
     a2 = bf16_bits(rng.standard_normal(n * c2))
