@@ -26,9 +26,8 @@ namespace ldw {
 namespace {
 
 using namespace tile;
-using S = Shape64x4;
 
-template <typename T, int ORDER, int METHOD, bool GZ>
+template <typename T, class S, int ORDER, int METHOD, bool GZ>
 __global__ void __launch_bounds__(S::NT)
 binary_step_kernel(Geom g, FeParams fe, LbParams prm, int xchunk, int nbz, int nby, int nbx, int remap,
 		   const T * __restrict__ phi, T * __restrict__ phi_out,
@@ -46,10 +45,10 @@ binary_step_kernel(Geom g, FeParams fe, LbParams prm, int xchunk, int nbz, int n
 				     u_prev, u_next, rho, g_in, g_out, fz0, fz1, fz2);
 }
 
-template <typename T>
-void launch(const Geom & g, const FeParams & fe, const LbParams & prm, const T * phi,
-	    T * phi_out, const T * u_prev, T * u_next, T * rho, const T * g_in,
-	    T * g_out, const double fzero[3], hipStream_t st) {
+template <typename T, class S>
+void launch_shape(const Geom & g, const FeParams & fe, const LbParams & prm, const T * phi,
+		  T * phi_out, const T * u_prev, T * u_next, T * rho, const T * g_in,
+		  T * g_out, const double fzero[3], hipStream_t st) {
   const int bz = (g.nz + S::TZ - 1)/S::TZ;
   const int by = (g.ny + S::TY - 1)/S::TY;
   int xchunk = g.nx;
@@ -60,7 +59,7 @@ void launch(const Geom & g, const FeParams & fe, const LbParams & prm, const T *
   dim3 grid(((ntile + NXCD - 1)/NXCD)*NXCD), block(S::TZ, S::TY, 1);
   const T f0 = T(fzero[0]), f1 = T(fzero[1]), f2 = T(fzero[2]);
   const bool gz = (prm.relax == 0);
-#define L_(O, M, G) hipLaunchKernelGGL((binary_step_kernel<T, O, M, G>), grid, block, 0, st, \
+#define L_(O, M, G) hipLaunchKernelGGL((binary_step_kernel<T, S, O, M, G>), grid, block, 0, st, \
 				       g, fe, prm, xchunk, bz, by, bx, tune_remap(), phi, phi_out, u_prev, u_next, rho, \
 				       g_in, g_out, f0, f1, f2)
 #define L_G(O, M) do { if (gz) L_(O, M, true); else L_(O, M, false); } while (0)
@@ -71,6 +70,22 @@ void launch(const Geom & g, const FeParams & fe, const LbParams & prm, const T *
 #undef L_M
 #undef L_G
 #undef L_
+}
+
+// The block tile: 64 x 4 (z, y) columns (256 threads), or the 512-thread
+// 64 x 8 / 128 x 4 tiles, which halve the share of pulled rows that cross a
+// tile edge in y or z; chosen by precision and lattice size
+// (tune_step_tile).  The tile changes only which block computes a site,
+// not the arithmetic.
+template <typename T>
+void launch(const Geom & g, const FeParams & fe, const LbParams & prm, const T * phi,
+	    T * phi_out, const T * u_prev, T * u_next, T * rho, const T * g_in,
+	    T * g_out, const double fzero[3], hipStream_t st) {
+  switch (tune_step_tile(sizeof(T) == 8 ? 64 : 32, g.nsite)) {
+  case 1: launch_shape<T, Shape<64, 8>>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero, st); break;
+  case 2: launch_shape<T, Shape<128, 4>>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero, st); break;
+  default: launch_shape<T, Shape64x4>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero, st); break;
+  }
 }
 
 }  // namespace

@@ -39,6 +39,9 @@ struct FeParams {
 // LDW_MIN_BLOCKS (x-chunks are halved until the grid has this many blocks).
 int tune_remap(void);
 int tune_min_blocks(void);
+// tile of the single-pass step kernel: 0 = 64 x 4, 1 = 64 x 8, 2 = 128 x 4
+// (z x y); LDW_STEP_TILE forces one, else chosen by precision and size
+int tune_step_tile(int precision, int64_t nsite);
 
 // K1: fused D3Q19 MRT collision + pull propagation.
 //   g_in/g_out: 19 planes of nsite (post-collision distributions; when

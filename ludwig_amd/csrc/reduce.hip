@@ -7,6 +7,7 @@
 // few ulp regardless of the order.
 
 #include <hip/hip_runtime.h>
+#include <string.h>
 
 #include <cstdlib>
 
@@ -118,6 +119,23 @@ int tune_min_blocks(void) {
   static int v = -1;
   if (v < 0) v = env_int("LDW_MIN_BLOCKS", 2048);
   return v;
+}
+
+int tune_step_tile(int precision, int64_t nsite) {
+  static int v = -2;
+  if (v == -2) {
+    const char * e = getenv("LDW_STEP_TILE");
+    v = -1;
+    if (e && strcmp(e, "64x4") == 0) v = 0;
+    else if (e && strcmp(e, "64x8") == 0) v = 1;
+    else if (e && strcmp(e, "128x4") == 0) v = 2;
+  }
+  if (v >= 0) return v;
+  // measured on MI355X (C2 physics): FP64 is fastest with 64 x 8 tiles at
+  // 128^3 and 256^3; FP32 with 128 x 4 tiles up to 128^3 (-17 %) and with
+  // 64 x 4 tiles at 256^3
+  if (precision == 64) return 1;
+  return (nsite <= ((int64_t) 1 << 22)) ? 2 : 0;
 }
 
 int sum_blocks(int64_t n) {
