@@ -47,10 +47,11 @@ KERNEL_DESC = {
     "collide": "collide_kernel (MRT collision + pull propagation)",
     "force_ch": "force_ch_kernel (force + Cahn-Hilliard)",
 }
-# HBM bytes per site of binary_step_kernel from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE, each scaled by the same counter's reading for the plain copy
-# kernel (fetch x 1.9884, write x 0.9702); profiles/binary_step_hbm_traffic_pmc.md
-TRAFFIC_B_SITE = {("binary_step", 64, 128): 439.5, ("binary_step", 64, 256): 431.8}
+# HBM bytes per site of binary_step_kernel (FP64, default 64 x 8 tiles) from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's
+# reading for the plain copy kernel (fetch x 1.9884, write x 0.9706);
+# profiles/binary_step_hbm_traffic_pmc.md
+TRAFFIC_B_SITE = {("binary_step", 64, 128): 429.9, ("binary_step", 64, 256): 423.1}
 C2 = dict(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25, tau=1.0, order=1)
 SEED = 8361235
 
