@@ -34,7 +34,8 @@ class LbParam(ctypes.Structure):
 class FeParam(ctypes.Structure):
     _fields_ = [("a", ctypes.c_double), ("b", ctypes.c_double),
                 ("kappa", ctypes.c_double), ("mobility", ctypes.c_double),
-                ("order", ctypes.c_int), ("force_method", ctypes.c_int)]
+                ("order", ctypes.c_int), ("force_method", ctypes.c_int),
+                ("stencil", ctypes.c_int)]
 
 
 def lib():
@@ -77,8 +78,8 @@ def lb_param(tau=None, eta=None, eta_bulk=None, relax=M10, rho0=1.0,
 
 
 def fe_param(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.45, order=1,
-             force_method=0):
-    return FeParam(a, b, kappa, mobility, order, force_method)
+             force_method=0, stencil=7):
+    return FeParam(a, b, kappa, mobility, order, force_method, stencil)
 
 
 def tables():
@@ -121,6 +122,15 @@ def equilibrium(shape, rho=None, u=None, rho_uniform=1.0):
     lib().ora_equilibrium(nx, ny, nz, _p(rho), _p(u),
                           ctypes.c_double(rho_uniform), _p(f))
     return f
+
+
+def grad27(shape, phi):
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    g = np.zeros(3 * n)
+    d = np.zeros(n)
+    lib().ora_grad27(nx, ny, nz, _p(np.ascontiguousarray(phi)), _p(g), _p(d))
+    return g, d
 
 
 def grad7(shape, phi):

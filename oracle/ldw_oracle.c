@@ -351,6 +351,68 @@ void ora_grad7(int nx, int ny, int nz, const double * phi, double * grad,
   }
 }
 
+/* grad_3d_27pt_kernel (gradient_3d_27pt_fluid.c:219-368): each
+ * component sums the 9 differences across the cube in the kernel's order
+ * (y slowest, then z, for d_x; x slowest, then z, for d_y; x, then y,
+ * for d_z), times 0.5/9; the Laplacian sums the 26 neighbours in x, y, z
+ * order minus 26 phi, times 1/9. */
+
+void ora_grad27(int nx, int ny, int nz, const double * phi, double * grad,
+		double * delsq) {
+  const long nsite = (long) nx*ny*nz;
+  const double r9 = (1.0/9.0);
+  #pragma omp parallel for schedule(static)
+  for (int x = 0; x < nx; x++) {
+    for (int y = 0; y < ny; y++) {
+      for (int z = 0; z < nz; z++) {
+	double p[3][3][3];
+	for (int i = 0; i < 3; i++)
+	  for (int j = 0; j < 3; j++)
+	    for (int k = 0; k < 3; k++)
+	      p[i][j][k] = phi[site3(wrap(x + i - 1, nx), wrap(y + j - 1, ny), wrap(z + k - 1, nz), ny, nz)];
+	long is = site3(x, y, z, ny, nz);
+	double sx = 0.0, sy = 0.0, sz = 0.0;
+	int first = 1;
+	for (int j = 0; j < 3; j++)
+	  for (int k = 0; k < 3; k++) {
+	    if (first) sx = + p[2][j][k] - p[0][j][k];
+	    else sx = sx + p[2][j][k] - p[0][j][k];
+	    first = 0;
+	  }
+	first = 1;
+	for (int i = 0; i < 3; i++)
+	  for (int k = 0; k < 3; k++) {
+	    if (first) sy = + p[i][2][k] - p[i][0][k];
+	    else sy = sy + p[i][2][k] - p[i][0][k];
+	    first = 0;
+	  }
+	first = 1;
+	for (int i = 0; i < 3; i++)
+	  for (int j = 0; j < 3; j++) {
+	    if (first) sz = + p[i][j][2] - p[i][j][0];
+	    else sz = sz + p[i][j][2] - p[i][j][0];
+	    first = 0;
+	  }
+	double sd = 0.0;
+	first = 1;
+	for (int i = 0; i < 3; i++)
+	  for (int j = 0; j < 3; j++)
+	    for (int k = 0; k < 3; k++) {
+	      if (i == 1 && j == 1 && k == 1) continue;
+	      if (first) sd = + p[i][j][k];
+	      else sd = sd + p[i][j][k];
+	      first = 0;
+	    }
+	sd = sd - 26.0*p[1][1][1];
+	grad[0*nsite + is] = 0.5*r9*sx;
+	grad[1*nsite + is] = 0.5*r9*sy;
+	grad[2*nsite + is] = 0.5*r9*sz;
+	delsq[is] = r9*sd;
+      }
+    }
+  }
+}
+
 /* fe_symm_mu (symmetric.c:308-318) */
 
 static inline double mu_site(const ora_fe_param_t * fe, double phi,
@@ -578,7 +640,8 @@ void ora_binary_steps(int nx, int ny, int nz, int nsteps,
   double * ftmp = (double *) malloc(sizeof(double)*NVEL*nsite);
 
   for (int step = 0; step < nsteps; step++) {
-    ora_grad7(nx, ny, nz, phi, grad, delsq);
+    if (fe->stencil == 27) ora_grad27(nx, ny, nz, phi, grad, delsq);
+    else ora_grad7(nx, ny, nz, phi, grad, delsq);
     ora_force(nx, ny, nz, fe, phi, grad, delsq, F);
     ora_cahn_hilliard(nx, ny, nz, fe, phi, delsq, u);
     ora_collide(nx, ny, nz, lb, f, F, rho, u);

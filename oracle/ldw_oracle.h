@@ -44,6 +44,7 @@ typedef struct ora_fe_param_s {
   double mobility;         /* Cahn-Hilliard mobility M */
   int    order;            /* advection order 1, 2 or 3 */
   int    force_method;     /* 0 stress divergence, 1 phi grad mu */
+  int    stencil;          /* gradient: 7 (3d_7pt_fluid, also 0) or 27 (3d_27pt_fluid) */
 } ora_fe_param_t;
 
 /* D3Q19 tables: velocities, weights, ma (modes x velocities),
@@ -67,6 +68,9 @@ void ora_equilibrium(int nx, int ny, int nz, const double * rho,
 		     const double * u, double rho_uniform, double * f);
 
 /* 7-point gradient and Laplacian (gradient_3d_7pt_fluid.c:231-310) */
+/* grad_3d_27pt_kernel (gradient_3d_27pt_fluid.c:219-368) */
+void ora_grad27(int nx, int ny, int nz, const double * phi, double * grad,
+		double * delsq);
 void ora_grad7(int nx, int ny, int nz, const double * phi, double * grad,
 	       double * delsq);
 

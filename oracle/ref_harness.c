@@ -30,7 +30,8 @@
  *    lb_collide          collision.c:143
  *    lb_halo             lb_data.c:754
  *    lb_propagation      propagation.c:43
- *    field_grad_compute  field_grad.c:319 (grad_3d_7pt_fluid_d2)
+ *    field_grad_compute  field_grad.c:319 (grad_3d_7pt_fluid_d2, or
+ *                        grad_3d_27pt_fluid_d2 with LDW_HARNESS_STENCIL=27)
  *    phi_force_calculation phi_force.c:74
  *    phi_cahn_hilliard   phi_cahn_hilliard.c:213
  *    field_phi_init_spinodal field_phi_init.c:484
@@ -56,6 +57,7 @@
 #include "field.h"
 #include "field_grad.h"
 #include "gradient_3d_7pt_fluid.h"
+#include "gradient_3d_27pt_fluid.h"
 #include "symmetric.h"
 #include "phi_force.h"
 #include "phi_force_stress.h"
@@ -161,7 +163,12 @@ static void sys_create(sys_t * s, int n, int nhalo, lb_relaxation_enum_t relax,
     field_options_t opts = field_options_ndata_nhalo(1, nhalo);
     field_create(s->pe, s->cs, s->le, "phi", &opts, &s->phi);
     field_grad_create(s->pe, s->phi, 2, &s->dphi);
-    field_grad_set(s->dphi, grad_3d_7pt_fluid_d2, NULL);
+    {
+      /* LDW_HARNESS_STENCIL=27 selects the 27-point gradient */
+      const char * st = getenv("LDW_HARNESS_STENCIL");
+      if (st && atoi(st) == 27) field_grad_set(s->dphi, grad_3d_27pt_fluid_d2, NULL);
+      else field_grad_set(s->dphi, grad_3d_7pt_fluid_d2, NULL);
+    }
     fe_symm_create(s->pe, s->cs, s->phi, s->dphi, &s->fe);
     pth_create(s->pe, s->cs, force_method, &s->pth);
     {

@@ -68,6 +68,17 @@ CASES = {
                           [], ["phi"]),
     # C1: plain D3Q19 MRT, 32^3, tau 0.8, 100 steps, random-velocity start
     "single32_s100": (["single", 32, 100, 12345, 0.8], 32, [], ["u_init", "f_init", "f", "rho", "u"]),
+    # the 27-point gradient (fd_gradient_calculation 3d_27pt_fluid)
+    "force27_stress": (["force", 8, 11, 0, A, B, K], 8, ["phi", "grad", "delsq", "mu", "force"], [],
+                       {"LDW_HARNESS_STENCIL": "27"}),
+    "force27_gradmu": (["force", 8, 12, 1, A, B, K], 8, ["phi", "grad", "delsq", "mu", "force"], [],
+                       {"LDW_HARNESS_STENCIL": "27"}),
+    # serial-spin-fd1.inp physics (A = -0.00625, B = 0.00625, K = 0.004,
+    # M = 1.25, viscosity 0.00625 -> tau = 0.51875) with the 27-point gradient
+    "binary16_g27_s10": (["binary", 16, 10, 8361235, 1, -0.00625, 0.00625, 0.004, 1.25, 0.51875, 0.0, 0.1],
+                         16, ["phi", "rho", "u"], ["f"], {"LDW_HARNESS_STENCIL": "27"}),
+    "binary16_g27_o3_s10": (["binary", 16, 10, 99, 3, A, B, K, MS, 1.0, 0.0, 0.1],
+                            16, ["phi", "rho", "u"], ["f"], {"LDW_HARNESS_STENCIL": "27"}),
 }
 
 NSAMPLE = 256
@@ -78,16 +89,25 @@ def sha(a):
 
 
 def main():
+    """python make_golden.py [case ...]: all cases, or only the named ones
+    (the sample sites of every case stay the same either way)."""
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
+    only = set(sys.argv[1:])
     rng = np.random.default_rng(20261202)
-    for name, (args, n, full, hashed) in CASES.items():
+    for name, spec in CASES.items():
+        args, n, full, hashed = spec[:4]
+        env = dict(os.environ, OMP_NUM_THREADS="8", **(spec[4] if len(spec) > 4 else {}))
+        nsite = n ** 3
+        samples = np.sort(rng.choice(nsite, size=min(NSAMPLE, nsite), replace=False))
+        if only and name not in only:
+            continue
         with tempfile.TemporaryDirectory() as tmp:
             cmd = [HARNESS, tmp] + [str(a) for a in args]
-            subprocess.run(cmd, check=True, env=dict(os.environ, OMP_NUM_THREADS="8"))
+            subprocess.run(cmd, check=True, env=env)
             out = {"args": np.array([str(a) for a in args]), "n": np.array(n)}
-            nsite = n ** 3
-            samples = np.sort(rng.choice(nsite, size=min(NSAMPLE, nsite), replace=False))
+            if len(spec) > 4:
+                out["stencil"] = np.array(int(spec[4].get("LDW_HARNESS_STENCIL", 7)))
             out["sample_sites"] = samples
             for key in full:
                 out[key] = np.fromfile(os.path.join(tmp, key + ".f64"))

@@ -79,13 +79,31 @@ def _binary(g):
     a, b, k, m, tau, phi0, amp = map(float, args[5:12])
     s = (n, n, n)
     lb = O.lb_param(tau=tau)
-    fe = O.fe_param(a=a, b=b, kappa=k, mobility=m, order=order)
+    stencil = int(g["stencil"]) if "stencil" in g else 7
+    fe = O.fe_param(a=a, b=b, kappa=k, mobility=m, order=order, stencil=stencil)
     phi = O.spinodal(s, seed, phi0, amp)
     f = O.equilibrium(s, rho_uniform=1.0)
     return s, nsteps, lb, fe, phi, f
 
 
-@pytest.mark.parametrize("case", ["binary16_o1_s20", "binary16_o3_s10", "binary16_o2_s5"])
+@pytest.mark.parametrize("case,method", [("force27_stress", 0), ("force27_gradmu", 1)])
+def test_gradient27_mu_force(golden, case, method):
+    """The 27-point gradient (gradient_3d_27pt_fluid.c:219-368) and what
+    follows from it, bit-exact against the reference built with it."""
+    g = golden(case)
+    assert int(g["stencil"]) == 27
+    n = int(g["n"])
+    s = (n, n, n)
+    fe = O.fe_param(force_method=method, stencil=27)
+    grad, delsq = O.grad27(s, g["phi"])
+    assert_exact(grad, g["grad"], "grad")
+    assert_exact(delsq, g["delsq"], "delsq")
+    assert_exact(O.mu(fe, g["phi"], delsq), g["mu"], "mu")
+    assert_exact(O.force(s, fe, g["phi"], grad, delsq), g["force"], "force")
+
+
+@pytest.mark.parametrize("case", ["binary16_o1_s20", "binary16_o3_s10", "binary16_o2_s5",
+                                  "binary16_g27_s10", "binary16_g27_o3_s10"])
 def test_binary_steps_full_fields(golden, case):
     g = golden(case)
     s, nsteps, lb, fe, phi, f = _binary(g)

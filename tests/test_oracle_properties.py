@@ -243,3 +243,24 @@ def test_compensated_sums():
     exact = math.fsum(b)
     assert abs(O.kahan_sum(b) - exact) <= 4 * EPS * np.abs(b).sum() * 1e-6 + abs(exact) * 4 * EPS
     assert abs(O.klein_sum(b) - exact) <= abs(exact) * 2 * EPS
+
+
+def test_gradient27_closed_form_on_a_plane_wave():
+    """27-point stencil on phi = cos(k.r): the 26 neighbours sum to
+    prod_a (1 + 2 cos k_a) - 1 times phi, so
+    del2 phi = (prod_a (1 + 2 cos k_a) - 27)/9 phi and
+    d_x phi = -(1/9) sin k_x (1 + 2 cos k_y)(1 + 2 cos k_z) sin(k.r)."""
+    nx, ny, nz = 10, 12, 14
+    m = (3, -2, 5)
+    k = [2 * np.pi * m[0] / nx, 2 * np.pi * m[1] / ny, 2 * np.pi * m[2] / nz]
+    X, Y, Z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    arg = k[0] * X + k[1] * Y + k[2] * Z
+    phi = np.cos(arg).ravel()
+    grad, delsq = O.grad27((nx, ny, nz), phi)
+    c = [1 + 2 * np.cos(ka) for ka in k]
+    assert np.allclose(delsq, (c[0] * c[1] * c[2] - 27) / 9 * phi, atol=1e-13)
+    s = np.sin(arg).ravel()
+    n = phi.size
+    assert np.allclose(grad[:n], -np.sin(k[0]) * c[1] * c[2] / 9 * s, atol=1e-13)
+    assert np.allclose(grad[n:2 * n], -np.sin(k[1]) * c[0] * c[2] / 9 * s, atol=1e-13)
+    assert np.allclose(grad[2 * n:], -np.sin(k[2]) * c[0] * c[1] / 9 * s, atol=1e-13)
