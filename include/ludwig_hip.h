@@ -236,6 +236,7 @@ int field_grad_scalar_delsq(field_grad_t * obj, int index, double * delsq);
 
 /* The 7-point fluid stencil (gradient_3d_7pt_fluid.c:73) */
 int grad_3d_7pt_fluid_d2(field_grad_t * fgrad);
+int grad_3d_27pt_fluid_d2(field_grad_t * fgrad);   /* gradient_3d_27pt_fluid.h */
 
 /* ------------------------------------------------------------------------ */
 /* Hydrodynamic fields (hydro.h)                                            */

@@ -110,6 +110,7 @@ def lib():
         "field_free": [vp], "field_memcpy": [vp, ctypes.c_int], "field_halo": [vp],
         "field_grad_create": [vp, vp, ctypes.c_int, pvp], "field_grad_free": [vp],
         "field_grad_compute": [vp], "field_grad_memcpy": [vp, ctypes.c_int],
+        "field_grad_set": [vp, vp, vp],
         "hydro_create": [vp, vp, vp, ctypes.POINTER(HydroOptions), pvp], "hydro_free": [vp],
         "hydro_f_zero": [vp, ctypes.POINTER(ctypes.c_double)],
         "hydro_u_zero": [vp, ctypes.POINTER(ctypes.c_double)],
@@ -362,7 +363,7 @@ class BinaryFluid(_Base):
     solver and the 7-point gradient (nhalo 2)."""
 
     def __init__(self, shape, a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25,
-                 order=1, force_method=FORCE_STRESS_DIVERGENCE, **kw):
+                 order=1, force_method=FORCE_STRESS_DIVERGENCE, stencil=7, **kw):
         super().__init__(shape, 2, **kw)
         L = lib()
         vp = ctypes.c_void_p
@@ -375,6 +376,10 @@ class BinaryFluid(_Base):
             "field_create")
         self.dphi = vp()
         _ck(L.field_grad_create(self.pe, self.phi, 2, ctypes.byref(self.dphi)), "field_grad_create")
+        if stencil not in (7, 27):
+            raise ValueError(f"stencil {stencil}: 7 (3d_7pt_fluid) or 27 (3d_27pt_fluid)")
+        d2 = L.grad_3d_27pt_fluid_d2 if stencil == 27 else L.grad_3d_7pt_fluid_d2
+        _ck(L.field_grad_set(self.dphi, ctypes.cast(d2, ctypes.c_void_p), None), "field_grad_set")
         self.fe = vp()
         _ck(L.fe_symm_create(self.pe, self.cs, self.phi, self.dphi, ctypes.byref(self.fe)),
             "fe_symm_create")

@@ -232,6 +232,7 @@ extern "C" int ldw_ensemble_create(int nrep, const ldw_replica_t * spec, int pre
     R.fe.a = s.a; R.fe.b = s.b; R.fe.kappa = s.kappa; R.fe.mobility = s.mobility;
     R.fe.order = adv_order;
     R.fe.force_method = e->method;
+    R.fe.stencil = 7;
     R.prm = ldw::api::lb_params(s.rho0, s.eta_shear, s.eta_bulk, LB_RELAXATION_M10, nullptr);
     off += R.g.nsite;
   }

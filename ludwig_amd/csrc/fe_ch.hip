@@ -24,6 +24,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "grad27.h"
 #include "kernels.h"
 
 namespace ldw {
@@ -53,10 +54,18 @@ struct GradPt {
   T phi, g[3], delsq;
 };
 
-// gradient and Laplacian at (x,y,z): gradient_3d_7pt_fluid.c:270-305
+// gradient and Laplacian at (x,y,z): the 7-point stencil
+// (gradient_3d_7pt_fluid.c:270-305), or the 27-point one (grad27.h) when
+// stencil is 27
 template <typename T>
-__device__ __forceinline__ GradPt<T> grad_at(const Lat<T> & L, int x, int y, int z) {
+__device__ __forceinline__ GradPt<T> grad_at(const Lat<T> & L, int x, int y, int z, int stencil) {
   GradPt<T> r;
+  if (stencil == 27) {
+    auto p = [&](int i, int j, int k) { return L.at(x + i - 1, y + j - 1, z + k - 1); };
+    r.phi = L.at(x, y, z);
+    grad27<T>(p, r.g, r.delsq);
+    return r;
+  }
   const T xp = L.at(x + 1, y, z), xm = L.at(x - 1, y, z);
   const T yp = L.at(x, y + 1, z), ym = L.at(x, y - 1, z);
   const T zp = L.at(x, y, z + 1), zm = L.at(x, y, z - 1);
@@ -104,16 +113,16 @@ phi_force_mu_kernel(Geom geo, FeParams fe, const T * __restrict__ phi,
   const int z = rem - y*geo.nz;
 
   const Lat<T> L = {phi, geo.nx, geo.ny, geo.nz};
-  const GradPt<T> c0 = grad_at(L, x, y, z);
+  const GradPt<T> c0 = grad_at(L, x, y, z, fe.stencil);
 
   T F[3];
 
   if (force != nullptr) {
     if (fe.force_method == 0) {
       const T kappa = T(fe.kappa);
-      const GradPt<T> cxp = grad_at(L, x + 1, y, z), cxm = grad_at(L, x - 1, y, z);
-      const GradPt<T> cyp = grad_at(L, x, y + 1, z), cym = grad_at(L, x, y - 1, z);
-      const GradPt<T> czp = grad_at(L, x, y, z + 1), czm = grad_at(L, x, y, z - 1);
+      const GradPt<T> cxp = grad_at(L, x + 1, y, z, fe.stencil), cxm = grad_at(L, x - 1, y, z, fe.stencil);
+      const GradPt<T> cyp = grad_at(L, x, y + 1, z, fe.stencil), cym = grad_at(L, x, y - 1, z, fe.stencil);
+      const GradPt<T> czp = grad_at(L, x, y, z + 1, fe.stencil), czm = grad_at(L, x, y, z - 1, fe.stencil);
       const T p00 = p0_of(fe, c0);
       const T pxp = p0_of(fe, cxp), pxm = p0_of(fe, cxm);
       const T pyp = p0_of(fe, cyp), pym = p0_of(fe, cym);
@@ -137,11 +146,11 @@ phi_force_mu_kernel(Geom geo, FeParams fe, const T * __restrict__ phi,
       // phi grad mu: F_a = 0 + (-phi 1/2)(mu(x+e_a) - mu(x-e_a))
       const T phi0 = c0.phi;
       GradPt<T> m, p;
-      m = grad_at(L, x - 1, y, z); p = grad_at(L, x + 1, y, z);
+      m = grad_at(L, x - 1, y, z, fe.stencil); p = grad_at(L, x + 1, y, z, fe.stencil);
       F[0] = T(0.0) + -phi0*T(0.5)*(mu_of(fe, p.phi, p.delsq) - mu_of(fe, m.phi, m.delsq));
-      m = grad_at(L, x, y - 1, z); p = grad_at(L, x, y + 1, z);
+      m = grad_at(L, x, y - 1, z, fe.stencil); p = grad_at(L, x, y + 1, z, fe.stencil);
       F[1] = T(0.0) + -phi0*T(0.5)*(mu_of(fe, p.phi, p.delsq) - mu_of(fe, m.phi, m.delsq));
-      m = grad_at(L, x, y, z - 1); p = grad_at(L, x, y, z + 1);
+      m = grad_at(L, x, y, z - 1, fe.stencil); p = grad_at(L, x, y, z + 1, fe.stencil);
       F[2] = T(0.0) + -phi0*T(0.5)*(mu_of(fe, p.phi, p.delsq) - mu_of(fe, m.phi, m.delsq));
     }
 
@@ -173,13 +182,13 @@ mu_kernel(Geom geo, FeParams fe, const T * __restrict__ phi, T * __restrict__ mu
   const int y = rem / geo.nz;
   const int z = rem - y*geo.nz;
   const Lat<T> L = {phi, geo.nx, geo.ny, geo.nz};
-  const GradPt<T> c0 = grad_at(L, x, y, z);
+  const GradPt<T> c0 = grad_at(L, x, y, z, fe.stencil);
   mu[site] = mu_of(fe, c0.phi, c0.delsq);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-grad7_kernel(Geom geo, const T * __restrict__ phi, T * __restrict__ grad,
+grad_kernel(Geom geo, int stencil, const T * __restrict__ phi, T * __restrict__ grad,
 	     T * __restrict__ delsq) {
   const int64_t site = (int64_t) blockIdx.x*blockDim.x + threadIdx.x;
   if (site >= geo.nsite) return;
@@ -189,7 +198,7 @@ grad7_kernel(Geom geo, const T * __restrict__ phi, T * __restrict__ grad,
   const int y = rem / geo.nz;
   const int z = rem - y*geo.nz;
   const Lat<T> L = {phi, geo.nx, geo.ny, geo.nz};
-  const GradPt<T> c0 = grad_at(L, x, y, z);
+  const GradPt<T> c0 = grad_at(L, x, y, z, stencil);
   grad[site] = c0.g[0];
   grad[geo.nsite + site] = c0.g[1];
   grad[2*geo.nsite + site] = c0.g[2];
@@ -351,10 +360,10 @@ void launch_mu_f32(const Geom & g, const FeParams & fe, const float * phi,
 		     g, fe, phi, mu);
 }
 
-void launch_grad7_f64(const Geom & g, const double * phi, double * grad,
-		      double * delsq, hipStream_t s) {
-  hipLaunchKernelGGL(grad7_kernel<double>, dim3(nblocks(g.nsite)), dim3(256), 0, s,
-		     g, phi, grad, delsq);
+void launch_grad_f64(const Geom & g, int stencil, const double * phi, double * grad,
+		     double * delsq, hipStream_t s) {
+  hipLaunchKernelGGL(grad_kernel<double>, dim3(nblocks(g.nsite)), dim3(256), 0, s,
+		     g, stencil, phi, grad, delsq);
 }
 
 void launch_cahn_hilliard_f64(const Geom & g, const FeParams & fe,

@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "grad27.h"
 #include "kernels.h"
 
 namespace ldw {
@@ -148,13 +149,23 @@ __device__ __forceinline__ void compute_q_plane(Smem<T, S> & s, const FeParams &
     const int yy = i / QZ, zz = i - yy*QZ;
     const int c = (yy + 1)*PZ + (zz + 1);         // same point in the phi tile
     const T phi = s.phi[s0][c];
-    const T xp = s.phi[sp][c], xm = s.phi[sm][c];
-    const T yp = s.phi[s0][c + PZ], ym = s.phi[s0][c - PZ];
-    const T zp = s.phi[s0][c + 1], zm = s.phi[s0][c - 1];
-    const T g0 = T(0.5)*(xp - xm);
-    const T g1 = T(0.5)*(yp - ym);
-    const T g2 = T(0.5)*(zp - zm);
-    const T delsq = xp + xm + yp + ym + zp + zm - T(6.0)*phi;
+    T g0, g1, g2, delsq;
+    if (fe.stencil == 27) {
+      const int sl[3] = {sm, s0, sp};
+      auto p = [&](int i, int j, int k) { return s.phi[sl[i]][c + (j - 1)*PZ + (k - 1)]; };
+      T gg[3];
+      grad27<T>(p, gg, delsq);
+      g0 = gg[0]; g1 = gg[1]; g2 = gg[2];
+    }
+    else {
+      const T xp = s.phi[sp][c], xm = s.phi[sm][c];
+      const T yp = s.phi[s0][c + PZ], ym = s.phi[s0][c - PZ];
+      const T zp = s.phi[s0][c + 1], zm = s.phi[s0][c - 1];
+      g0 = T(0.5)*(xp - xm);
+      g1 = T(0.5)*(yp - ym);
+      g2 = T(0.5)*(zp - zm);
+      delsq = xp + xm + yp + ym + zp + zm - T(6.0)*phi;
+    }
     const T p0 = T(0.5)*a*phi*phi + T(0.75)*b*phi*phi*phi*phi - kappa*phi*delsq
       - T(0.5)*kappa*(g0*g0 + g1*g1 + g2*g2);
     const T mu = a*phi + b*phi*phi*phi - kappa*delsq;

@@ -32,6 +32,7 @@ struct FeParams {
   double mobility;
   int order;                // advection order 1, 2, 3
   int force_method;         // 0 stress divergence, 1 phi grad mu
+  int stencil;              // gradient: 7 (3d_7pt_fluid) or 27 (3d_27pt_fluid)
 };
 
 // Launch tuning of the tiled kernels, read once from the environment:
@@ -81,9 +82,10 @@ void launch_mu_f64(const Geom & g, const FeParams & fe, const double * phi,
 void launch_mu_f32(const Geom & g, const FeParams & fe, const float * phi,
 		   float * mu, hipStream_t s);
 
-// Gradient and Laplacian materialised (for field_grad reads)
-void launch_grad7_f64(const Geom & g, const double * phi, double * grad,
-		      double * delsq, hipStream_t s);
+// Gradient and Laplacian materialised (for field_grad reads), 7- or
+// 27-point stencil
+void launch_grad_f64(const Geom & g, int stencil, const double * phi, double * grad,
+		     double * delsq, hipStream_t s);
 
 // K3: Cahn-Hilliard advective + diffusive fluxes and the phi update fused.
 //   u may be nullptr (no hydrodynamics: zero advective flux).

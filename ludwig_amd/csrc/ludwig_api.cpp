@@ -785,7 +785,11 @@ extern "C" int field_memcpy(field_t * f, tdpMemcpyKind flag) {
 
 extern "C" int field_halo(field_t *) { return 0; }
 
+/* The stencils are evaluated inside the kernels; these entry points only
+ * name the choice for field_grad_set() (gradient_3d_7pt_fluid.h,
+ * gradient_3d_27pt_fluid.h) */
 extern "C" int grad_3d_7pt_fluid_d2(field_grad_t *) { return 0; }
+extern "C" int grad_3d_27pt_fluid_d2(field_grad_t *) { return 0; }
 
 extern "C" int field_grad_create(pe_t *, field_t * f, int level, field_grad_t ** pobj) {
   if (!f || !pobj) FAIL("field_grad_create: bad arguments");
@@ -814,9 +818,14 @@ extern "C" void field_grad_free(field_grad_t * g) {
 }
 
 extern "C" int field_grad_set(field_grad_t * g, grad_ft d2, grad_ft) {
-  if (d2 != grad_3d_7pt_fluid_d2) FAIL("field_grad_set: only grad_3d_7pt_fluid_d2 is built");
+  if (d2 != grad_3d_7pt_fluid_d2 && d2 != grad_3d_27pt_fluid_d2)
+    FAIL("field_grad_set: the built stencils are grad_3d_7pt_fluid_d2 and grad_3d_27pt_fluid_d2");
   g->d2 = d2;
   return 0;
+}
+
+static int grad_stencil(const field_grad_t * g) {
+  return (g && g->d2 == grad_3d_27pt_fluid_d2) ? 27 : 7;
 }
 
 /* Deferred: the stencil is evaluated inside the kernels that consume it.
@@ -843,8 +852,8 @@ extern "C" int field_grad_memcpy(field_grad_t * g, tdpMemcpyKind flag) {
   // a single update ago (still in the other buffer)
   int buf = f->cur;
   if (f->version == g->grad_version + 1 && g->grad_buf == 1 - f->cur) buf = g->grad_buf;
-  ldw::launch_grad7_f64(geo, (const double *) f->d_data[buf], (double *) g->d_grad,
-			(double *) g->d_delsq, g_stream);
+  ldw::launch_grad_f64(geo, grad_stencil(g), (const double *) f->d_data[buf], (double *) g->d_grad,
+		       (double *) g->d_delsq, g_stream);
   CHECK_LAUNCH();
   std::vector<double> b1(3*geo.nsite), b2(geo.nsite);
   HIP_CHECK(hipMemcpyAsync(b1.data(), g->d_grad, 8*3*geo.nsite, hipMemcpyDeviceToHost, g_stream));
@@ -1151,6 +1160,7 @@ static FeParams fe_params_of(fe_symm_t * fe, int method) {
   advection_order(&order);
   p.order = order;
   p.force_method = (method == FE_FORCE_METHOD_PHI_GRADMU) ? 1 : 0;
+  p.stencil = grad_stencil(fe->dphi);
   return p;
 }
 

@@ -22,7 +22,7 @@
  *    viscosity viscosity_bulk fluid_rho0
  *    free_energy (none | symmetric)  A B K | symmetric_a symmetric_b
  *    symmetric_kappa  mobility  phi0  phi_initialisation (spinodal)
- *    noise  random_seed  fd_gradient_calculation (3d_7pt_fluid)
+ *    noise  random_seed  fd_gradient_calculation (3d_7pt_fluid, 3d_27pt_fluid)
  *    fd_advection_scheme_order  fe_force_method  lb_relaxation_scheme
  *    ghost_modes  lb_ghost_modes  cahn_hilliard_options_conserve
  *    distribution_initialisation (3d_uniform_u) distribution_uniform_u
@@ -376,10 +376,13 @@ int main(int argc, char ** argv) {
     field_options_t fo = field_options_ndata_nhalo(1, 2);
     field_create(pe, cs, le, "phi", &fo, &phi);
     field_grad_create(pe, phi, 2, &dphi);
-    if (get_string("fd_gradient_calculation", s, sizeof(s)) && strcmp(s, "3d_7pt_fluid") != 0) {
-      fatal("fd_gradient_calculation not supported by this build (3d_7pt_fluid is): ", s);
+    grad_ft d2 = grad_3d_7pt_fluid_d2;
+    if (get_string("fd_gradient_calculation", s, sizeof(s))) {
+      if (strcmp(s, "3d_27pt_fluid") == 0) d2 = grad_3d_27pt_fluid_d2;
+      else if (strcmp(s, "3d_7pt_fluid") != 0)
+	fatal("fd_gradient_calculation not supported by this build (3d_7pt_fluid, 3d_27pt_fluid are): ", s);
     }
-    field_grad_set(dphi, grad_3d_7pt_fluid_d2, NULL);
+    field_grad_set(dphi, d2, NULL);
     fe_symm_create(pe, cs, phi, dphi, &fe);
     fe_symm_param_t fp = {-0.003125, 0.003125, 0.002, 0.0, 0.0};   /* symmetric.c:40-42 */
     get_double("A", &fp.a); get_double("symmetric_a", &fp.a);

@@ -1,8 +1,9 @@
 """The host driver (ludwig_amd/ludwig_hip) runs unchanged Ludwig input files
 and its statistics lines are compared with logs written by the reference's
-own Ludwig.exe (and one committed regression log of the reference,
-tests/regression/d3q19-short/serial-dist-3du.log, kept here as
-serial-dist-3du.ref), token by token with the
+own Ludwig.exe (and two committed regression logs of the reference,
+tests/regression/d3q19-short/serial-dist-3du.log and serial-spin-fd1.log,
+kept here as .ref files with their unchanged .inp inputs; serial-spin-fd1
+uses the 27-point gradient), token by token with the
 absolute tolerance 1e-12 of the reference's tests/awk-fp-diff.sh."""
 
 import os
@@ -55,8 +56,8 @@ def run_driver(name, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["serial-dist-3du", "spin7-o1", "spin7-o3", "c2-m025",
-                                  "gradmu-trt", "single-bgk"])
+@pytest.mark.parametrize("name", ["serial-dist-3du", "serial-spin-fd1", "spin7-o1", "spin7-o3",
+                                  "c2-m025", "gradmu-trt", "single-bgk"])
 def test_driver_matches_reference_log(name, tmp_path):
     got = run_driver(name, tmp_path)
     want = open(os.path.join(LOGS, name + ".ref")).read()

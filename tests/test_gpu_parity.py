@@ -78,6 +78,23 @@ def test_force_golden(golden, case, method):
     b.close()
 
 
+@pytest.mark.parametrize("case,method", [("force27_stress", L.FORCE_STRESS_DIVERGENCE),
+                                         ("force27_gradmu", L.FORCE_PHI_GRADMU)])
+def test_force27_golden(golden, case, method):
+    """The 27-point gradient (gradient_3d_27pt_fluid.c) through the force."""
+    g = golden(case)
+    n = int(g["n"])
+    args = [str(a) for a in g["args"]]
+    a, b_, k = map(float, args[4:7])
+    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, tau=1.0, force_method=method, stencil=27)
+    b.set_phi(g["phi"])
+    b.f_zero()
+    b.force_stage()
+    _, _, F = b.hydro_fields()
+    assert_exact(F, g["force"], "force")
+    b.close()
+
+
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_cahn_hilliard_golden(golden, order):
     g = golden(f"ch_order{order}")
@@ -99,12 +116,15 @@ def _binary_args(g):
     return n, nsteps, seed, order, a, b, k, m, tau, phi0, amp
 
 
-@pytest.mark.parametrize("case", ["binary16_o1_s20", "binary16_o3_s10", "binary16_o2_s5"])
+@pytest.mark.parametrize("case", ["binary16_o1_s20", "binary16_o3_s10", "binary16_o2_s5",
+                                  "binary16_g27_s10", "binary16_g27_o3_s10"])
 def test_binary_step_golden(golden, case):
     g = golden(case)
     n, nsteps, seed, order, a, b_, k, m, tau, phi0, amp = _binary_args(g)
+    stencil = int(g["stencil"]) if "stencil" in g else 7
     print(f"case {case}", flush=True)
-    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, order=order, tau=tau)
+    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, order=order, tau=tau,
+                      stencil=stencil)
     b.init_spinodal(seed, phi0, amp)
     if "phi_init" in g:
         assert_exact(b.phi_field(), g["phi_init"], "phi_init")
