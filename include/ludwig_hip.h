@@ -418,7 +418,7 @@ int ldw_ensemble_create(int nreplica, const ldw_replica_t * spec, int precision,
 			int adv_order, int force_method, ldw_ensemble_t ** pens);
 int ldw_ensemble_free(ldw_ensemble_t * ens);
 
-/* Autotuner: times every tile shape (64x4, 32x8, 16x16 in z x y) and
+/* Autotuner: times every tile shape (64x4, 32x8, 16x16, 64x8, 128x4 in z x y) and
  * several x planes per work item on the current state, `reps` launches
  * each, and keeps the fastest.  The state is not advanced.  Runs by itself
  * before the first step unless a configuration was set. */

@@ -16,10 +16,11 @@
 // finish resets it (and its own finish counter), so the launch can be
 // replayed from a graph without any memset node.
 //
-// Tile shapes (TZ x TY, 256 threads): 64 x 4, 32 x 8, 16 x 16.  The sizes
-// 48, 64, 80, 96 are all multiples of 16, so 16 x 16 tiles leave no idle
-// lanes; wider z rows make longer contiguous HBM segments.  The host-side
-// autotuner picks the shape and the planes per item by timing.
+// Tile shapes (TZ x TY): 64 x 4, 32 x 8, 16 x 16 (256 threads) and 64 x 8,
+// 128 x 4 (512 threads, 1 block per CU).  The sizes 48, 64, 80, 96 are all
+// multiples of 16, so 16 x 16 tiles leave no idle lanes; wider z rows make
+// longer contiguous HBM segments, and larger tiles re-read fewer edge rows.
+// The host-side autotuner picks the shape and the planes per item by timing.
 
 #include <hip/hip_runtime.h>
 
@@ -122,7 +123,9 @@ const void * pick_shape(int shape, int order, int method) {
   switch (shape) {
   case 0: return pick<T, Shape<64, 4>>(order, method);
   case 1: return pick<T, Shape<32, 8>>(order, method);
-  default: return pick<T, Shape<16, 16>>(order, method);
+  case 2: return pick<T, Shape<16, 16>>(order, method);
+  case 3: return pick<T, Shape<64, 8>>(order, method);
+  default: return pick<T, Shape<128, 4>>(order, method);
   }
 }
 
@@ -141,8 +144,14 @@ hipError_t launch(int shape, int order, int method, int nblocks, const EnsRep * 
 
 }  // namespace
 
-int ens_shape_tz(int shape) { return shape == 0 ? 64 : (shape == 1 ? 32 : 16); }
-int ens_shape_ty(int shape) { return shape == 0 ? 4 : (shape == 1 ? 8 : 16); }
+int ens_shape_tz(int shape) {
+  static const int tz[ENS_NSHAPE] = {64, 32, 16, 64, 128};
+  return tz[shape];
+}
+int ens_shape_ty(int shape) {
+  static const int ty[ENS_NSHAPE] = {4, 8, 16, 8, 4};
+  return ty[shape];
+}
 
 int ens_blocks_per_cu(int shape, int precision, int order, int method) {
   const void * k = (precision == 64) ? pick_shape<double>(shape, order, method)

@@ -318,7 +318,7 @@ extern "C" int ldw_ensemble_free(ldw_ensemble_t * e) {
 extern "C" int ldw_ensemble_config_set(ldw_ensemble_t * e, int tz, int ty, int xplanes, int blocks) {
   if (e == nullptr) LDW_FAIL("ldw_ensemble_config_set: null handle");
   const int shape = shape_of(tz, ty);
-  if (shape < 0) LDW_FAIL("ldw_ensemble_config_set: tile %d x %d is not built (64x4, 32x8, 16x16)", tz, ty);
+  if (shape < 0) LDW_FAIL("ldw_ensemble_config_set: tile %d x %d is not built (64x4, 32x8, 16x16, 64x8, 128x4)", tz, ty);
   if (xplanes < 1) LDW_FAIL("ldw_ensemble_config_set: xplanes %d < 1", xplanes);
   const int resident = resident_blocks(e, shape);
   if (resident < 1) LDW_FAIL("ldw_ensemble_config_set: occupancy query failed");

@@ -135,8 +135,9 @@ struct EnsItem {
   int rep, y0, z0, xb, xe, pad[3];
 };
 
-// tile shapes: 0 -> 64 x 4, 1 -> 32 x 8, 2 -> 16 x 16 (TZ x TY)
-constexpr int ENS_NSHAPE = 3;
+// tile shapes: 0 -> 64 x 4, 1 -> 32 x 8, 2 -> 16 x 16, 3 -> 64 x 8,
+// 4 -> 128 x 4 (TZ x TY)
+constexpr int ENS_NSHAPE = 5;
 int ens_shape_tz(int shape);
 int ens_shape_ty(int shape);
 // resident blocks per CU of the ensemble kernel (0 on error)

@@ -15,7 +15,7 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(64, 4), (32, 8), (16, 16)]
+SHAPES = [(64, 4), (32, 8), (16, 16), (64, 8), (128, 4)]
 
 
 def assert_exact(got, want, what):
