@@ -82,6 +82,8 @@ int cs_ntotal(cs_t * cs, int ntotal[3]);                      /* coords.h:90 */
 int cs_nlocal(cs_t * cs, int n[3]);                           /* coords.h:95 */
 int cs_nsites(cs_t * cs, int * nsites);                       /* coords.h:104 */
 int cs_index(cs_t * cs, int ic, int jc, int kc);              /* coords.h:102 */
+int cs_lmin(cs_t * cs, double lmin[3]);                      /* coords.h:96 */
+int cs_ltot(cs_t * cs, double ltot[3]);                      /* coords.h:97 */
 
 /* ------------------------------------------------------------------------ */
 /* Physical constants (physics.h)                                           */
@@ -117,6 +119,47 @@ typedef enum {
 } tdpMemcpyKind;
 
 /* ------------------------------------------------------------------------ */
+/* I/O options (io_options.h, io_info_args.h, io_event.h)                   */
+/* ------------------------------------------------------------------------ */
+
+typedef enum io_mode_enum {IO_MODE_INVALID, IO_MODE_MPIIO} io_mode_enum_t;   /* io_options.h:30 */
+typedef enum io_record_format_enum {IO_RECORD_INVALID,
+				    IO_RECORD_ASCII,
+				    IO_RECORD_BINARY} io_record_format_enum_t; /* io_options.h:34 */
+typedef enum io_metadata_version_enum {IO_METADATA_INVALID,
+				       IO_METADATA_SINGLE_V1,
+				       IO_METADATA_MULTI_V1,
+				       IO_METADATA_V2} io_metadata_version_enum_t; /* io_options.h:40 */
+
+typedef struct io_options_s {                                 /* io_options.h:51 */
+  io_mode_enum_t mode;
+  io_record_format_enum_t iorformat;
+  io_metadata_version_enum_t metadata_version;
+  int report;
+  int asynchronous;
+  int compression_levl;
+  int iogrid[3];
+} io_options_t;
+
+typedef struct io_info_args_s {                               /* io_info_args.h:30 */
+  io_options_t input;
+  io_options_t output;
+  int grid[3];
+  int iofreq;
+} io_info_args_t;
+
+/* mode mpiio, binary records, metadata version 2, I/O grid 1 1 1
+ * (io_options.c:26-38, io_info_args.c:28-35) */
+io_options_t io_options_default(void);
+io_info_args_t io_info_args_default(void);
+
+typedef struct io_event_s {                                   /* io_event.h:35 */
+  const char * extra_name;
+  void * extra_json;                  /* not used: no extra metadata here */
+  double time[6];
+} io_event_t;
+
+/* ------------------------------------------------------------------------ */
 /* Lattice Boltzmann distributions (lb_data.h)                              */
 /* ------------------------------------------------------------------------ */
 
@@ -126,7 +169,7 @@ typedef enum lb_relaxation_enum {LB_RELAXATION_M10,
 typedef enum lb_halo_enum {LB_HALO_FULL = 1, LB_HALO_REDUCED = 2} lb_halo_enum_t;
 typedef enum lb_dist_enum_type {LB_RHO = 0, LB_PHI = 1} lb_dist_enum_t;
 
-/* lb_data_options.h:32-42 without the I/O member */
+/* lb_data_options.h:32-42 */
 typedef struct lb_data_options_s {
   int ndim;
   int nvel;
@@ -135,6 +178,7 @@ typedef struct lb_data_options_s {
   lb_halo_enum_t halo;
   int reportimbalance;
   int usefirsttouch;
+  io_info_args_t iodata;
 } lb_data_options_t;
 
 typedef struct lb_data_s lb_t;
@@ -170,6 +214,18 @@ int lb_collide(lb_t * lb, hydro_t * hydro, map_t * map, noise_t * noise,
 	       fe_t * fe, visc_t * visc);                     /* collision.c:143 */
 
 int lb_propagation(lb_t * lb);                                /* propagation.c:43 */
+
+/* Distribution files "dist-ttttttttt.001-001" (io_subfile.c:186-204) in
+ * the working directory, with "dist-metadata.001-001" (io_metadata.c:
+ * 246-281) written before the first data file.  Records are the 19
+ * distributions of each site as doubles in host byte order, sites in
+ * x-slowest, z-fastest order (binary), or " %22.15e" per value and one line
+ * per velocity (ascii), as lb_write_buf / lb_write_buf_ascii
+ * (lb_data.c:1533-1640).  The record format comes from the options'
+ * iodata at lb_data_create(); the I/O grid must be 1 1 1 (one GPU).  A
+ * missing or short file is an error. */
+int lb_io_write(lb_t * lb, int timestep, io_event_t * event); /* lb_data.c:1716 */
+int lb_io_read(lb_t * lb, int timestep, io_event_t * event);  /* lb_data.c:1779 */
 
 /* ------------------------------------------------------------------------ */
 /* Lees-Edwards, map, wall: the periodic, all-fluid, wall-free case only    */
@@ -209,6 +265,7 @@ typedef struct field_options_s {                              /* field_options.h
   int haloscheme;
   int haloverbose;
   int usefirsttouch;
+  io_info_args_t iodata;
 } field_options_t;
 
 typedef struct field_s field_t;
@@ -224,6 +281,13 @@ int field_halo(field_t * obj);                                /* field.h:104 */
 int field_nf(field_t * obj, int * nf);
 int field_scalar(field_t * obj, int index, double * phi);     /* field.h:109 */
 int field_scalar_set(field_t * obj, int index, double phi);   /* field.h:110 */
+
+/* "<name>-ttttttttt.001-001" with "<name>-metadata.001-001": nf doubles
+ * per site (binary) or nf " %22.15e" values and a newline (ascii), as
+ * field_write_buf / field_write_buf_ascii (field.c:896-996); otherwise as
+ * lb_io_write. */
+int field_io_write(field_t * field, int timestep, io_event_t * event); /* field.c:1633 */
+int field_io_read(field_t * field, int timestep, io_event_t * event);  /* field.c:1690 */
 
 int field_grad_create(pe_t * pe, field_t * f, int level,
 		      field_grad_t ** pobj);                  /* field_grad.h:44 */
@@ -242,7 +306,13 @@ int grad_3d_27pt_fluid_d2(field_grad_t * fgrad);   /* gradient_3d_27pt_fluid.h *
 /* Hydrodynamic fields (hydro.h)                                            */
 /* ------------------------------------------------------------------------ */
 
-typedef struct hydro_options_s {int nhcomm;} hydro_options_t;
+typedef struct hydro_options_s {                              /* hydro_options.h:22 */
+  int nhcomm;
+  field_options_t rho;
+  field_options_t u;
+  field_options_t force;
+  field_options_t eta;
+} hydro_options_t;
 
 hydro_options_t hydro_options_default(void);                  /* hydro_options.h:32 */
 int hydro_create(pe_t * pe, cs_t * cs, lees_edw_t * le,
@@ -258,6 +328,12 @@ int hydro_u_set(hydro_t * obj, int index, const double u[3]);
 int hydro_rho(hydro_t * obj, int index, double * rho);
 int hydro_f_local(hydro_t * obj, int index, double force[3]);
 int hydro_f_local_set(hydro_t * obj, int index, const double force[3]);
+
+/* Configuration output and input of the fields "rho" (1 double per site)
+ * and "vel" (3), as field_io_write/read with the options' rho.iodata and
+ * u.iodata (hydro.c:941-967). */
+int hydro_io_write(hydro_t * obj, int timestep, io_event_t * event); /* hydro.c:941 */
+int hydro_io_read(hydro_t * obj, int timestep, io_event_t * event);  /* hydro.c:959 */
 
 /* ------------------------------------------------------------------------ */
 /* Symmetric free energy (symmetric.h)                                      */
@@ -277,6 +353,7 @@ int fe_symm_create(pe_t * pe, cs_t * cs, field_t * f, field_grad_t * grd,
 int fe_symm_free(fe_symm_t * fe);                             /* symmetric.h:50 */
 int fe_symm_param_set(fe_symm_t * fe, fe_symm_param_t values); /* symmetric.h:51 */
 int fe_symm_param(fe_symm_t * fe, fe_symm_param_t * values);
+int fe_symm_interfacial_width(fe_symm_t * fe, double * xi);   /* symmetric.c:267 */
 /* host evaluation on the host copies of phi and its gradients */
 int fe_symm_mu(fe_symm_t * fe, int index, double * mu);       /* symmetric.c:308 */
 int fe_symm_fed(fe_symm_t * fe, int index, double * fed);     /* symmetric.c:285 */
@@ -332,6 +409,12 @@ int advection_order(int * order);                             /* advection.h:41 
  * reference (field_phi_init.c:484-518, noise.c:278-341, noise.h:96-109);
  * writes the host copy and the device copy */
 int field_phi_init_spinodal(field_t * phi, int seed, double phi0, double amp);
+
+/* phi = phistar tanh((r - radius)/xi), r the distance of the site from the
+ * box centre (0.5 L, or Lmin + 0.5 L when is_centred), as
+ * field_phi_init.c:38-82; writes the host copy and the device copy */
+int field_phi_init_drop(field_t * phi, double xi, double radius, double phistar,
+			int is_centred);
 
 /* ------------------------------------------------------------------------ */
 /* Additions (no counterpart in the reference)                              */

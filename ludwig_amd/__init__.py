@@ -32,19 +32,45 @@ FORCE_NO_FORCE, FORCE_STRESS_DIVERGENCE, FORCE_PHI_GRADMU = 1, 2, 3
 _lib = None
 
 
+IO_RECORD_ASCII, IO_RECORD_BINARY = 1, 2
+
+
+class IoOptions(ctypes.Structure):
+    """io_options_t (io_options.h:51)."""
+    _fields_ = [("mode", ctypes.c_int), ("iorformat", ctypes.c_int),
+                ("metadata_version", ctypes.c_int), ("report", ctypes.c_int),
+                ("asynchronous", ctypes.c_int), ("compression_levl", ctypes.c_int),
+                ("iogrid", ctypes.c_int * 3)]
+
+
+class IoInfoArgs(ctypes.Structure):
+    """io_info_args_t (io_info_args.h:30)."""
+    _fields_ = [("input", IoOptions), ("output", IoOptions), ("grid", ctypes.c_int * 3),
+                ("iofreq", ctypes.c_int)]
+
+
 class LbDataOptions(ctypes.Structure):
     _fields_ = [("ndim", ctypes.c_int), ("nvel", ctypes.c_int), ("ndist", ctypes.c_int),
                 ("nrelax", ctypes.c_int), ("halo", ctypes.c_int),
-                ("reportimbalance", ctypes.c_int), ("usefirsttouch", ctypes.c_int)]
+                ("reportimbalance", ctypes.c_int), ("usefirsttouch", ctypes.c_int),
+                ("iodata", IoInfoArgs)]
 
 
 class FieldOptions(ctypes.Structure):
     _fields_ = [("ndata", ctypes.c_int), ("nhcomm", ctypes.c_int), ("haloscheme", ctypes.c_int),
-                ("haloverbose", ctypes.c_int), ("usefirsttouch", ctypes.c_int)]
+                ("haloverbose", ctypes.c_int), ("usefirsttouch", ctypes.c_int),
+                ("iodata", IoInfoArgs)]
 
 
 class HydroOptions(ctypes.Structure):
-    _fields_ = [("nhcomm", ctypes.c_int)]
+    _fields_ = [("nhcomm", ctypes.c_int), ("rho", FieldOptions), ("u", FieldOptions),
+                ("force", FieldOptions), ("eta", FieldOptions)]
+
+
+class IoEvent(ctypes.Structure):
+    """io_event_t (io_event.h:35)."""
+    _fields_ = [("extra_name", ctypes.c_char_p), ("extra_json", ctypes.c_void_p),
+                ("time", ctypes.c_double * 6)]
 
 
 class MapOptions(ctypes.Structure):
@@ -167,6 +193,20 @@ def lib():
         f.argtypes = args
         f.restype = ctypes.c_int
     L.cs_ntotal_set.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    for name in ("lb_io_write", "lb_io_read", "field_io_write", "field_io_read",
+                 "hydro_io_write", "hydro_io_read"):
+        f = getattr(L, name)
+        f.argtypes = [vp, ctypes.c_int, ctypes.POINTER(IoEvent)]
+        f.restype = ctypes.c_int
+    L.field_phi_init_drop.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int]
+    L.field_phi_init_drop.restype = ctypes.c_int
+    L.lb_data_options_default.argtypes = []
+    L.lb_data_options_default.restype = LbDataOptions
+    L.field_options_ndata_nhalo.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.field_options_ndata_nhalo.restype = FieldOptions
+    L.hydro_options_default.argtypes = []
+    L.hydro_options_default.restype = HydroOptions
     L.ldw_last_error.restype = ctypes.c_char_p
     L.ldw_last_error.argtypes = []
     L.ldw_precision.restype = ctypes.c_int
@@ -236,7 +276,10 @@ class _Base:
     nx*ny*nz periodic lattice."""
 
     def __init__(self, shape, nhalo, tau=None, eta=None, eta_bulk=None, rho0=1.0,
-                 relax=M10, fbody=(0.0, 0.0, 0.0), precision=64, fatal=False):
+                 relax=M10, fbody=(0.0, 0.0, 0.0), precision=64, fatal=False,
+                 io_format=IO_RECORD_BINARY):
+        """io_format: record format of configuration files, IO_RECORD_BINARY
+        or IO_RECORD_ASCII, or an (input, output) pair."""
         L = lib()
         L.ldw_error_mode_set(1 if fatal else 0)
         set_precision(precision)
@@ -264,13 +307,20 @@ class _Base:
         self.le = vp()
         _ck(L.lees_edw_create(self.pe, self.cs, ctypes.byref(LeOptions()), ctypes.byref(self.le)),
             "lees_edw_create")
-        opts = LbDataOptions(3, 19, 1, relax, 1, 0, 0)
+        opts = L.lb_data_options_default()
+        opts.nrelax = relax
+        io = io_format if isinstance(io_format, tuple) else (io_format, io_format)
+        self.io_format = io
+        opts.iodata.input.iorformat, opts.iodata.output.iorformat = io
         self.lb = vp()
         _ck(L.lb_data_create(self.pe, self.cs, ctypes.byref(opts), ctypes.byref(self.lb)),
             "lb_data_create")
         _ck(L.lb_collision_relaxation_set(self.lb, relax), "lb_collision_relaxation_set")
         self.hydro = vp()
-        _ck(L.hydro_create(self.pe, self.cs, self.le, ctypes.byref(HydroOptions(1)),
+        ho = L.hydro_options_default()
+        for fo in (ho.rho, ho.u):
+            fo.iodata.input.iorformat, fo.iodata.output.iorformat = io
+        _ck(L.hydro_create(self.pe, self.cs, self.le, ctypes.byref(ho),
                            ctypes.byref(self.hydro)), "hydro_create")
         self.map = vp()
         _ck(L.map_create(self.pe, self.cs, ctypes.byref(MapOptions(0, 0)), ctypes.byref(self.map)),
@@ -314,6 +364,18 @@ class _Base:
     def propagate(self):
         _ck(lib().lb_halo(self.lb), "lb_halo")
         _ck(lib().lb_propagation(self.lb), "lb_propagation")
+
+    # configuration files of time step t in the working directory
+    # (lb_io_write / hydro_io_write; lb_io_read / hydro_io_read)
+    def write_config(self, t):
+        ev = IoEvent()
+        _ck(lib().lb_io_write(self.lb, t, ctypes.byref(ev)), "lb_io_write")
+        _ck(lib().hydro_io_write(self.hydro, t, ctypes.byref(ev)), "hydro_io_write")
+
+    def read_config(self, t):
+        ev = IoEvent()
+        _ck(lib().lb_io_read(self.lb, t, ctypes.byref(ev)), "lb_io_read")
+        _ck(lib().hydro_io_read(self.hydro, t, ctypes.byref(ev)), "hydro_io_read")
 
     def f_zero(self, fzero=(0.0, 0.0, 0.0)):
         _ck(lib().hydro_f_zero(self.hydro, (ctypes.c_double * 3)(*fzero)), "hydro_f_zero")
@@ -371,7 +433,8 @@ class BinaryFluid(_Base):
         _ck(L.advection_order_set(order), "advection_order_set")
         self.order = order
         self.phi = vp()
-        fo = FieldOptions(1, 2, 0, 0, 0)
+        fo = L.field_options_ndata_nhalo(1, 2)
+        fo.iodata.input.iorformat, fo.iodata.output.iorformat = self.io_format
         _ck(L.field_create(self.pe, self.cs, self.le, b"phi", ctypes.byref(fo), ctypes.byref(self.phi)),
             "field_create")
         self.dphi = vp()
@@ -395,6 +458,24 @@ class BinaryFluid(_Base):
     def init_spinodal(self, seed, phi0=0.0, amp=0.1, rho0=1.0):
         _ck(lib().field_phi_init_spinodal(self.phi, seed, phi0, amp), "field_phi_init_spinodal")
         _ck(lib().lb_init_rest_f(self.lb, rho0), "lb_init_rest_f")
+
+    def init_drop(self, radius, xi, phistar=1.0, centred=False, rho0=1.0):
+        _ck(lib().field_phi_init_drop(self.phi, xi, radius, phistar, 1 if centred else 0),
+            "field_phi_init_drop")
+        _ck(lib().lb_init_rest_f(self.lb, rho0), "lb_init_rest_f")
+
+    def write_config(self, t):
+        """dist, phi, rho and vel files of time step t, as a configuration
+        step of the reference writes them (ludwig.c:868-962)."""
+        super().write_config(t)
+        _ck(lib().field_io_write(self.phi, t, ctypes.byref(IoEvent())), "field_io_write")
+
+    def read_config(self, t):
+        """The restart read of ludwig.c:313-364: dist, phi, rho and vel."""
+        ev = IoEvent()
+        _ck(lib().lb_io_read(self.lb, t, ctypes.byref(ev)), "lb_io_read")
+        _ck(lib().field_io_read(self.phi, t, ctypes.byref(ev)), "field_io_read")
+        _ck(lib().hydro_io_read(self.hydro, t, ctypes.byref(ev)), "hydro_io_read")
 
     def step(self, nsteps=1):
         _ck(lib().advection_order_set(self.order), "advection_order_set")

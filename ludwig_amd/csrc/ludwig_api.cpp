@@ -34,6 +34,7 @@
 #include "../../include/ludwig_hip.h"
 #include "api_internal.h"
 #include "d3q19.h"
+#include "io_format.h"
 #include "kernels.h"
 
 using ldw::Geom;
@@ -319,6 +320,17 @@ extern "C" int cs_ntotal(cs_t * cs, int n[3]) { for (int a = 0; a < 3; a++) n[a]
 extern "C" int cs_nlocal(cs_t * cs, int n[3]) { for (int a = 0; a < 3; a++) n[a] = cs->ntotal[a]; return 0; }
 extern "C" int cs_nsites(cs_t * cs, int * nsites) { *nsites = cs->nsites; return 0; }
 
+/* coords.c:419-445: the box is [Lmin, Lmin + L) with Lmin = 0.5 */
+extern "C" int cs_ltot(cs_t * cs, double ltot[3]) {
+  for (int a = 0; a < 3; a++) ltot[a] = (double) cs->ntotal[a];
+  return 0;
+}
+
+extern "C" int cs_lmin(cs_t *, double lmin[3]) {
+  for (int a = 0; a < 3; a++) lmin[a] = 0.5;
+  return 0;
+}
+
 /* coords.c:617-631 */
 extern "C" int cs_index(cs_t * cs, int ic, int jc, int kc) {
   return cs->str[0]*(cs->nhalo + ic - 1) + cs->str[1]*(cs->nhalo + jc - 1)
@@ -434,13 +446,39 @@ struct lb_data_s {
   Geom geo;
   double * d_work;         // reduction scratch
   double * d_out;
+  io_info_args_t iodata;   // distribution file options
+  int meta_written;        // dist-metadata written
 };
+
+extern "C" io_options_t io_options_default(void) {
+  io_options_t o = {IO_MODE_MPIIO, IO_RECORD_BINARY, IO_METADATA_V2, 0, 0, 0, {1, 1, 1}};
+  return o;
+}
+
+extern "C" io_info_args_t io_info_args_default(void) {
+  io_info_args_t a;
+  a.input = io_options_default();
+  a.output = io_options_default();
+  a.grid[0] = a.grid[1] = a.grid[2] = 1;
+  a.iofreq = 0;
+  return a;
+}
+
+// Both directions of an object's I/O options are ones this library handles
+static int io_args_check(const char * who, const io_info_args_t & a) {
+  std::string err;
+  if (ldw::io::options_supported(a.input, err) || ldw::io::options_supported(a.output, err)) {
+    FAIL("%s: %s", who, err.c_str());
+  }
+  return 0;
+}
 
 extern "C" lb_data_options_t lb_data_options_default(void) {
   lb_data_options_t o;
   o.ndim = 3; o.nvel = 19; o.ndist = 1;
   o.nrelax = LB_RELAXATION_M10; o.halo = LB_HALO_FULL;
   o.reportimbalance = 0; o.usefirsttouch = 0;
+  o.iodata = io_info_args_default();
   return o;
 }
 
@@ -449,6 +487,7 @@ extern "C" int lb_data_create(pe_t * pe, cs_t * cs, const lb_data_options_t * op
   if (!cs || !cs->init || !opts || !plb) FAIL("lb_data_create: bad arguments");
   if (opts->ndim != 3 || opts->nvel != 19) FAIL("lb_data_create: only D3Q19 is built (ndim %d nvel %d)", opts->ndim, opts->nvel);
   if (opts->ndist != 1) FAIL("lb_data_create: ndist %d not supported (the two-distribution binary LB is out of scope)", opts->ndist);
+  if (io_args_check("lb_data_create", opts->iodata)) return -1;
   if (device_init()) return -1;
 
   lb_t * lb = (lb_t *) calloc(1, sizeof(lb_t));
@@ -468,6 +507,7 @@ extern "C" int lb_data_create(pe_t * pe, cs_t * cs, const lb_data_options_t * op
   if (dmalloc((void **) &lb->d_out, sizeof(double)*4)) return -1;
   lb->cur = 0;
   lb->streamed = 0;
+  lb->iodata = opts->iodata;
   *plb = lb;
   return 0;
 }
@@ -681,6 +721,8 @@ struct field_s {
   int cur;
   long version;            // bumped whenever the device values change
   Geom geo;
+  io_info_args_t iodata;   // data file options
+  int meta_written;
 };
 
 struct field_grad_s {
@@ -698,7 +740,7 @@ struct field_grad_s {
 };
 
 extern "C" field_options_t field_options_ndata_nhalo(int ndata, int nhalo) {
-  field_options_t o = {ndata, nhalo, 0, 0, 0};
+  field_options_t o = {ndata, nhalo, 0, 0, 0, io_info_args_default()};
   return o;
 }
 
@@ -707,6 +749,7 @@ extern "C" int field_create(pe_t * pe, cs_t * cs, lees_edw_t *, const char * nam
   if (!cs || !cs->init || !opts || !pobj) FAIL("field_create: bad arguments");
   if (opts->ndata != 1) FAIL("field_create: only scalar fields (ndata 1) are on the path, got %d", opts->ndata);
   if (opts->nhcomm > cs->nhalo) FAIL("field_create: halo %d exceeds coordinate halo %d", opts->nhcomm, cs->nhalo);
+  if (io_args_check("field_create", opts->iodata)) return -1;
   field_t * f = (field_t *) calloc(1, sizeof(field_t));
   f->pe = pe; f->cs = cs;
   snprintf(f->name, sizeof(f->name), "%s", name ? name : "");
@@ -722,6 +765,7 @@ extern "C" int field_create(pe_t * pe, cs_t * cs, lees_edw_t *, const char * nam
     HIP_CHECK(hipMemsetAsync(f->d_data[b], 0, es*f->geo.nsite, g_stream));
   }
   f->version = 1;
+  f->iodata = opts->iodata;
   *pobj = f;
   return 0;
 }
@@ -906,6 +950,8 @@ struct hydro_s {
   int fd_buf;
   FeParams fd_params;
   field_grad_t * fd_dphi;
+  io_info_args_t rho_io, u_io;     // "rho" and "vel" file options
+  int rho_meta_written, u_meta_written;
 };
 
 static hydro_t * g_pending = nullptr;   // at most one deferred force
@@ -922,14 +968,29 @@ static void pending_drop_field(field_t * f) {
   }
 }
 
-extern "C" hydro_options_t hydro_options_default(void) { hydro_options_t o = {1}; return o; }
+/* hydro_options.c:32-46 */
+extern "C" hydro_options_t hydro_options_default(void) {
+  hydro_options_t o;
+  o.nhcomm = 1;
+  o.rho = field_options_ndata_nhalo(1, 1);
+  o.u = field_options_ndata_nhalo(3, 1);
+  o.force = field_options_ndata_nhalo(3, 1);
+  o.eta = field_options_ndata_nhalo(1, 1);
+  return o;
+}
 
 extern "C" int hydro_create(pe_t * pe, cs_t * cs, lees_edw_t *, const hydro_options_t * opts,
 			    hydro_t ** pobj) {
   if (!cs || !cs->init || !pobj) FAIL("hydro_create: bad arguments");
+  const hydro_options_t dflt = hydro_options_default();
+  if (!opts) opts = &dflt;
+  if (io_args_check("hydro_create (rho)", opts->rho.iodata)) return -1;
+  if (io_args_check("hydro_create (vel)", opts->u.iodata)) return -1;
   hydro_t * h = (hydro_t *) calloc(1, sizeof(hydro_t));
   h->pe = pe; h->cs = cs;
-  h->nhcomm = opts ? opts->nhcomm : 1;
+  h->nhcomm = opts->nhcomm;
+  h->rho_io = opts->rho.iodata;
+  h->u_io = opts->u.iodata;
   h->nsite = cs->nsites;
   h->geo = geom_of(cs);
   h->prec = precision_now();
@@ -1118,6 +1179,12 @@ extern "C" int fe_symm_param_set(fe_symm_t * fe, fe_symm_param_t values) {
 
 extern "C" int fe_symm_param(fe_symm_t * fe, fe_symm_param_t * values) {
   *values = fe->param;
+  return 0;
+}
+
+/* symmetric.c:267-274 */
+extern "C" int fe_symm_interfacial_width(fe_symm_t * fe, double * xi) {
+  *xi = sqrt(-2.0*fe->param.kappa/fe->param.a);
   return 0;
 }
 
@@ -1707,6 +1774,162 @@ extern "C" int field_phi_init_spinodal(field_t * phi, int seed, double phi0, dou
 	phi->data[cs_index(phi->cs, ic, jc, kc)] = phi0 + amp*(ran - 0.5);
       }
   return field_memcpy(phi, tdpMemcpyHostToDevice);
+}
+
+/* field_phi_init.c:38-82 (one rank: no local offset) */
+extern "C" int field_phi_init_drop(field_t * phi, double xi, double radius, double phistar,
+				   int is_centred) {
+  double ltot[3], lmin[3], centre[3];
+  cs_ltot(phi->cs, ltot);
+  cs_lmin(phi->cs, lmin);
+  const double rxi0 = 1.0/xi;
+  for (int a = 0; a < 3; a++) centre[a] = is_centred*lmin[a] + 0.5*ltot[a];
+  const Geom g = phi->geo;
+  for (int ic = 1; ic <= g.nx; ic++)
+    for (int jc = 1; jc <= g.ny; jc++)
+      for (int kc = 1; kc <= g.nz; kc++) {
+	const double pos[3] = {1.0*ic - centre[0], 1.0*jc - centre[1], 1.0*kc - centre[2]};
+	const double r = sqrt(pos[0]*pos[0] + pos[1]*pos[1] + pos[2]*pos[2]);
+	phi->data[cs_index(phi->cs, ic, jc, kc)] = phistar*tanh(rxi0*(r - radius));
+      }
+  return field_memcpy(phi, tdpMemcpyHostToDevice);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Configuration files (lb_data.c:1716-1830, field.c:1633-1738,             */
+/* hydro.c:941-967); the format is in io_format.h                           */
+/* ------------------------------------------------------------------------ */
+
+// count components of a host array (data[c*stride + cs_index]) as records
+// in file order, rec[site*count + c]
+static void gather_records(cs_t * cs, const double * data, int64_t stride, int count,
+			   double * rec) {
+  int64_t s = 0;
+  for (int ic = 1; ic <= cs->ntotal[0]; ic++)
+    for (int jc = 1; jc <= cs->ntotal[1]; jc++)
+      for (int kc = 1; kc <= cs->ntotal[2]; kc++, s++) {
+	const int index = cs_index(cs, ic, jc, kc);
+	for (int c = 0; c < count; c++) rec[s*count + c] = data[c*stride + index];
+      }
+}
+
+static void scatter_records(cs_t * cs, const double * rec, int64_t stride, int count,
+			    double * data) {
+  int64_t s = 0;
+  for (int ic = 1; ic <= cs->ntotal[0]; ic++)
+    for (int jc = 1; jc <= cs->ntotal[1]; jc++)
+      for (int kc = 1; kc <= cs->ntotal[2]; kc++, s++) {
+	const int index = cs_index(cs, ic, jc, kc);
+	for (int c = 0; c < count; c++) data[c*stride + index] = rec[s*count + c];
+      }
+}
+
+static ldw::io::Layout io_layout(cs_t * cs, int count, int per_line) {
+  ldw::io::Layout l;
+  for (int a = 0; a < 3; a++) l.ntotal[a] = cs->ntotal[a];
+  l.count = count;
+  l.per_line = per_line;
+  return l;
+}
+
+// Metadata before the first data file of the object, then the data
+static int io_write(const char * who, const char * stub, int timestep, const io_options_t & o,
+		    int & meta_written, cs_t * cs, const double * data, int64_t stride,
+		    int count, int per_line) {
+  if (timestep < 0 || timestep >= 1000*1000*1000) FAIL("%s: time step %d out of range", who, timestep);
+  const ldw::io::Layout l = io_layout(cs, count, per_line);
+  std::string err;
+  if (!meta_written) {
+    double lmin[3];
+    cs_lmin(cs, lmin);
+    if (ldw::io::write_text(ldw::io::metadata_name(stub), ldw::io::metadata_json(l, o, lmin), err)) {
+      FAIL("%s: %s", who, err.c_str());
+    }
+    meta_written = 1;
+  }
+  std::vector<double> rec((size_t) cs->ntotal[0]*cs->ntotal[1]*cs->ntotal[2]*count);
+  gather_records(cs, data, stride, count, rec.data());
+  const std::string name = ldw::io::data_name(stub, timestep);
+  if (ldw::io::write_data(name, l, o.iorformat, rec.data(), err)) FAIL("%s: %s", who, err.c_str());
+  if (o.report) printf("MPIIO wrote to %s\n", name.c_str());
+  return 0;
+}
+
+static int io_read(const char * who, const char * stub, int timestep, const io_options_t & o,
+		   cs_t * cs, double * data, int64_t stride, int count, int per_line) {
+  if (timestep < 0 || timestep >= 1000*1000*1000) FAIL("%s: time step %d out of range", who, timestep);
+  const ldw::io::Layout l = io_layout(cs, count, per_line);
+  std::vector<double> rec((size_t) cs->ntotal[0]*cs->ntotal[1]*cs->ntotal[2]*count);
+  const std::string name = ldw::io::data_name(stub, timestep);
+  std::string err;
+  if (ldw::io::read_data(name, l, o.iorformat, rec.data(), err)) FAIL("%s: %s", who, err.c_str());
+  scatter_records(cs, rec.data(), stride, count, data);
+  if (o.report) printf("MPIIO read from %s\n", name.c_str());
+  return 0;
+}
+
+extern "C" int lb_io_write(lb_t * lb, int timestep, io_event_t *) {
+  if (!lb) FAIL("lb_io_write: null lb");
+  if (lb_memcpy(lb, tdpMemcpyDeviceToHost)) return -1;
+  return io_write("lb_io_write", "dist", timestep, lb->iodata.output, lb->meta_written, lb->cs,
+		  lb->f, lb->nsite, 19*lb->ndist, lb->ndist);
+}
+
+extern "C" int lb_io_read(lb_t * lb, int timestep, io_event_t *) {
+  if (!lb) FAIL("lb_io_read: null lb");
+  if (io_read("lb_io_read", "dist", timestep, lb->iodata.input, lb->cs, lb->f, lb->nsite,
+	      19*lb->ndist, lb->ndist)) return -1;
+  return lb_memcpy(lb, tdpMemcpyHostToDevice);
+}
+
+extern "C" int field_io_write(field_t * f, int timestep, io_event_t *) {
+  if (!f) FAIL("field_io_write: null field");
+  if (field_memcpy(f, tdpMemcpyDeviceToHost)) return -1;
+  return io_write("field_io_write", f->name, timestep, f->iodata.output, f->meta_written, f->cs,
+		  f->data, f->nsites, f->nf, f->nf);
+}
+
+extern "C" int field_io_read(field_t * f, int timestep, io_event_t *) {
+  if (!f) FAIL("field_io_read: null field");
+  if (io_read("field_io_read", f->name, timestep, f->iodata.input, f->cs, f->data, f->nsites,
+	      f->nf, f->nf)) return -1;
+  return field_memcpy(f, tdpMemcpyHostToDevice);
+}
+
+extern "C" int hydro_io_write(hydro_t * h, int timestep, io_event_t *) {
+  if (!h) FAIL("hydro_io_write: null hydro");
+  if (hydro_memcpy(h, tdpMemcpyDeviceToHost)) return -1;
+  if (io_write("hydro_io_write", "rho", timestep, h->rho_io.output, h->rho_meta_written, h->cs,
+	       h->rho, h->nsite, 1, 1)) return -1;
+  return io_write("hydro_io_write", "vel", timestep, h->u_io.output, h->u_meta_written, h->cs,
+		  h->u, h->nsite, 3, 3);
+}
+
+// rho and u from the host copy to the device; the force is left alone
+extern "C" int hydro_io_read(hydro_t * h, int timestep, io_event_t *) {
+  if (!h) FAIL("hydro_io_read: null hydro");
+  if (flush_force() || materialise_derived()) return -1;
+  if (io_read("hydro_io_read", "rho", timestep, h->rho_io.input, h->cs, h->rho, h->nsite, 1, 1)) return -1;
+  if (io_read("hydro_io_read", "vel", timestep, h->u_io.input, h->cs, h->u, h->nsite, 3, 3)) return -1;
+  const Geom g = h->geo;
+  if (h->prec == 64) {
+    std::vector<double> b(g.nsite), b3(3*g.nsite);
+    pack_host<double>(h->cs, h->rho, 1, b.data());
+    pack_host<double>(h->cs, h->u, 3, b3.data());
+    HIP_CHECK(hipMemcpyAsync(h->d_rho, b.data(), 8*g.nsite, hipMemcpyHostToDevice, g_stream));
+    HIP_CHECK(hipMemcpyAsync(h->d_u, b3.data(), 24*g.nsite, hipMemcpyHostToDevice, g_stream));
+    HIP_CHECK(hipStreamSynchronize(g_stream));
+  }
+  else {
+    std::vector<float> b(g.nsite), b3(3*g.nsite);
+    pack_host<float>(h->cs, h->rho, 1, b.data());
+    pack_host<float>(h->cs, h->u, 3, b3.data());
+    HIP_CHECK(hipMemcpyAsync(h->d_rho, b.data(), 4*g.nsite, hipMemcpyHostToDevice, g_stream));
+    HIP_CHECK(hipMemcpyAsync(h->d_u, b3.data(), 12*g.nsite, hipMemcpyHostToDevice, g_stream));
+    HIP_CHECK(hipStreamSynchronize(g_stream));
+  }
+  h->u_uniform = 0;
+  return 0;
 }
 
 extern "C" int ldw_lb_init_random_velocity(lb_t * lb, int seed, double amp) {

@@ -21,13 +21,22 @@
  *    N_cycles N_start size grid periodicity boundary_walls
  *    viscosity viscosity_bulk fluid_rho0
  *    free_energy (none | symmetric)  A B K | symmetric_a symmetric_b
- *    symmetric_kappa  mobility  phi0  phi_initialisation (spinodal)
+ *    symmetric_kappa  mobility  phi0  phi_initialisation (spinodal, drop)
+ *    phi_init_drop_radius phi_init_drop_amplitude phi_init_drop_centred
  *    noise  random_seed  fd_gradient_calculation (3d_7pt_fluid, 3d_27pt_fluid)
  *    fd_advection_scheme_order  fe_force_method  lb_relaxation_scheme
  *    ghost_modes  lb_ghost_modes  cahn_hilliard_options_conserve
  *    distribution_initialisation (3d_uniform_u) distribution_uniform_u
- *    freq_statistics  config_at_end  colloid_init  hydrodynamics
+ *    freq_statistics  freq_config  config_at_end  colloid_init  hydrodynamics
  *    isothermal_fluctuations lb_fluctuations temperature
+ *    {default, default_input, default_output, lb, lb_input, lb_output, phi,
+ *     rho, vel, ...}_io_{mode, format, report}, {default, lb, phi, rho,
+ *     vel}_io_{grid, freq}   (io_info_args_rt.c:33-140, io_options_rt.c)
+ *
+ *  Restart (N_start > 0) reads the distribution, phi, rho and vel files of
+ *  step N_start from the working directory (ludwig.c:313-364); a
+ *  configuration step (freq_config, config_at_end; control.c:138-147)
+ *  writes them (ludwig.c:868-962), in the reference's file format.
  *
  *****************************************************************************/
 
@@ -140,11 +149,70 @@ static int get_dvec(const char * key, double v[3]) {
   return 1;
 }
 
+/* runtime.c:535-554 */
 static int switch_on(const char * key) {
   char s[64] = "";
   if (!get_string(key, s, sizeof(s))) return 0;
-  return (strcmp(s, "on") == 0 || strcmp(s, "yes") == 0 || strcmp(s, "1") == 0);
+  return (strcmp(s, "Yes") == 0 || strcmp(s, "yes") == 0 || strcmp(s, "1") == 0
+	  || strcmp(s, "on") == 0 || strcmp(s, "true") == 0);
 }
+
+static void lower(char * s) {
+  for (; *s; s++) if (*s >= 'A' && *s <= 'Z') *s = (char) (*s - 'A' + 'a');
+}
+
+/* io_options_rt.c:38-180: "<stub>_io_mode", "<stub>_io_format",
+ * "<stub>_io_report", where present */
+static void io_options_keys(const char * stub, io_options_t * o) {
+  char key[192], s[64];
+  snprintf(key, sizeof(key), "%s_io_mode", stub);
+  if (get_string(key, s, sizeof(s))) {
+    lower(s);
+    if (strcmp(s, "mpiio") != 0) fatal("I/O mode not recognised (mpiio is) for key ", key);
+    o->mode = IO_MODE_MPIIO;
+  }
+  snprintf(key, sizeof(key), "%s_io_format", stub);
+  if (get_string(key, s, sizeof(s))) {
+    lower(s);
+    if (strcmp(s, "ascii") == 0) o->iorformat = IO_RECORD_ASCII;
+    else if (strcmp(s, "binary") == 0) o->iorformat = IO_RECORD_BINARY;
+    else fatal("I/O record format should be either 'ascii' or 'binary' for key ", key);
+  }
+  snprintf(key, sizeof(key), "%s_io_report", stub);
+  if (get(key)) o->report = switch_on(key);
+}
+
+/* io_info_args_rt.c:33-140: input and output options in increasing order
+ * of precedence, then the shared I/O grid and the output frequency */
+static io_info_args_t io_args_keys(const char * stub) {
+  io_info_args_t a = io_info_args_default();
+  char k[160];
+  io_options_keys("default", &a.input);
+  io_options_keys("default_input", &a.input);
+  io_options_keys(stub, &a.input);
+  snprintf(k, sizeof(k), "%s_input", stub);
+  io_options_keys(k, &a.input);
+  io_options_keys("default", &a.output);
+  io_options_keys("default_output", &a.output);
+  io_options_keys(stub, &a.output);
+  snprintf(k, sizeof(k), "%s_output", stub);
+  io_options_keys(k, &a.output);
+  get_ivec("default_io_grid", a.grid);
+  snprintf(k, sizeof(k), "%s_io_grid", stub);
+  get_ivec(k, a.grid);
+  if (a.grid[0] != 1 || a.grid[1] != 1 || a.grid[2] != 1) {
+    fatal("One GPU writes one file: the I/O grid must be 1_1_1 for ", stub);
+  }
+  for (int d = 0; d < 3; d++) a.input.iogrid[d] = a.output.iogrid[d] = a.grid[d];
+  get_int("default_io_freq", &a.iofreq);
+  snprintf(k, sizeof(k), "%s_io_freq", stub);
+  get_int(k, &a.iofreq);
+  if (a.iofreq < 0) fatal("The I/O frequency must be a non-negative integer for ", stub);
+  return a;
+}
+
+/* util.c:1082-1085 */
+static int util_mod(int a, int b) { return (b == 0) ? a : a % b; }
 
 /* ------------------------------------------------------------------------ */
 /* Statistics in the reference's formats                                    */
@@ -284,7 +352,7 @@ int main(int argc, char ** argv) {
   int ncycles = 0, nstart = 0;
   if (!get_int("N_cycles", &ncycles)) fatal("Please set N_cycles in the input file", NULL);
   get_int("N_start", &nstart);
-  if (nstart != 0) fatal("N_start > 0 (restart) is not supported by this build", NULL);
+  if (nstart < 0) fatal("N_start must not be negative", NULL);
 
   int size[3] = {64, 64, 64}, grid[3] = {1, 1, 1}, per[3] = {1, 1, 1}, walls[3] = {0, 0, 0};
   get_ivec("size", size);
@@ -350,6 +418,7 @@ int main(int argc, char ** argv) {
   lees_edw_create(pe, cs, &leo, &le);
 
   lb_data_options_t lbo = lb_data_options_default();
+  lbo.iodata = io_args_keys("lb");
   if (get_string("lb_relaxation_scheme", s, sizeof(s))) {
     if (strcmp(s, "m10") == 0 || strcmp(s, "M10") == 0) lbo.nrelax = LB_RELAXATION_M10;
     else if (strcmp(s, "bgk") == 0 || strcmp(s, "BGK") == 0) lbo.nrelax = LB_RELAXATION_BGK;
@@ -363,17 +432,26 @@ int main(int argc, char ** argv) {
   if (get_string("lb_ghost_modes", s, sizeof(s)) && strcmp(s, "off") == 0) lb_collision_ghost_modes_off(lb);
 
   hydro_options_t ho = hydro_options_default();
+  ho.rho.iodata = io_args_keys("rho");
+  ho.u.iodata = io_args_keys("vel");
+  if (ho.rho.iodata.iofreq > 0 || ho.u.iodata.iofreq > 0) {
+    fatal("rho and vel output between configuration steps (rho_io_freq, vel_io_freq, "
+	  "default_io_freq) is not supported by this build", NULL);
+  }
   hydro_create(pe, cs, le, &ho, &hydro);
   map_options_t mo = map_options_default();
   map_create(pe, cs, &mo, &map);
   wall_create(pe, cs, map, lb, &wall);
 
   int seed = 13;                           /* field_phi_init_rt.c:23 */
+  int phi_iofreq = 0;
   get_int("random_seed", &seed);
   printf("[User   ] Random number seed: %d\n", seed);
 
   if (symmetric) {
     field_options_t fo = field_options_ndata_nhalo(1, 2);
+    fo.iodata = io_args_keys("phi");
+    phi_iofreq = fo.iodata.iofreq;
     field_create(pe, cs, le, "phi", &fo, &phi);
     field_grad_create(pe, phi, 2, &dphi);
     grad_ft d2 = grad_3d_7pt_fluid_d2;
@@ -410,11 +488,27 @@ int main(int argc, char ** argv) {
     double phi0 = 0.0, noise = 0.1;
     get_double("phi0", &phi0);
     physics_phi0_set(phys, phi0);
-    if (get_string("phi_initialisation", s, sizeof(s)) && strcmp(s, "spinodal") != 0) {
-      fatal("phi_initialisation not supported by this build (spinodal is): ", s);
+    /* field_phi_init_rt.c:40-160 */
+    char init[64] = "spinodal";
+    get_string("phi_initialisation", init, sizeof(init));
+    if (strcmp(init, "spinodal") == 0) {
+      printf("Initialising phi for spinodal\n");
+      get_double("noise", &noise);
+      field_phi_init_spinodal(phi, seed, phi0, noise);
     }
-    get_double("noise", &noise);
-    field_phi_init_spinodal(phi, seed, phi0, noise);
+    else if (strcmp(init, "drop") == 0) {
+      double radius = 8.0, phistar = 1.0, xi = 0.0;
+      get_double("phi_init_drop_radius", &radius);
+      get_double("phi_init_drop_amplitude", &phistar);
+      int centred = switch_on("phi_init_drop_centred");
+      printf("Initialising droplet radius:     %14.7e\n", radius);
+      printf("Initialising droplet amplitude:  %14.7e\n", phistar);
+      fe_symm_interfacial_width(fe, &xi);
+      field_phi_init_drop(phi, xi, radius, phistar, centred);
+    }
+    else {
+      fatal("phi_initialisation not supported by this build (spinodal, drop are): ", init);
+    }
   }
 
   /* LB initial state (distribution_rt.c): rest at rho0, or uniform u */
@@ -433,10 +527,28 @@ int main(int argc, char ** argv) {
     lb_init_rest_f(lb, rho0);
   }
 
-  int freq_stats = 0;
+  /* Restart (ludwig.c:313-364): distributions, phi, then rho and vel */
+  if (nstart > 0) {
+    io_event_t event = {0};
+    printf("Re-starting simulation at step %d with data read from file\n", nstart);
+    printf("Reading distribution files for step %d\n", nstart);
+    lb_io_read(lb, nstart, &event);
+    if (symmetric) {
+      printf("Reading phi files for step %d\n", nstart);
+      field_io_read(phi, nstart, &event);
+    }
+    printf("Reading rho/vel files for step %d\n", nstart);
+    hydro_io_read(hydro, nstart, &event);
+  }
+
+  /* control.c:42-86, 138-147 */
+  int freq_stats = 0, freq_config = 100000000, config_at_end = 1;
   get_int("freq_statistics", &freq_stats);
-  get_string("config_at_end", s, sizeof(s));
+  get_int("freq_config", &freq_config);
+  if (freq_config < 1) freq_config = nstart + ncycles + 1;
+  if (get_string("config_at_end", s, sizeof(s)) && strcmp(s, "no") == 0) config_at_end = 0;
   get_string("colloid_io_freq", s, sizeof(s));
+  const int nsteps_end = nstart + ncycles;
 
   /* step 0 statistics (ludwig.c:517-518) */
   lb_memcpy(lb, tdpMemcpyDeviceToHost);
@@ -446,14 +558,14 @@ int main(int argc, char ** argv) {
     field_memcpy(phi, tdpMemcpyDeviceToHost);
     field_grad_memcpy(dphi, tdpMemcpyDeviceToHost);
     stats_phi(phi, cs);
-    stats_fed(fe, cs, 0);
+    stats_fed(fe, cs, nstart);
   }
   stats_momentum(lb, cs);
   printf("\nStarting time step loop.\n");
   fflush(stdout);
 
   double tloop = 0.0;
-  for (int step = 1; step <= ncycles; step++) {
+  for (int step = nstart + 1; step <= nstart + ncycles; step++) {
     double t0 = wtime();
     if (symmetric) {
       ldw_step_binary(lb, hydro, phi, dphi, (fe_t *) fe, pth, pch, 1);
@@ -463,6 +575,24 @@ int main(int argc, char ** argv) {
     }
     ldw_device_synchronize();
     tloop += wtime() - t0;
+
+    /* Configuration and field output (ludwig.c:868-962) */
+    const int config = (step % freq_config == 0) || (config_at_end && step == nsteps_end);
+    if (config) {
+      io_event_t event = {0};
+      printf("Writing distribution output at step %d!\n", step);
+      lb_io_write(lb, step, &event);
+    }
+    if (symmetric && (config || util_mod(step, phi_iofreq) == 0)) {
+      io_event_t event = {0};
+      printf("Writing phi file at step %d!\n", step);
+      field_io_write(phi, step, &event);
+    }
+    if (config) {
+      io_event_t event = {0};
+      printf("Writing rho/velocity output at step %d!\n", step);
+      hydro_io_write(hydro, step, &event);
+    }
 
     if (freq_stats > 0 && step % freq_stats == 0) {
       lb_memcpy(lb, tdpMemcpyDeviceToHost);

@@ -4,8 +4,18 @@ own Ludwig.exe (and two committed regression logs of the reference,
 tests/regression/d3q19-short/serial-dist-3du.log and serial-spin-fd1.log,
 kept here as .ref files with their unchanged .inp inputs; serial-spin-fd1
 uses the 27-point gradient), token by token with the
-absolute tolerance 1e-12 of the reference's tests/awk-fp-diff.sh."""
+absolute tolerance 1e-12 of the reference's tests/awk-fp-diff.sh.
 
+Restart: the reference's tests/regression/d3q19-io/iodrop-mpi1-io{1,2,3}
+run a droplet 0 -> 20 in one go, 0 -> 10 writing a configuration, and
+10 -> 20 from that configuration.  The driver runs the same chain; its logs
+match the reference's, the files it writes at step 10 are byte for byte
+the ones Ludwig.exe writes (SHA-256 and metadata text recorded by
+tests/golden/make_logs.py), and the restarted run ends exactly where the
+single run ends.  The same with ascii records."""
+
+import hashlib
+import json
 import os
 import re
 import subprocess
@@ -74,3 +84,50 @@ def test_log_comparison_catches_a_change():
     with pytest.raises(AssertionError):
         compare_logs("\n".join(lines), want)
     compare_logs(want, want)
+
+
+CHAINS = {"iodrop": ["iodrop-mpi1-io1", "iodrop-mpi1-io2", "iodrop-mpi1-io3"],
+          "iodrop-ascii": ["iodrop-ascii-io2", "iodrop-ascii-io3"]}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("chain", list(CHAINS))
+def test_restart_chain_matches_reference(chain, tmp_path):
+    outs = {}
+    for name in CHAINS[chain]:
+        outs[name] = run_driver(name, tmp_path)
+        want = open(os.path.join(LOGS, name + ".ref")).read()
+        print(name, "\n".join(stat_lines(outs[name])[-4:]), flush=True)
+        compare_logs(outs[name], want)
+    # the configuration files of step 10, against Ludwig.exe's
+    ref = json.load(open(os.path.join(LOGS, chain + "-files.json")))
+    for fn, d in sorted(ref.items()):
+        path = tmp_path / fn
+        assert path.exists(), f"{fn} not written"
+        if "text" in d:
+            assert path.read_text() == d["text"], fn
+        else:
+            data = path.read_bytes()
+            assert len(data) == d["size"], fn
+            assert hashlib.sha256(data).hexdigest() == d["sha256"], f"{fn} differs from the reference's"
+    # 10 -> 20 from the files ends exactly where 0 -> 20 in one run ends
+    if "iodrop-mpi1-io1" in outs:
+        assert stat_lines(outs["iodrop-mpi1-io3"])[-7:] == stat_lines(outs["iodrop-mpi1-io1"])[-7:]
+
+
+def test_reference_restart_is_exact_in_its_own_logs():
+    """The golden logs themselves: Ludwig.exe's 10 -> 20 restart prints the
+    same step-20 statistics as its 0 -> 20 run (momentum aside, whose last
+    digits depend on the OpenMP summation order)."""
+    one = [l for l in stat_lines(open(os.path.join(LOGS, "iodrop-mpi1-io1.ref")).read())[-7:]
+           if not l.startswith(("[total", "[fluid"))]
+    res = [l for l in stat_lines(open(os.path.join(LOGS, "iodrop-mpi1-io3.ref")).read())[-7:]
+           if not l.startswith(("[total", "[fluid"))]
+    assert one == res and len(one) == 5
+    meta = json.load(open(os.path.join(LOGS, "iodrop-files.json")))
+    d = json.loads(meta["dist-metadata.001-001"]["text"])
+    assert d["io_element"]["Count"] == 19 and d["io_options"]["Record format"] == "binary"
+    assert meta["dist-000000010.001-001"]["size"] == 32**3 * 19 * 8
+    a = json.load(open(os.path.join(LOGS, "iodrop-ascii-files.json")))
+    assert a["dist-000000010.001-001"]["size"] == 32**3 * 19 * 24
+    assert a["vel-000000010.001-001"]["size"] == 32**3 * (3 * 23 + 1)
