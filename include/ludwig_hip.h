@@ -398,6 +398,19 @@ int phi_ch_free(phi_ch_t * pch);                              /* phi_cahn_hillia
 int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi,
 		      hydro_t * hydro, map_t * map,
 		      noise_t * noise);                       /* phi_cahn_hilliard.c:213 */
+/* cahn_hilliard_options_conserve (phi_ch_info_t.conserve):
+ *   0  the forward step;
+ *   1  the update as per-site compensated sums, the compensation kept
+ *      between steps (phi_cahn_hilliard.c:1059-1097, 1181-1218);
+ *   2  the forward step, then phi -= (S - field_init_sum)/N with S the
+ *      plain sum of phi in site order, as the reference forms it on one
+ *      thread (phi_cahn_hilliard.c:1102-1175, 1225-1327).  Needs
+ *      cahn_hilliard_stats_time0() first.  The sum is one serial chain of
+ *      additions, so this option costs time in proportion to N.
+ * Either option runs the staged kernels, not the single-pass step. */
+/* field_init_sum of phi, as the reference computes it on one thread
+ * (cahn_hilliard_stats.c:59-76) */
+int cahn_hilliard_stats_time0(phi_ch_t * pch, field_t * phi, map_t * map);
 int advection_order_set(const int order);                     /* advection.h:40 */
 int advection_order(int * order);                             /* advection.h:41 */
 

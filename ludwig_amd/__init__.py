@@ -148,6 +148,7 @@ def lib():
         "phi_ch_create": [vp, vp, vp, ctypes.POINTER(PhiChInfo), pvp], "phi_ch_free": [vp],
         "phi_cahn_hilliard": [vp, vp, vp, vp, vp, vp],
         "advection_order_set": [ctypes.c_int],
+        "cahn_hilliard_stats_time0": [vp, vp, vp],
         "field_phi_init_spinodal": [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double],
         "ldw_lb_init_random_velocity": [vp, ctypes.c_int, ctypes.c_double],
         "ldw_step_binary": [vp, vp, vp, vp, vp, vp, vp, ctypes.c_int],
@@ -425,7 +426,10 @@ class BinaryFluid(_Base):
     solver and the 7-point gradient (nhalo 2)."""
 
     def __init__(self, shape, a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25,
-                 order=1, force_method=FORCE_STRESS_DIVERGENCE, stencil=7, **kw):
+                 order=1, force_method=FORCE_STRESS_DIVERGENCE, stencil=7, conserve=0, **kw):
+        """conserve: cahn_hilliard_options_conserve 0, 1 (compensated
+        update) or 2 (global subtraction; the reference sum is taken by
+        the init_* and read_config methods)."""
         super().__init__(shape, 2, **kw)
         L = lib()
         vp = ctypes.c_void_p
@@ -450,7 +454,8 @@ class BinaryFluid(_Base):
         self.pth = vp()
         _ck(L.pth_create(self.pe, self.cs, force_method, ctypes.byref(self.pth)), "pth_create")
         self.pch = vp()
-        _ck(L.phi_ch_create(self.pe, self.cs, self.le, ctypes.byref(PhiChInfo(0, 0)),
+        self.conserve = conserve
+        _ck(L.phi_ch_create(self.pe, self.cs, self.le, ctypes.byref(PhiChInfo(conserve, 0)),
                             ctypes.byref(self.pch)), "phi_ch_create")
         self.wall = vp()
         _ck(L.wall_create(self.pe, self.cs, self.map, self.lb, ctypes.byref(self.wall)), "wall_create")
@@ -458,11 +463,19 @@ class BinaryFluid(_Base):
     def init_spinodal(self, seed, phi0=0.0, amp=0.1, rho0=1.0):
         _ck(lib().field_phi_init_spinodal(self.phi, seed, phi0, amp), "field_phi_init_spinodal")
         _ck(lib().lb_init_rest_f(self.lb, rho0), "lb_init_rest_f")
+        self._reference_sum()
 
     def init_drop(self, radius, xi, phistar=1.0, centred=False, rho0=1.0):
         _ck(lib().field_phi_init_drop(self.phi, xi, radius, phistar, 1 if centred else 0),
             "field_phi_init_drop")
         _ck(lib().lb_init_rest_f(self.lb, rho0), "lb_init_rest_f")
+        self._reference_sum()
+
+    def _reference_sum(self):
+        # ludwig.c:420-425: conserve 2 needs the initial sum
+        if self.conserve == 2:
+            _ck(lib().cahn_hilliard_stats_time0(self.pch, self.phi, self.map),
+                "cahn_hilliard_stats_time0")
 
     def write_config(self, t):
         """dist, phi, rho and vel files of time step t, as a configuration
@@ -476,6 +489,7 @@ class BinaryFluid(_Base):
         _ck(lib().lb_io_read(self.lb, t, ctypes.byref(ev)), "lb_io_read")
         _ck(lib().field_io_read(self.phi, t, ctypes.byref(ev)), "field_io_read")
         _ck(lib().hydro_io_read(self.hydro, t, ctypes.byref(ev)), "hydro_io_read")
+        self._reference_sum()
 
     def step(self, nsteps=1):
         _ck(lib().advection_order_set(self.order), "advection_order_set")

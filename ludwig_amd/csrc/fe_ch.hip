@@ -24,6 +24,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "ch_conserve.h"
 #include "grad27.h"
 #include "kernels.h"
 
@@ -221,7 +222,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 cahn_hilliard_kernel(Geom geo, FeParams fe, const T * __restrict__ phi,
 		     const T * __restrict__ mu, const T * __restrict__ u,
-		     T * __restrict__ phi_out) {
+		     T * __restrict__ phi_out, T * __restrict__ csum) {
 
   const int64_t site = (int64_t) blockIdx.x*blockDim.x + threadIdx.x;
   if (site >= geo.nsite) return;
@@ -314,9 +315,17 @@ cahn_hilliard_kernel(Geom geo, FeParams fe, const T * __restrict__ phi,
     fzm -= mob*(m0 - M.at(x, y, z - 1));
   }
 
-  // phi_ch_ufs_kernel (phi_cahn_hilliard.c:1039-1044), wz = 1
+  // phi_ch_ufs_kernel (phi_cahn_hilliard.c:1039-1044), wz = 1, or the
+  // compensated update of conserve 1 (phi_cahn_hilliard.c:1181-1218)
   T p = P.at(x, y, z);
-  p -= (+ fe_ - fw + fyp - fym + fzp - fzm);
+  if (csum != nullptr) {
+    T cs = csum[site];
+    ch_update_compensated(p, cs, fe_, fw, fyp, fym, fzp, fzm);
+    csum[site] = cs;
+  }
+  else {
+    p -= (+ fe_ - fw + fyp - fym + fzp - fzm);
+  }
   phi_out[site] = p;
 }
 
@@ -368,18 +377,18 @@ void launch_grad_f64(const Geom & g, int stencil, const double * phi, double * g
 
 void launch_cahn_hilliard_f64(const Geom & g, const FeParams & fe,
 			      const double * phi, const double * mu,
-			      const double * u, double * phi_out,
+			      const double * u, double * phi_out, double * csum,
 			      hipStream_t s) {
   hipLaunchKernelGGL(cahn_hilliard_kernel<double>, dim3(nblocks(g.nsite)), dim3(256),
-		     0, s, g, fe, phi, mu, u, phi_out);
+		     0, s, g, fe, phi, mu, u, phi_out, csum);
 }
 
 void launch_cahn_hilliard_f32(const Geom & g, const FeParams & fe,
 			      const float * phi, const float * mu,
-			      const float * u, float * phi_out,
+			      const float * u, float * phi_out, float * csum,
 			      hipStream_t s) {
   hipLaunchKernelGGL(cahn_hilliard_kernel<float>, dim3(nblocks(g.nsite)), dim3(256),
-		     0, s, g, fe, phi, mu, u, phi_out);
+		     0, s, g, fe, phi, mu, u, phi_out, csum);
 }
 
 void launch_fill_f64(double * a, int64_t n, double v, hipStream_t s) {

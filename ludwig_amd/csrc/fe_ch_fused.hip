@@ -35,7 +35,7 @@ template <typename T, int ORDER, int METHOD, bool HAVE_U>
 __global__ void __launch_bounds__(S::NT)
 force_ch_kernel(Geom g, FeParams fe, int xchunk, int nbz, int nby, int nbx, int remap, const T * __restrict__ phi,
 		const T * __restrict__ u, T * __restrict__ force, T * __restrict__ phi_out,
-		int force_mode, T fz0, T fz1, T fz2) {
+		T * __restrict__ csum, int force_mode, T fz0, T fz1, T fz2) {
 
   __shared__ Smem<T, S> s;
 
@@ -78,7 +78,8 @@ force_ch_kernel(Geom g, FeParams fe, int xchunk, int nbz, int nby, int nbx, int 
     if (active) {
       const int64_t site = ((int64_t) x*g.ny + y)*g.nz + z;
       T F[3], pnew;
-      force_ch_site<T, S, ORDER, METHOD, HAVE_U>(s, fe, x, ty, tz, uc, F, pnew);
+      force_ch_site<T, S, ORDER, METHOD, HAVE_U>(s, fe, x, ty, tz, uc, F, pnew,
+						 csum ? csum + site : nullptr);
       if (force_mode == 0) {
 	force[site] = fz0 + F[0];
 	force[g.nsite + site] = fz1 + F[1];
@@ -101,7 +102,7 @@ force_ch_kernel(Geom g, FeParams fe, int xchunk, int nbz, int nby, int nbx, int 
 
 template <typename T>
 void launch(const Geom & g, const FeParams & fe, const T * phi, const T * u, T * force,
-	    T * phi_out, int force_mode, const double fzero[3], hipStream_t st) {
+	    T * phi_out, T * csum, int force_mode, const double fzero[3], hipStream_t st) {
   const int bz = (g.nz + S::TZ - 1)/S::TZ;
   const int by = (g.ny + S::TY - 1)/S::TY;
   // enough blocks to fill 256 CUs several times; chunk of x planes per block
@@ -113,7 +114,8 @@ void launch(const Geom & g, const FeParams & fe, const T * phi, const T * u, T *
   dim3 grid(((ntile + NXCD - 1)/NXCD)*NXCD), block(S::TZ, S::TY, 1);
   const T f0 = T(fzero[0]), f1 = T(fzero[1]), f2 = T(fzero[2]);
 #define L_(O, M, U) hipLaunchKernelGGL((force_ch_kernel<T, O, M, U>), grid, block, 0, st, \
-				       g, fe, xchunk, bz, by, bx, tune_remap(), phi, u, force, phi_out, force_mode, f0, f1, f2)
+				       g, fe, xchunk, bz, by, bx, tune_remap(), phi, u, force, phi_out, csum, force_mode, \
+				       f0, f1, f2)
 #define L_U(O, M) do { if (u) L_(O, M, true); else L_(O, M, false); } while (0)
 #define L_M(O) do { if (fe.force_method == 0) L_U(O, 0); else L_U(O, 1); } while (0)
   if (fe.order == 1) L_M(1);
@@ -127,15 +129,15 @@ void launch(const Geom & g, const FeParams & fe, const T * phi, const T * u, T *
 }  // namespace
 
 void launch_force_ch_f64(const Geom & g, const FeParams & fe, const double * phi,
-			 const double * u, double * force, double * phi_out,
+			 const double * u, double * force, double * phi_out, double * csum,
 			 int force_mode, const double fzero[3], hipStream_t s) {
-  launch<double>(g, fe, phi, u, force, phi_out, force_mode, fzero, s);
+  launch<double>(g, fe, phi, u, force, phi_out, csum, force_mode, fzero, s);
 }
 
 void launch_force_ch_f32(const Geom & g, const FeParams & fe, const float * phi,
-			 const float * u, float * force, float * phi_out,
+			 const float * u, float * force, float * phi_out, float * csum,
 			 int force_mode, const double fzero[3], hipStream_t s) {
-  launch<float>(g, fe, phi, u, force, phi_out, force_mode, fzero, s);
+  launch<float>(g, fe, phi, u, force, phi_out, csum, force_mode, fzero, s);
 }
 
 }  // namespace ldw

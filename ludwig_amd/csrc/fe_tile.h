@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "ch_conserve.h"
 #include "grad27.h"
 #include "kernels.h"
 
@@ -193,11 +194,12 @@ __device__ __forceinline__ T adv1(T uf, T phi_self, T phi_other, bool west) {
 
 // Force and Cahn-Hilliard update at site (x, y, z) of the output plane x,
 // from the LDS rings (Q planes x-1, x, x+1; phi planes x-2 .. x+2) and the
-// velocities uc of load_u().  Returns F and the updated phi.
+// velocities uc of load_u().  Returns F and the updated phi; with csum_site
+// (the site's compensation) the update is the compensated one.
 template <typename T, class S, int ORDER, int METHOD, bool HAVE_U>
 __device__ __forceinline__ void force_ch_site(const Smem<T, S> & s, const FeParams & fe,
 					      int x, int ty, int tz, const T (&uc)[9],
-					      T (&F)[3], T & pnew) {
+					      T (&F)[3], T & pnew, T * csum_site = nullptr) {
   constexpr int PZ = S::PZ, QZ = S::QZ, PPLANE = S::PPLANE, QPLANE = S::QPLANE;
   const T kappa = T(fe.kappa);
   const T mob = T(fe.mobility);
@@ -295,7 +297,14 @@ __device__ __forceinline__ void force_ch_site(const Smem<T, S> & s, const FePara
 	fzm -= mob*(m0 - q0[4][c - 1]);
   }
   pnew = p0;
-  pnew -= (+ fe_ - fw + fyp - fym + fzp - fzm);
+  if (csum_site != nullptr) {
+    T cs = *csum_site;
+    ch_update_compensated(pnew, cs, fe_, fw, fyp, fym, fzp, fzm);
+    *csum_site = cs;
+  }
+  else {
+    pnew -= (+ fe_ - fw + fyp - fym + fzp - fzm);
+  }
 }
 
 }  // namespace tile

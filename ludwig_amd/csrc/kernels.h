@@ -88,23 +88,25 @@ void launch_grad_f64(const Geom & g, int stencil, const double * phi, double * g
 		     double * delsq, hipStream_t s);
 
 // K3: Cahn-Hilliard advective + diffusive fluxes and the phi update fused.
-//   u may be nullptr (no hydrodynamics: zero advective flux).
+//   u may be nullptr (no hydrodynamics: zero advective flux).  csum
+//   (nullptr, or nsite values kept between steps) selects the compensated
+//   update of cahn_hilliard_options_conserve 1.
 void launch_cahn_hilliard_f64(const Geom & g, const FeParams & fe,
 			      const double * phi, const double * mu,
-			      const double * u, double * phi_out,
+			      const double * u, double * phi_out, double * csum,
 			      hipStream_t s);
 void launch_cahn_hilliard_f32(const Geom & g, const FeParams & fe,
 			      const float * phi, const float * mu,
-			      const float * u, float * phi_out,
+			      const float * u, float * phi_out, float * csum,
 			      hipStream_t s);
 
 // K23: force (as K2) and the Cahn-Hilliard update (as K3) in one pass,
-// 2.5D-tiled through LDS.  Does not write mu.
+// 2.5D-tiled through LDS.  Does not write mu.  csum as for K3.
 void launch_force_ch_f64(const Geom & g, const FeParams & fe, const double * phi,
-			 const double * u, double * force, double * phi_out,
+			 const double * u, double * force, double * phi_out, double * csum,
 			 int force_mode, const double fzero[3], hipStream_t s);
 void launch_force_ch_f32(const Geom & g, const FeParams & fe, const float * phi,
-			 const float * u, float * force, float * phi_out,
+			 const float * u, float * force, float * phi_out, float * csum,
 			 int force_mode, const double fzero[3], hipStream_t s);
 
 // K123: the whole binary-fluid step (force, Cahn-Hilliard, collision with
@@ -247,6 +249,17 @@ void launch_klein_sum_f32(const float * a, int64_t n, double * work,
 // sum of the 19 planes of g (distribution total), plus rho0*nsite shift
 void launch_klein_sum_planes_f32(const float * g, int64_t nsite, int nplane,
 				 double * work, double * out, hipStream_t s);
+
+// cahn_hilliard_options_conserve 2: out[0] = the plain FP64 sum of a in
+// site order, ((a0 + a1) + a2) + ..., as the reference forms it on one
+// thread (one block; its cost grows with n, not with the GPU), then
+// a -= (out[0] - target)/n at every site.
+void launch_site_order_sum_f64(const double * a, int64_t n, double * out, hipStream_t s);
+void launch_site_order_sum_f32(const float * a, int64_t n, double * out, hipStream_t s);
+void launch_subtract_drift_f64(double * a, int64_t n, const double * sum, double target,
+			       hipStream_t s);
+void launch_subtract_drift_f32(float * a, int64_t n, const double * sum, double target,
+			       hipStream_t s);
 
 }  // namespace ldw
 

@@ -723,6 +723,7 @@ struct field_s {
   Geom geo;
   io_info_args_t iodata;   // data file options
   int meta_written;
+  double init_sum;         // field_init_sum (cahn_hilliard_stats_time0)
 };
 
 struct field_grad_s {
@@ -1353,25 +1354,105 @@ struct phi_ch_s {
   pe_t * pe;
   cs_t * cs;
   lees_edw_t * le;
+  int prec;
+  void * d_csum;           // conserve 1: per-site compensation, kept between steps
+  double * d_sum;          // conserve 2: the site-order sum of phi
 };
 
+/* phi_cahn_hilliard.c:128-163: conserve 1 keeps a compensation field
+ * ("compensated sum"), zero at the start */
 extern "C" int phi_ch_create(pe_t * pe, cs_t * cs, lees_edw_t * le, phi_ch_info_t * info,
 			     phi_ch_t ** pch) {
+  if (!cs || !cs->init || !pch) FAIL("phi_ch_create: bad arguments");
   if (info && info->noise) FAIL("phi_ch_create: order parameter noise not supported");
-  if (info && info->conserve) FAIL("phi_ch_create: cahn_hilliard_options_conserve %d not supported", info->conserve);
+  if (info && (info->conserve < 0 || info->conserve > 2)) {
+    FAIL("phi_ch_create: cahn_hilliard_options_conserve %d (0, 1 or 2)", info->conserve);
+  }
   phi_ch_t * p = (phi_ch_t *) calloc(1, sizeof(phi_ch_t));
   if (info) p->info = *info;
   p->pe = pe; p->cs = cs; p->le = le;
+  p->prec = precision_now();
+  if (p->info.conserve == 1) {
+    const Geom g = geom_of(cs);
+    const size_t bytes = ((p->prec == 64) ? 8 : 4)*(size_t) g.nsite;
+    if (dmalloc(&p->d_csum, bytes)) return -1;
+    HIP_CHECK(hipMemsetAsync(p->d_csum, 0, bytes, g_stream));
+    HIP_CHECK(hipStreamSynchronize(g_stream));
+  }
+  if (p->info.conserve == 2) {
+    if (dmalloc((void **) &p->d_sum, sizeof(double))) return -1;
+  }
   *pch = p;
   return 0;
 }
 
-extern "C" int phi_ch_free(phi_ch_t * pch) { free(pch); return 0; }
+extern "C" int phi_ch_free(phi_ch_t * pch) {
+  if (!pch) return 0;
+  if (g_ready) (void) hipStreamSynchronize(g_stream);
+  if (pch->d_csum) (void) hipFree(pch->d_csum);
+  if (pch->d_sum) (void) hipFree(pch->d_sum);
+  free(pch);
+  return 0;
+}
+
+/* kahan_add_double (util_sum.c:30-40), host side; no value-changing
+ * optimisations are enabled, so the operations keep their order */
+static void kahan_add_host(double & sum, double & cs, double val) {
+  const double y = val + cs;
+  const double t = sum + y;
+  cs = y - (t - sum);
+  sum = t;
+}
+
+/* cahn_hilliard_stats.c:59-76: the reference sum of conserve 2, formed as
+ * the reference forms it on one thread (cahn_hilliard_stats.c:233-276:
+ * a Kahan sum over the sites in order, added as a Kahan pair into a zero
+ * sum twice, then sum + cs) */
+extern "C" int cahn_hilliard_stats_time0(phi_ch_t * pch, field_t * phi, map_t *) {
+  if (!pch || !phi) FAIL("cahn_hilliard_stats_time0: null argument");
+  if (field_memcpy(phi, tdpMemcpyDeviceToHost)) return -1;
+  double s[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
+  const Geom g = phi->geo;
+  for (int ic = 1; ic <= g.nx; ic++)
+    for (int jc = 1; jc <= g.ny; jc++)
+      for (int kc = 1; kc <= g.nz; kc++) {
+	kahan_add_host(s[0], c[0], phi->data[cs_index(phi->cs, ic, jc, kc)]);
+      }
+  kahan_add_host(s[1], c[1], s[0]);
+  kahan_add_host(s[1], c[1], c[0]);
+  kahan_add_host(s[2], c[2], s[1]);
+  kahan_add_host(s[2], c[2], c[1]);
+  phi->init_sum = s[2] + c[2];
+  return 0;
+}
+
+/* conserve 2 after the forward step (phi_cahn_hilliard.c:1102-1175):
+ * phi -= (S - field_init_sum)/nfluid, S the site-order sum; both kernels
+ * stay on the stream (the reference copies S to the host and back) */
+static int ch_subtract_drift(phi_ch_t * pch, field_t * phi) {
+  if (pch->info.conserve != 2) return 0;
+  const Geom g = phi->geo;
+  if (phi->prec == 64) {
+    ldw::launch_site_order_sum_f64((const double *) phi->d_data[phi->cur], g.nsite, pch->d_sum, g_stream);
+    ldw::launch_subtract_drift_f64((double *) phi->d_data[phi->cur], g.nsite, pch->d_sum,
+				   phi->init_sum, g_stream);
+  }
+  else {
+    ldw::launch_site_order_sum_f32((const float *) phi->d_data[phi->cur], g.nsite, pch->d_sum, g_stream);
+    ldw::launch_subtract_drift_f32((float *) phi->d_data[phi->cur], g.nsite, pch->d_sum,
+				   phi->init_sum, g_stream);
+  }
+  CHECK_LAUNCH();
+  return 0;
+}
 
 /* phi_cahn_hilliard.c:213-290 */
 extern "C" int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi, hydro_t * hydro,
 				 map_t *, noise_t * noise) {
   if (noise && pch->info.noise) FAIL("phi_cahn_hilliard: noise not supported");
+  if (pch->info.conserve && pch->prec != phi->prec) {
+    FAIL("phi_cahn_hilliard: phi_ch_t (%d-bit) and phi (%d-bit) differ in precision", pch->prec, phi->prec);
+  }
   fe_symm_t * fs = as_symm(fe);
   if (!fs) FAIL("phi_cahn_hilliard: only the symmetric free energy is built");
   field_grad_t * dphi = fs->dphi;
@@ -1410,19 +1491,19 @@ extern "C" int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi, hydro
       if (phi->prec == 64) {
 	ldw::launch_force_ch_f64(g, q, (const double *) phi->d_data[phi->cur], (const double *) uf,
 				 (double *) hydro->d_force, (double *) phi->d_data[1 - phi->cur],
-				 mode, hydro->fzero, g_stream);
+				 (double *) pch->d_csum, mode, hydro->fzero, g_stream);
       }
       else {
 	ldw::launch_force_ch_f32(g, q, (const float *) phi->d_data[phi->cur], (const float *) uf,
 				 (float *) hydro->d_force, (float *) phi->d_data[1 - phi->cur],
-				 mode, hydro->fzero, g_stream);
+				 (float *) pch->d_csum, mode, hydro->fzero, g_stream);
       }
     }
     CHECK_LAUNCH();
     hydro->force_uniform = 0;
     phi->cur = 1 - phi->cur;
     phi->version++;
-    return 0;
+    return ch_subtract_drift(pch, phi);
   }
   if (flush_force()) return -1;
 
@@ -1454,18 +1535,20 @@ extern "C" int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi, hydro
     if (phi->prec == 64) {
       ldw::launch_cahn_hilliard_f64(g, p, (const double *) phi->d_data[phi->cur],
 				    (const double *) dphi->d_mu, (const double *) u,
-				    (double *) phi->d_data[1 - phi->cur], g_stream);
+				    (double *) phi->d_data[1 - phi->cur], (double *) pch->d_csum,
+				    g_stream);
     }
     else {
       ldw::launch_cahn_hilliard_f32(g, p, (const float *) phi->d_data[phi->cur],
 				    (const float *) dphi->d_mu, (const float *) u,
-				    (float *) phi->d_data[1 - phi->cur], g_stream);
+				    (float *) phi->d_data[1 - phi->cur], (float *) pch->d_csum,
+				    g_stream);
     }
   }
   CHECK_LAUNCH();
   phi->cur = 1 - phi->cur;
   phi->version++;
-  return 0;
+  return ch_subtract_drift(pch, phi);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -166,4 +166,87 @@ void launch_klein_sum_planes_f32(const float * g, int64_t nsite, int nplane,
   hipLaunchKernelGGL(sum_stage2, dim3(1), dim3(RB), 0, s, work, nb, out);
 }
 
+/* ------------------------------------------------------------------------ */
+/* cahn_hilliard_options_conserve 2 (phi_cahn_hilliard.c:1102-1175,         */
+/* 1225-1327)                                                               */
+/* ------------------------------------------------------------------------ */
+
+// The plain sum in site order.  Each addition depends on the previous one,
+// so one thread adds; waves 1-3 of the block stage the next tile of values
+// in LDS meanwhile (double-buffered), and thread 0 reads only LDS.
+constexpr int SO_TILE = 2048;
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+site_order_sum_kernel(const T * __restrict__ a, int64_t n, double * __restrict__ out) {
+  __shared__ double buf[2][SO_TILE];
+  const int tid = threadIdx.x;
+  const int64_t ntile = (n + SO_TILE - 1)/SO_TILE;
+  double s = 0.0;
+  for (int i = tid; i < SO_TILE; i += 256) buf[0][i] = (i < n) ? (double) a[i] : 0.0;
+  __syncthreads();
+  for (int64_t t = 0; t < ntile; t++) {
+    const int cur = (int) (t & 1);
+    if (tid == 0) {
+      // batches of 16 LDS reads ahead of the 16 dependent additions, so
+      // the chain waits on the adder and not on LDS
+      const int m = (int) ((n - t*SO_TILE < SO_TILE) ? n - t*SO_TILE : SO_TILE);
+      int i = 0;
+      for (; i + 16 <= m; i += 16) {
+	double v[16];
+#pragma unroll
+	for (int k = 0; k < 16; k++) v[k] = buf[cur][i + k];
+#pragma unroll
+	for (int k = 0; k < 16; k++) s += v[k];
+      }
+      for (; i < m; i++) s += buf[cur][i];
+    }
+    else if (tid >= 64 && t + 1 < ntile) {
+      const int64_t base = (t + 1)*SO_TILE;
+      for (int i = tid - 64; i < SO_TILE; i += 192) {
+	buf[1 - cur][i] = (base + i < n) ? (double) a[base + i] : 0.0;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) out[0] = s;
+}
+
+// phi -= (S - phi0)/nfluid at every (all fluid) site, phi_ch_subtract_kernel2
+template <typename T>
+__global__ void __launch_bounds__(256)
+subtract_drift_kernel(T * __restrict__ a, int64_t n, const double * __restrict__ sum,
+		      double target) {
+  const double c = (sum[0] - target)/(double) (int) n;
+  for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x; i < n;
+       i += (int64_t) gridDim.x*blockDim.x) {
+    a[i] = (T) ((double) a[i] - c);
+  }
+}
+
+static unsigned drift_blocks(int64_t n) {
+  int64_t nb = (n + 255)/256;
+  return (unsigned) ((nb > 4096) ? 4096 : nb);
+}
+
+void launch_site_order_sum_f64(const double * a, int64_t n, double * out, hipStream_t s) {
+  hipLaunchKernelGGL(site_order_sum_kernel<double>, dim3(1), dim3(256), 0, s, a, n, out);
+}
+
+void launch_site_order_sum_f32(const float * a, int64_t n, double * out, hipStream_t s) {
+  hipLaunchKernelGGL(site_order_sum_kernel<float>, dim3(1), dim3(256), 0, s, a, n, out);
+}
+
+void launch_subtract_drift_f64(double * a, int64_t n, const double * sum, double target,
+			       hipStream_t s) {
+  hipLaunchKernelGGL(subtract_drift_kernel<double>, dim3(drift_blocks(n)), dim3(256), 0, s,
+		     a, n, sum, target);
+}
+
+void launch_subtract_drift_f32(float * a, int64_t n, const double * sum, double target,
+			       hipStream_t s) {
+  hipLaunchKernelGGL(subtract_drift_kernel<float>, dim3(drift_blocks(n)), dim3(256), 0, s,
+		     a, n, sum, target);
+}
+
 }  // namespace ldw

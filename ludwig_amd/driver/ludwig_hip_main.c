@@ -445,6 +445,7 @@ int main(int argc, char ** argv) {
 
   int seed = 13;                           /* field_phi_init_rt.c:23 */
   int phi_iofreq = 0;
+  int conserve = 0;
   get_int("random_seed", &seed);
   printf("[User   ] Random number seed: %d\n", seed);
 
@@ -470,7 +471,6 @@ int main(int argc, char ** argv) {
     double mob = 0.0;
     if (!get_double("mobility", &mob)) fatal("Please set mobility in the input file", NULL);
     physics_mobility_set(phys, mob);
-    int conserve = 0;
     get_int("cahn_hilliard_options_conserve", &conserve);
     phi_ch_info_t chi = {conserve, 0};
     phi_ch_create(pe, cs, le, &chi, &pch);
@@ -540,6 +540,9 @@ int main(int argc, char ** argv) {
     printf("Reading rho/vel files for step %d\n", nstart);
     hydro_io_read(hydro, nstart, &event);
   }
+
+  /* the reference sum of conserve 2 (ludwig.c:420-425) */
+  if (symmetric && conserve == 2) cahn_hilliard_stats_time0(pch, phi, map);
 
   /* control.c:42-86, 138-147 */
   int freq_stats = 0, freq_config = 100000000, config_at_end = 1;

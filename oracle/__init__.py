@@ -35,7 +35,7 @@ class FeParam(ctypes.Structure):
     _fields_ = [("a", ctypes.c_double), ("b", ctypes.c_double),
                 ("kappa", ctypes.c_double), ("mobility", ctypes.c_double),
                 ("order", ctypes.c_int), ("force_method", ctypes.c_int),
-                ("stencil", ctypes.c_int)]
+                ("stencil", ctypes.c_int), ("conserve", ctypes.c_int)]
 
 
 def lib():
@@ -48,6 +48,8 @@ def lib():
         _lib.ora_kahan_sum.restype = ctypes.c_double
         _lib.ora_klein_sum.restype = ctypes.c_double
         _lib.ora_kahan_sum.argtypes = [ctypes.c_int64, ctypes.c_void_p]
+        _lib.ora_phi_init_sum.restype = ctypes.c_double
+        _lib.ora_phi_init_sum.argtypes = [ctypes.c_int64, ctypes.c_void_p]
         _lib.ora_klein_sum.argtypes = [ctypes.c_int64, ctypes.c_void_p]
     return _lib
 
@@ -78,8 +80,8 @@ def lb_param(tau=None, eta=None, eta_bulk=None, relax=M10, rho0=1.0,
 
 
 def fe_param(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.45, order=1,
-             force_method=0, stencil=7):
-    return FeParam(a, b, kappa, mobility, order, force_method, stencil)
+             force_method=0, stencil=7, conserve=0):
+    return FeParam(a, b, kappa, mobility, order, force_method, stencil, conserve)
 
 
 def tables():
@@ -169,6 +171,36 @@ def cahn_hilliard(shape, fe, phi, delsq, u):
     lib().ora_cahn_hilliard(nx, ny, nz, ctypes.byref(fe), _p(out), _p(delsq),
                             _p(np.ascontiguousarray(u)))
     return out
+
+
+def cahn_hilliard_ext(shape, fe, phi, delsq, u, csum=None, phi_init_sum=0.0):
+    """The update with fe.conserve; returns (phi, csum)."""
+    nx, ny, nz = shape
+    out = np.ascontiguousarray(phi, dtype=np.float64).copy()
+    cs = np.zeros(out.size) if csum is None else np.ascontiguousarray(csum, dtype=np.float64).copy()
+    lib().ora_cahn_hilliard_ext(nx, ny, nz, ctypes.byref(fe), _p(out), _p(delsq),
+                                _p(np.ascontiguousarray(u)), _p(cs), ctypes.c_double(phi_init_sum))
+    return out, cs
+
+
+def phi_init_sum(phi):
+    a = np.ascontiguousarray(phi, dtype=np.float64)
+    return lib().ora_phi_init_sum(a.size, _p(a))
+
+
+def binary_steps_ext(shape, nsteps, lb, fe, f, phi, rho=None, u=None, csum=None,
+                     phi_init_sum=0.0):
+    """binary_steps with the conservation state; returns f, phi, rho, u, csum."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    f = np.ascontiguousarray(f, dtype=np.float64).copy()
+    phi = np.ascontiguousarray(phi, dtype=np.float64).copy()
+    rho = np.ones(n) if rho is None else rho.copy()
+    u = np.zeros(3 * n) if u is None else u.copy()
+    cs = np.zeros(n) if csum is None else np.ascontiguousarray(csum, dtype=np.float64).copy()
+    lib().ora_binary_steps_ext(nx, ny, nz, nsteps, ctypes.byref(lb), ctypes.byref(fe), _p(f),
+                               _p(phi), _p(rho), _p(u), _p(cs), ctypes.c_double(phi_init_sum))
+    return f, phi, rho, u, cs
 
 
 def binary_steps(shape, nsteps, lb, fe, f, phi, rho=None, u=None):

@@ -520,8 +520,25 @@ void ora_force(int nx, int ny, int nz, const ora_fe_param_t * fe,
  * masks (identity for an all-fluid lattice, advection_bcs.c:80-134) and
  * the forward-step divergence (phi_cahn_hilliard.c:1018-1052). */
 
+/* kahan_add_double (util_sum.c:30-40); built without value-changing
+ * optimisations, so the four operations keep their order and rounding */
+static void kahan_add(double * sum, double * cs, double val) {
+  double y = val + *cs;
+  double t = *sum + y;
+  *cs = y - (t - *sum);
+  *sum = t;
+}
+
 void ora_cahn_hilliard(int nx, int ny, int nz, const ora_fe_param_t * fe,
 		       double * phi, const double * delsq, const double * u) {
+  ora_fe_param_t plain = *fe;
+  plain.conserve = 0;
+  ora_cahn_hilliard_ext(nx, ny, nz, &plain, phi, delsq, u, NULL, 0.0);
+}
+
+void ora_cahn_hilliard_ext(int nx, int ny, int nz, const ora_fe_param_t * fe,
+			   double * phi, const double * delsq, const double * u,
+			   double * csum, double phi_init_sum) {
 
   const long nsite = (long) nx*ny*nz;
   const double a1 = -0.213933;
@@ -620,18 +637,59 @@ void ora_cahn_hilliard(int nx, int ny, int nz, const ora_fe_param_t * fe,
 	long ym1 = site3(x, wrap(y - 1, ny), z, ny, nz);
 	long zm1 = site3(x, y, wrap(z - 1, nz), ny, nz);
 	const double wz = 1.0;
-	phi[i0] -= (+ fe_[i0] - fw[i0] + fy[i0] - fy[ym1]
-		    + wz*fz[i0] - wz*fz[zm1]);
+	if (fe->conserve == 1) {
+	  /* phi_ch_csum_kernel (phi_cahn_hilliard.c:1181-1218) */
+	  double sum = phi[i0], cs = csum[i0];
+	  kahan_add(&sum, &cs, -fe_[i0]);
+	  kahan_add(&sum, &cs,  fw[i0]);
+	  kahan_add(&sum, &cs, -fy[i0]);
+	  kahan_add(&sum, &cs,  fy[ym1]);
+	  kahan_add(&sum, &cs, -wz*fz[i0]);
+	  kahan_add(&sum, &cs,  wz*fz[zm1]);
+	  csum[i0] = cs;
+	  phi[i0] = sum;
+	}
+	else {
+	  phi[i0] -= (+ fe_[i0] - fw[i0] + fy[i0] - fy[ym1]
+		      + wz*fz[i0] - wz*fz[zm1]);
+	}
       }
     }
+  }
+
+  if (fe->conserve == 2) {
+    /* phi_ch_subtract_kernel1/2: the sum in site order from 0.0, then the
+     * same correction at every (fluid) site */
+    double s = 0.0;
+    for (long i = 0; i < nsite; i++) s += phi[i];
+    for (long i = 0; i < nsite; i++) phi[i] -= (s - phi_init_sum)/(int) nsite;
   }
 
   free(fw);
 }
 
+double ora_phi_init_sum(int64_t n, const double * phi) {
+  double s1 = 0.0, c1 = 0.0, s2 = 0.0, c2 = 0.0, s3 = 0.0, c3 = 0.0;
+  for (int64_t i = 0; i < n; i++) kahan_add(&s1, &c1, phi[i]);
+  kahan_add(&s2, &c2, s1);       /* kahan_add(&sum, phib[0]) */
+  kahan_add(&s2, &c2, c1);
+  kahan_add(&s3, &c3, s2);       /* kahan_add(&stats->sum1, sum) */
+  kahan_add(&s3, &c3, c2);
+  return s3 + c3;                /* kahan_sum() */
+}
+
 void ora_binary_steps(int nx, int ny, int nz, int nsteps,
 		      const ora_lb_param_t * lb, const ora_fe_param_t * fe,
 		      double * f, double * phi, double * rho, double * u) {
+  ora_fe_param_t plain = *fe;
+  plain.conserve = 0;
+  ora_binary_steps_ext(nx, ny, nz, nsteps, lb, &plain, f, phi, rho, u, NULL, 0.0);
+}
+
+void ora_binary_steps_ext(int nx, int ny, int nz, int nsteps,
+			  const ora_lb_param_t * lb, const ora_fe_param_t * fe,
+			  double * f, double * phi, double * rho, double * u,
+			  double * csum, double phi_init_sum) {
 
   const long nsite = (long) nx*ny*nz;
   double * grad = (double *) malloc(sizeof(double)*3*nsite);
@@ -643,7 +701,7 @@ void ora_binary_steps(int nx, int ny, int nz, int nsteps,
     if (fe->stencil == 27) ora_grad27(nx, ny, nz, phi, grad, delsq);
     else ora_grad7(nx, ny, nz, phi, grad, delsq);
     ora_force(nx, ny, nz, fe, phi, grad, delsq, F);
-    ora_cahn_hilliard(nx, ny, nz, fe, phi, delsq, u);
+    ora_cahn_hilliard_ext(nx, ny, nz, fe, phi, delsq, u, csum, phi_init_sum);
     ora_collide(nx, ny, nz, lb, f, F, rho, u);
     ora_propagate(nx, ny, nz, f, ftmp);
     memcpy(f, ftmp, sizeof(double)*NVEL*nsite);

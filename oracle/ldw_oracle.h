@@ -45,6 +45,7 @@ typedef struct ora_fe_param_s {
   int    order;            /* advection order 1, 2 or 3 */
   int    force_method;     /* 0 stress divergence, 1 phi grad mu */
   int    stencil;          /* gradient: 7 (3d_7pt_fluid, also 0) or 27 (3d_27pt_fluid) */
+  int    conserve;         /* cahn_hilliard_options_conserve 0, 1 or 2 */
 } ora_fe_param_t;
 
 /* D3Q19 tables: velocities, weights, ma (modes x velocities),
@@ -93,6 +94,21 @@ void ora_force(int nx, int ny, int nz, const ora_fe_param_t * fe,
  * advection.c, advection_bcs.c), using delsq of the same phi and u. */
 void ora_cahn_hilliard(int nx, int ny, int nz, const ora_fe_param_t * fe,
 		       double * phi, const double * delsq, const double * u);
+/* The same with the conservation option fe->conserve:
+ *   1: the update as per-site compensated sums, with the compensation
+ *      carried between steps in csum (phi_cahn_hilliard.c:1181-1218,
+ *      util_sum.c:30-40);
+ *   2: the forward step, then phi -= (S - phi_init_sum)/nsite with S the
+ *      plain sum of phi in site order, as the reference computes it on one
+ *      thread (phi_cahn_hilliard.c:1102-1175, 1225-1327).
+ * csum may be NULL unless conserve is 1. */
+void ora_cahn_hilliard_ext(int nx, int ny, int nz, const ora_fe_param_t * fe,
+			   double * phi, const double * delsq, const double * u,
+			   double * csum, double phi_init_sum);
+/* field_init_sum as cahn_hilliard_stats_time0 forms it on one thread
+ * (cahn_hilliard_stats.c:59-76, 233-276): a Kahan sum over sites, added
+ * as a Kahan pair into a zero sum twice, then sum + cs. */
+double ora_phi_init_sum(int64_t n, const double * phi);
 
 /* Full binary-fluid steps in the reference's loop order
  * (ludwig.c:528-860): force, Cahn-Hilliard, collision, propagation.
@@ -101,6 +117,11 @@ void ora_cahn_hilliard(int nx, int ny, int nz, const ora_fe_param_t * fe,
 void ora_binary_steps(int nx, int ny, int nz, int nsteps,
 		      const ora_lb_param_t * lb, const ora_fe_param_t * fe,
 		      double * f, double * phi, double * rho, double * u);
+/* The same with the conservation state of ora_cahn_hilliard_ext */
+void ora_binary_steps_ext(int nx, int ny, int nz, int nsteps,
+			  const ora_lb_param_t * lb, const ora_fe_param_t * fe,
+			  double * f, double * phi, double * rho, double * u,
+			  double * csum, double phi_init_sum);
 
 /* Plain fluid steps (free_energy none): collision with uniform force
  * fglobal only, then propagation. */

@@ -32,6 +32,9 @@
  *    lb_propagation      propagation.c:43
  *    field_grad_compute  field_grad.c:319 (grad_3d_7pt_fluid_d2, or
  *                        grad_3d_27pt_fluid_d2 with LDW_HARNESS_STENCIL=27)
+ *    phi_ch_create       phi_cahn_hilliard.c:128, with
+ *                        LDW_HARNESS_CONSERVE=1|2 as the conserve option
+ *    cahn_hilliard_stats_time0 cahn_hilliard_stats.c:59 (conserve 2)
  *    phi_force_calculation phi_force.c:74
  *    phi_cahn_hilliard   phi_cahn_hilliard.c:213
  *    field_phi_init_spinodal field_phi_init.c:484
@@ -62,6 +65,7 @@
 #include "phi_force.h"
 #include "phi_force_stress.h"
 #include "phi_cahn_hilliard.h"
+#include "cahn_hilliard_stats.h"
 #include "advection.h"
 #include "noise.h"
 #include "leesedwards.h"
@@ -172,7 +176,10 @@ static void sys_create(sys_t * s, int n, int nhalo, lb_relaxation_enum_t relax,
     fe_symm_create(s->pe, s->cs, s->phi, s->dphi, &s->fe);
     pth_create(s->pe, s->cs, force_method, &s->pth);
     {
+      /* LDW_HARNESS_CONSERVE=1|2 selects cahn_hilliard_options_conserve */
+      const char * cv = getenv("LDW_HARNESS_CONSERVE");
       phi_ch_info_t info = {0};
+      if (cv) info.conserve = atoi(cv);
       phi_ch_create(s->pe, s->cs, s->le, &info, &s->pch);
     }
   }
@@ -446,6 +453,11 @@ int main(int argc, char ** argv) {
       dump("phi_init", p0, s.nsites);
       free(p0);
     }
+    /* the reference sum of conserve 2 (ludwig.c:420-425) */
+    if (s.pch->info.conserve == 2) {
+      cahn_hilliard_stats_time0(s.pch, s.phi, s.map);
+      dump("phi_init_sum", &s.phi->field_init_sum, 1);
+    }
     for (int step = 1; step <= nsteps; step++) {
       double fzero[3] = {0}, uzero[3] = {0};
       physics_control_next_step(s.phys);
@@ -472,6 +484,11 @@ int main(int argc, char ** argv) {
     dump("phi", p, s.nsites);
     dump("f", f, 19*(size_t) s.nsites);
     dump_hydro(&s, "");
+    if (s.pch->info.conserve == 1) {
+      double * cs = get_field(&s, s.pch->csum, 1);
+      dump("csum", cs, s.nsites);
+      free(cs);
+    }
   }
   else if (strcmp(which, "single") == 0) {
     /* N steps seed tau */

@@ -79,6 +79,22 @@ CASES = {
                          16, ["phi", "rho", "u"], ["f"], {"LDW_HARNESS_STENCIL": "27"}),
     "binary16_g27_o3_s10": (["binary", 16, 10, 99, 3, A, B, K, MS, 1.0, 0.0, 0.1],
                             16, ["phi", "rho", "u"], ["f"], {"LDW_HARNESS_STENCIL": "27"}),
+    # cahn_hilliard_options_conserve 1 (per-site compensated update) and 2
+    # (global subtraction; its sum depends on the OpenMP thread count, so
+    # the reference runs on one thread, the order the oracle and the GPU
+    # library reproduce)
+    "binary16_c1_s20": (["binary", 16, 20, 8361235, 1, A, B, K, MS, 1.0, 0.05, 0.1],
+                        16, ["phi", "rho", "u", "csum"], ["f"],
+                        {"LDW_HARNESS_CONSERVE": "1", "OMP_NUM_THREADS": "1"}),
+    "binary16_c1_o3_s10": (["binary", 16, 10, 515, 3, A, B, K, MS, 0.8, -0.1, 0.2],
+                           16, ["phi", "rho", "u", "csum"], ["f"],
+                           {"LDW_HARNESS_CONSERVE": "1", "OMP_NUM_THREADS": "1"}),
+    "binary16_c2_s20": (["binary", 16, 20, 8361235, 1, A, B, K, MS, 1.0, 0.05, 0.1],
+                        16, ["phi_init_sum", "phi", "rho", "u"], ["f"],
+                        {"LDW_HARNESS_CONSERVE": "2", "OMP_NUM_THREADS": "1"}),
+    "binary16_c2_o2_s10": (["binary", 16, 10, 616, 2, A, B, K, MS, 0.8, -0.1, 0.2],
+                           16, ["phi_init_sum", "phi", "rho", "u"], ["f"],
+                           {"LDW_HARNESS_CONSERVE": "2", "OMP_NUM_THREADS": "1"}),
 }
 
 NSAMPLE = 256
@@ -97,7 +113,8 @@ def main():
     rng = np.random.default_rng(20261202)
     for name, spec in CASES.items():
         args, n, full, hashed = spec[:4]
-        env = dict(os.environ, OMP_NUM_THREADS="8", **(spec[4] if len(spec) > 4 else {}))
+        env = dict(os.environ, OMP_NUM_THREADS="8")
+        env.update(spec[4] if len(spec) > 4 else {})
         nsite = n ** 3
         samples = np.sort(rng.choice(nsite, size=min(NSAMPLE, nsite), replace=False))
         if only and name not in only:
@@ -108,6 +125,7 @@ def main():
             out = {"args": np.array([str(a) for a in args]), "n": np.array(n)}
             if len(spec) > 4:
                 out["stencil"] = np.array(int(spec[4].get("LDW_HARNESS_STENCIL", 7)))
+                out["conserve"] = np.array(int(spec[4].get("LDW_HARNESS_CONSERVE", 0)))
             out["sample_sites"] = samples
             for key in full:
                 out[key] = np.fromfile(os.path.join(tmp, key + ".f64"))

@@ -142,6 +142,48 @@ def test_binary_step_golden(golden, case):
     b.close()
 
 
+@pytest.mark.parametrize("case", ["binary16_c1_s20", "binary16_c1_o3_s10", "binary16_c2_s20",
+                                  "binary16_c2_o2_s10"])
+def test_binary_conserve_golden(golden, case):
+    """cahn_hilliard_options_conserve 1 (compensated per-site update) and 2
+    (global subtraction with the site-order sum), bit-exact against the
+    reference run on one thread, through the staged kernels that the
+    options select."""
+    g = golden(case)
+    n, nsteps, seed, order, a, b_, k, m, tau, phi0, amp = _binary_args(g)
+    conserve = int(g["conserve"])
+    print(f"case {case} conserve {conserve}", flush=True)
+    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, order=order, tau=tau,
+                      conserve=conserve)
+    b.init_spinodal(seed, phi0, amp)
+    b.step(nsteps)
+    assert_exact(b.phi_field(), g["phi"], "phi")
+    rho, u, _ = b.hydro_fields()
+    assert_exact(rho, g["rho"], "rho")
+    assert_exact(u, g["u"], "u")
+    assert sha(b.f()) == str(g["f_sha256"])
+    b.close()
+
+
+def test_conserve2_vs_oracle_nonuniform_box():
+    """Conserve 2 on a box of odd sizes larger than one sum tile (2048
+    sites), several steps, against the oracle bit for bit."""
+    shape = (17, 13, 11)
+    b = L.BinaryFluid(shape, a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25, order=3, tau=0.9,
+                      conserve=2)
+    b.init_spinodal(4711, 0.2, 0.3)
+    b.step(7)
+    lb = O.lb_param(tau=0.9)
+    fe = O.fe_param(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25, order=3, conserve=2)
+    phi = O.spinodal(shape, 4711, 0.2, 0.3)
+    f = O.equilibrium(shape, rho_uniform=1.0)
+    f1, phi1, rho1, u1, _ = O.binary_steps_ext(shape, 7, lb, fe, f, phi,
+                                               phi_init_sum=O.phi_init_sum(phi))
+    assert_exact(b.phi_field(), phi1, "phi")
+    assert_exact(b.f(), f1, "f")
+    b.close()
+
+
 @pytest.mark.parametrize("case", ["binary32_o1_s100", "binary32_o3_s200", "binary32_m045_s20"])
 def test_binary32_long_golden(golden, case):
     g = golden(case)

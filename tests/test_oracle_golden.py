@@ -80,7 +80,9 @@ def _binary(g):
     s = (n, n, n)
     lb = O.lb_param(tau=tau)
     stencil = int(g["stencil"]) if "stencil" in g else 7
-    fe = O.fe_param(a=a, b=b, kappa=k, mobility=m, order=order, stencil=stencil)
+    conserve = int(g["conserve"]) if "conserve" in g else 0
+    fe = O.fe_param(a=a, b=b, kappa=k, mobility=m, order=order, stencil=stencil,
+                    conserve=conserve)
     phi = O.spinodal(s, seed, phi0, amp)
     f = O.equilibrium(s, rho_uniform=1.0)
     return s, nsteps, lb, fe, phi, f
@@ -117,6 +119,77 @@ def test_binary_steps_full_fields(golden, case):
     assert_exact(rho1, g["rho"], "rho")
     assert_exact(u1, g["u"], "u")
     assert sha(f1) == str(g["f_sha256"])
+
+
+@pytest.mark.parametrize("case", ["binary16_c1_s20", "binary16_c1_o3_s10", "binary16_c2_s20",
+                                  "binary16_c2_o2_s10"])
+def test_binary_steps_conserve(golden, case):
+    """cahn_hilliard_options_conserve 1 (per-site Kahan update with the
+    compensation field, phi_cahn_hilliard.c:1181-1218) and 2 (subtraction of
+    the drift of the site-order sum from the initial Kahan sum,
+    phi_cahn_hilliard.c:1102-1175, 1225-1327; cahn_hilliard_stats.c:59-76),
+    bit-exact against the reference run on one thread."""
+    g = golden(case)
+    s, nsteps, lb, fe, phi, f = _binary(g)
+    assert fe.conserve in (1, 2)
+    phi_sum = 0.0
+    if fe.conserve == 2:
+        phi_sum = O.phi_init_sum(phi)
+        assert phi_sum == float(g["phi_init_sum"][0])
+    f1, phi1, rho1, u1, cs = O.binary_steps_ext(s, nsteps, lb, fe, f, phi, phi_init_sum=phi_sum)
+    assert_exact(phi1, g["phi"], "phi")
+    assert_exact(rho1, g["rho"], "rho")
+    assert_exact(u1, g["u"], "u")
+    assert sha(f1) == str(g["f_sha256"])
+    if fe.conserve == 1:
+        assert_exact(cs, g["csum"], "csum")
+        assert np.any(cs != 0.0)
+    # the options change the trajectory: the plain update differs
+    fe0 = O.fe_param(a=fe.a, b=fe.b, kappa=fe.kappa, mobility=fe.mobility, order=fe.order)
+    _, phi_plain, _, _ = O.binary_steps(s, nsteps, lb, fe0, f, phi)
+    assert np.any(phi_plain != phi1)
+
+
+def test_conserve2_pulls_the_sum_to_the_reference_sum():
+    """Conserve 2 in closed form: with a reference sum 1e-3 above the
+    initial one, one step moves every site by the same (S - S_ref)/N, so
+    the site sum lands on the reference sum; the plain step stays put.  A
+    planted mistake -- the correction with the wrong sign -- misses."""
+    s = (12, 10, 8)
+    n = 12 * 10 * 8
+    phi = O.spinodal(s, 99, 0.3, 0.4)
+    f = O.equilibrium(s, rho_uniform=1.0)
+    lb = O.lb_param(tau=0.8)
+    target = O.phi_init_sum(phi) + 1.0e-3
+    fe2 = O.fe_param(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25, order=1, conserve=2)
+    fe0 = O.fe_param(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25, order=1)
+    _, p2, _, _, _ = O.binary_steps_ext(s, 1, lb, fe2, f, phi, phi_init_sum=target)
+    _, p0, _, _ = O.binary_steps(s, 1, lb, fe0, f, phi)
+    assert abs(float(np.sum(p2)) - target) < 1e-11
+    assert abs(float(np.sum(p0)) - target) > 0.9e-3
+    np.testing.assert_allclose(p2 - p0, 1.0e-3 / n, rtol=1e-8)
+    wrong = p0 - (p2 - p0)
+    assert abs(float(np.sum(wrong)) - target) > 1.9e-3
+
+
+def test_conserve1_is_the_plain_update_up_to_rounding():
+    """Conserve 1 from a zero compensation field: the same update as the
+    plain forward step up to rounding (a dropped or sign-flipped flux term
+    would differ by the size of a flux, ~1e-3 here), with a compensation
+    of at most one ulp of the new phi at every site."""
+    s = (8, 6, 6)
+    phi = O.spinodal(s, 5, 0.1, 0.5)
+    lb = O.lb_param(tau=0.8)
+    f = O.equilibrium(s, rho_uniform=1.0)
+    fe1 = O.fe_param(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25, order=2, conserve=1)
+    fe0 = O.fe_param(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25, order=2)
+    _, p1, _, _, cs = O.binary_steps_ext(s, 1, lb, fe1, f, phi)
+    _, p0, _, _ = O.binary_steps(s, 1, lb, fe0, f, phi)
+    eps = np.finfo(float).eps
+    assert np.max(np.abs(p1 - p0)) <= 8 * eps * np.max(np.abs(phi))
+    assert np.max(np.abs(p1 - phi)) > 1e-4
+    assert np.any(cs != 0.0)
+    assert np.all(np.abs(cs) <= np.spacing(np.abs(p1)))
 
 
 @pytest.mark.parametrize("case", ["binary32_o1_s100", "binary32_o3_s200", "binary32_m045_s20"])
