@@ -617,6 +617,14 @@ int ldw_nn_adam(float * w, const float * g, float * m, float * v, long long n, f
 int ldw_nn_sumsq(const float * a, long long n, double * out);
 /* a[i] *= f */
 int ldw_nn_scale(float * a, long long n, float f);
+/* *out = sum a[i] in FP64 (fixed order), complete on return */
+int ldw_nn_sum(const float * a, long long n, double * out);
+/* a[i] += v */
+int ldw_nn_shift(float * a, long long n, float v);
+/* For each of nplane planes of ns floats: out[p] += (sum now[p] - sum
+ * out[p])/ns, sums in FP64 in a fixed order, all on the stream (no host
+ * read-back) */
+int ldw_nn_restore_sums(const float * now, float * out, long long ns, int nplane);
 
 #ifdef __cplusplus
 }

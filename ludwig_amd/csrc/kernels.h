@@ -231,6 +231,13 @@ hipError_t nn_adam(float * w, const float * g, float * m, float * v, int64_t n, 
 hipError_t nn_scale(float * a, int64_t n, float f, hipStream_t s);
 // sum of a[i]^2 in FP64 into work[0]; work holds 1 + 512 doubles
 hipError_t nn_sumsq(const float * a, int64_t n, double * work, hipStream_t s);
+hipError_t nn_sum(const float * a, int64_t n, double * work, hipStream_t s);
+hipError_t nn_shift(float * a, int64_t n, float v, hipStream_t s);
+// out plane p += (sum now plane p - sum out plane p)/ns for nplane planes of
+// ns floats; work holds nn_restore_sums_scratch(nplane) doubles
+int nn_restore_sums_scratch(int nplane);
+hipError_t nn_restore_sums(const float * now, float * out, int64_t ns, int nplane, double * work,
+			   hipStream_t s);
 
 // Plain device copy of nbytes (multiple of 32) for bandwidth calibration
 void launch_copy(const void * src, void * dst, int64_t nbytes, hipStream_t s);

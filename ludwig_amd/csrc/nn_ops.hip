@@ -284,11 +284,16 @@ __global__ void scale_kernel(float * __restrict__ a, int64_t n, float s) {
   GRID_STRIDE(i, n) a[i] *= s;
 }
 
-// per-block sums of squares in FP64 (fixed reduction order)
+__global__ void shift_kernel(float * __restrict__ a, int64_t n, float s) {
+  GRID_STRIDE(i, n) a[i] += s;
+}
+
+// per-block sums (SQUARE: of squares) in FP64 (fixed reduction order)
+template <bool SQUARE>
 __global__ void sumsq_kernel(const float * __restrict__ a, int64_t n, double * __restrict__ part) {
   __shared__ double sh[256];
   double s = 0.0;
-  GRID_STRIDE(i, n) { const double v = a[i]; s += v*v; }
+  GRID_STRIDE(i, n) { const double v = a[i]; s += SQUARE ? v*v : v; }
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
@@ -296,6 +301,40 @@ __global__ void sumsq_kernel(const float * __restrict__ a, int64_t n, double * _
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Block partial sums of plane p of `now` (part[p*nb + b]) and of `out`
+// (part[(nplane + p)*nb + b]); grid (nb, nplane, 2)
+__global__ void plane_sums_kernel(const float * __restrict__ now, const float * __restrict__ out,
+				  int64_t ns, int nplane, double * __restrict__ part) {
+  __shared__ double sh[256];
+  const float * a = (blockIdx.z == 0 ? now : out) + (int64_t) blockIdx.y*ns;
+  double s = 0.0;
+  for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x; i < ns;
+       i += (int64_t) gridDim.x*blockDim.x) s += a[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int) threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[((int64_t) blockIdx.z*nplane + blockIdx.y)*gridDim.x + blockIdx.x] = sh[0];
+}
+
+// out[p] += (sum now[p] - sum out[p])/ns, the partial sums added in a
+// fixed order by every thread; grid (.., nplane)
+__global__ void plane_shift_kernel(float * __restrict__ out, int64_t ns, int nplane, int nb,
+				   const double * __restrict__ part) {
+  const int p = blockIdx.y;
+  double sa = 0.0, sb = 0.0;
+  for (int b = 0; b < nb; b++) {
+    sa += part[(int64_t) p*nb + b];
+    sb += part[((int64_t) nplane + p)*nb + b];
+  }
+  const float d = (float) ((sa - sb)/(double) ns);
+  float * a = out + (int64_t) p*ns;
+  for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x; i < ns;
+       i += (int64_t) gridDim.x*blockDim.x) a[i] += d;
 }
 
 __global__ void sum_parts_kernel(const double * __restrict__ part, int nb, double * __restrict__ out) {
@@ -428,8 +467,32 @@ hipError_t nn_scale(float * a, int64_t n, float f, hipStream_t s) {
 constexpr int SUMSQ_BLOCKS = 512;
 
 hipError_t nn_sumsq(const float * a, int64_t n, double * work, hipStream_t s) {
-  hipLaunchKernelGGL(sumsq_kernel, dim3(SUMSQ_BLOCKS), dim3(256), 0, s, a, n, work + 1);
+  hipLaunchKernelGGL(sumsq_kernel<true>, dim3(SUMSQ_BLOCKS), dim3(256), 0, s, a, n, work + 1);
   hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(64), 0, s, work + 1, SUMSQ_BLOCKS, work);
+  return hipGetLastError();
+}
+
+hipError_t nn_sum(const float * a, int64_t n, double * work, hipStream_t s) {
+  hipLaunchKernelGGL(sumsq_kernel<false>, dim3(SUMSQ_BLOCKS), dim3(256), 0, s, a, n, work + 1);
+  hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(64), 0, s, work + 1, SUMSQ_BLOCKS, work);
+  return hipGetLastError();
+}
+
+constexpr int RESTORE_BLOCKS = 64;
+
+int nn_restore_sums_scratch(int nplane) { return 2*nplane*RESTORE_BLOCKS; }
+
+hipError_t nn_restore_sums(const float * now, float * out, int64_t ns, int nplane, double * work,
+			   hipStream_t s) {
+  hipLaunchKernelGGL(plane_sums_kernel, dim3(RESTORE_BLOCKS, nplane, 2), dim3(256), 0, s,
+		     now, out, ns, nplane, work);
+  hipLaunchKernelGGL(plane_shift_kernel, dim3(256, nplane), dim3(256), 0, s,
+		     out, ns, nplane, RESTORE_BLOCKS, work);
+  return hipGetLastError();
+}
+
+hipError_t nn_shift(float * a, int64_t n, float v, hipStream_t s) {
+  LAUNCH(shift_kernel, n, a, n, v);
   return hipGetLastError();
 }
 

@@ -22,7 +22,7 @@ struct Scratch {
   size_t bytes = 0;
 };
 
-Scratch g_wgrad, g_bias, g_err, g_sum, g_sumsq;
+Scratch g_wgrad, g_bias, g_err, g_sum, g_sumsq, g_restore;
 
 int scratch(Scratch & s, size_t bytes, void ** out) {
   if (s.bytes < bytes) {
@@ -243,6 +243,34 @@ extern "C" int ldw_nn_adam(float * w, const float * g, float * m, float * v, lon
 extern "C" int ldw_nn_scale(float * a, long long n, float f) {
   READY();
   KCHECK(ldw::nn_scale(a, n, f, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_sum(const float * a, long long n, double * out) {
+  READY();
+  if (out == nullptr) LDW_FAIL("ldw_nn_sum: null output");
+  hipStream_t st = ldw::api::stream();
+  void * work = nullptr;
+  if (scratch(g_sumsq, 513*sizeof(double), &work)) return -1;
+  KCHECK(ldw::nn_sum(a, n, (double *) work, st));
+  LDW_HIP_CHECK(hipMemcpyAsync(out, work, sizeof(double), hipMemcpyDeviceToHost, st));
+  LDW_HIP_CHECK(hipStreamSynchronize(st));
+  return 0;
+}
+
+extern "C" int ldw_nn_restore_sums(const float * now, float * out, long long ns, int nplane) {
+  READY();
+  if (now == nullptr || out == nullptr) LDW_FAIL("ldw_nn_restore_sums: null buffer");
+  if (ns < 1 || nplane < 1 || nplane > 64) LDW_FAIL("ldw_nn_restore_sums: ns %lld nplane %d", ns, nplane);
+  void * work = nullptr;
+  if (scratch(g_restore, sizeof(double)*ldw::nn_restore_sums_scratch(nplane), &work)) return -1;
+  KCHECK(ldw::nn_restore_sums(now, out, ns, nplane, (double *) work, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_shift(float * a, long long n, float v) {
+  READY();
+  KCHECK(ldw::nn_shift(a, n, v, ldw::api::stream()));
   return 0;
 }
 

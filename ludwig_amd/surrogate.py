@@ -455,12 +455,19 @@ class Jumper:
     u_mode: "pred" takes the predicted velocity; "rescale" rescales it to
     the root-mean-square speed before the jump (a mean-square-trained
     prediction shrinks the part of u it cannot predict); "keep" keeps the
-    velocity from before the jump."""
+    velocity from before the jump.
 
-    def __init__(self, net, normaliser, u_mode="pred"):
+    conserve: shift the predicted phi and each velocity component by a
+    constant so that their sums over the replica equal those before the
+    jump.  The dynamics conserve sum phi exactly (flux form) and the total
+    momentum (no net force), which is sum u to within the density
+    variation; the network conserves neither."""
+
+    def __init__(self, net, normaliser, u_mode="pred", conserve=False):
         if u_mode not in ("pred", "rescale", "keep"):
             raise ValueError(f"u_mode {u_mode!r}")
         self.net, self.norm, self.u_mode = net, normaliser, u_mode
+        self.conserve = conserve
         self._io = {}
 
     def _bufs(self, shape, batch):
@@ -497,4 +504,7 @@ class Jumper:
                     if b.value > 0.0:
                         _ck(lib().ldw_nn_scale(_off(io["out"], 4 * ns), 3 * ns, math.sqrt(a.value / b.value)),
                             "ldw_nn_scale")
+                if self.conserve:
+                    _ck(lib().ldw_nn_restore_sums(_v(io["now"][j]), _v(io["out"]), ns, 4),
+                        "ldw_nn_restore_sums")
                 ensemble.load_device(r, io["out"].ptr)

@@ -19,6 +19,9 @@ Protocol:
    pointwise phi error against plain LB, and timed on the GPU (all held-out
    replicas advance together).  A control run of plain LB in FP64 storage
    shows how far two faithful runs drift apart on this chaotic problem.
+   Surrogate variants: the predicted (phi, u) as is, and with the sums of
+   phi and of each velocity component restored to their values before the
+   jump (both are conserved by the dynamics).
 
 Run as a script for the full protocol; prints one JSON object.
 """
@@ -46,7 +49,7 @@ def _sync():
     lib().ldw_device_synchronize()
 
 
-def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
+def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print, variants=None):
     """Plain LB, surrogate + LB and identity + LB on `specs` from the LB
     state at t_start."""
     T = jumps * (every + corr)
@@ -68,11 +71,11 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
     ref64 = [ctl.snapshot(r)[0] for r in range(len(specs))]
     ctl.close()
 
-    def jumped(with_net, u_mode="pred"):
+    def jumped(with_net, u_mode="pred", conserve=False):
         e = Ensemble(specs, precision=32, fatal=True)
         e.tune(reps=2)
         e.step(t_start)
-        jumper = Jumper(net, norm, u_mode)
+        jumper = Jumper(net, norm, u_mode, conserve=conserve)
         if with_net:                 # warm the buffers and kernels once
             e2 = Ensemble(specs, precision=32, fatal=True)
             e2.step(1)
@@ -90,17 +93,20 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
         e.close()
         return phis, dt
 
-    sur, out["surrogate_s"] = jumped(True)
-    ide, out["identity_s"] = jumped(False)
-    sur_r, out["surrogate_u_rescaled_s"] = jumped(True, "rescale")
-    sur_k, out["surrogate_u_kept_s"] = jumped(True, "keep")
+    if variants is None:
+        variants = {"surrogate": ("pred", False), "surrogate_conserved": ("pred", True)}
+    runs = {}
+    for name, (u_mode, conserve) in variants.items():
+        runs[name], out[name + "_s"] = jumped(True, u_mode, conserve)
+    runs["identity"], out["identity_s"] = jumped(False)
+    runs["fp64_control"] = ref64
+    names = tuple(runs)
     rows = []
     for r, s in enumerate(specs):
         shape = (s.nx, s.ny, s.nz)
         Lr = domain_length(ref[r], shape)
-        row = {"replica": r, "L_plain": Lr}
-        for name, phis in (("surrogate", sur), ("surrogate_u_rescaled", sur_r), ("surrogate_u_kept", sur_k),
-                           ("identity", ide), ("fp64_control", ref64)):
+        row = {"replica": r, "shape": shape[0], "L_plain": Lr}
+        for name, phis in runs.items():
             Ls = domain_length(phis[r], shape)
             dev = ref[r] - ref[r].mean()
             row[f"L_{name}"] = Ls
@@ -109,7 +115,6 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
                                                / max(np.sqrt(np.mean(dev ** 2)), 1e-30))
         rows.append(row)
     out["per_replica"] = rows
-    names = ("surrogate", "surrogate_u_rescaled", "surrogate_u_kept", "identity", "fp64_control")
     for k in [f"L_err_{n}" for n in names] + [f"phi_rms_err_{n}" for n in names]:
         out[k + "_mean"] = float(np.mean([row[k] for row in rows]))
         out[k + "_max"] = float(np.max([row[k] for row in rows]))
@@ -117,14 +122,14 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print):
     Lm = float(np.mean([row["L_plain"] for row in rows]))
     for name in names:
         out[f"meanL_err_{name}"] = abs(float(np.mean([row[f"L_{name}"] for row in rows])) - Lm) / Lm
-    out["speedup"] = out["plain_s"] / out["surrogate_s"]
-    out["speedup_u_rescaled"] = out["plain_s"] / out["surrogate_u_rescaled_s"]
+    for name in variants:
+        out["speedup" + name[len("surrogate"):]] = out["plain_s"] / out[name + "_s"]
     log(f"evaluation: {json.dumps({k: v for k, v in out.items() if k != 'per_replica'})}")
     return out
 
 
-def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=64, iters=2000,
-        batch=4, lr=5e-4, jumps=8, corr=20, seed=2026, max_eval=None, train_seconds=None,
+def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=0, iters=2000,
+        batch=4, lr=5e-4, jumps=8, corr=30, seed=2026, max_eval=None, train_seconds=None,
         t_start=500, replica_kw=None, init_from=None, save_to=None, log=print):
     specs = draw_replicas(nrep, seed=seed, sizes=sizes, **(replica_kw or {}))
     train_idx, val_idx = split_by_seed(specs, 0.2, seed)
@@ -158,7 +163,9 @@ def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=64, i
     data.free()
     if save_to:
         net.save(save_to)
-    ev_specs = [specs[i] for i in val_idx if specs[i].nx == eval_size] or [specs[i] for i in val_idx]
+    # eval_size 0: every held-out replica
+    ev_specs = [specs[i] for i in val_idx if specs[i].nx == eval_size or not eval_size] \
+        or [specs[i] for i in val_idx]
     if max_eval:
         ev_specs = ev_specs[:max_eval]
     result = {"replicas": nrep, "steps": steps, "every": every, "data_s": t_data,
@@ -176,12 +183,12 @@ def main():
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--every", type=int, default=250)
-    ap.add_argument("--eval-size", type=int, default=64)
+    ap.add_argument("--eval-size", type=int, default=0, help="0: every held-out size")
     ap.add_argument("--iters", type=int, default=2000)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--jumps", type=int, default=8)
-    ap.add_argument("--corr", type=int, default=20)
+    ap.add_argument("--corr", type=int, default=30)
     ap.add_argument("--max-eval", type=int, default=0)
     ap.add_argument("--train-seconds", type=float, default=0.0)
     ap.add_argument("--t-start", type=int, default=500)

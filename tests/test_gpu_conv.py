@@ -199,3 +199,25 @@ def test_mse_and_adam():
     ss = ctypes.c_double()
     assert lib.ldw_nn_sumsq(vp(dgr), n, ctypes.byref(ss)) == 0
     assert ss.value == pytest.approx(float(np.sum(g.astype(np.float64) ** 2)), rel=1e-12)
+
+
+def test_sum_shift_and_restore_sums():
+    """The jump's conservation step: plane sums of the new state are set to
+    those of the old one by a uniform shift of each plane."""
+    lib = L.lib()
+    rng = np.random.default_rng(5)
+    ns, nplane = 4099, 4
+    now = rng.standard_normal(nplane * ns).astype(np.float32)
+    out = (now + 0.01 + 0.1 * rng.standard_normal(nplane * ns)).astype(np.float32)
+    dnow, dout = dev(now, np.float32), dev(out, np.float32)
+    s = ctypes.c_double()
+    assert lib.ldw_nn_sum(vp(dnow), ns, ctypes.byref(s)) == 0
+    assert s.value == pytest.approx(float(np.sum(now[:ns].astype(np.float64))), rel=1e-12, abs=1e-9)
+    assert lib.ldw_nn_restore_sums(vp(dnow), vp(dout), ns, nplane) == 0
+    got = dout.download().reshape(nplane, ns).astype(np.float64)
+    want_shift = (now.reshape(nplane, ns).astype(np.float64).sum(1)
+                  - out.reshape(nplane, ns).astype(np.float64).sum(1)) / ns
+    assert np.allclose(got - out.reshape(nplane, ns), want_shift[:, None], atol=2e-7)
+    assert np.allclose(got.sum(1), now.reshape(nplane, ns).astype(np.float64).sum(1), atol=ns * 2e-7)
+    assert lib.ldw_nn_shift(vp(dnow), ns, 0.5) == 0
+    assert np.allclose(dnow.download()[:ns], now[:ns] + 0.5, atol=1e-6)
