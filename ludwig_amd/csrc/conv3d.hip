@@ -149,12 +149,20 @@ constexpr int LH_N = LH_X*LH_Y*LH_Z;          // 648 halo positions
 constexpr int LK = 32;                        // input channels per chunk
 constexpr int LNT_MAX = 6;                    // output channel blocks per block
 
+// LDS rows are 4 x 16 bytes (32 channels).  The 16 lanes of a fragment
+// read 16 consecutive rows at one 16-byte part, 64 bytes apart, which puts
+// them on the same banks; the part index is XOR-swizzled with bits of the
+// row index so that they spread over all banks.  swz_shift selects the bits.
+__device__ __forceinline__ int lds_part(int row, int part, int swz_shift) {
+  return part ^ ((row >> swz_shift) & 3);
+}
+
 template <int NT>
 __global__ void __launch_bounds__(256)
 conv3_fwd_lds_kernel(const uint16_t * __restrict__ x, const uint16_t * __restrict__ w,
 		     const float * __restrict__ bias, float * __restrict__ y, uint16_t * __restrict__ act,
 		     int nx, int ny, int nz, int cin, int cout, int accumulate,
-		     int ntx, int nty, int ntz) {
+		     int ntx, int nty, int ntz, int swz) {
   __shared__ uint4 sa[LH_N*(LK/8)];           // halo tile: [pos][4 x 8 channels]   41.5 KB
   __shared__ uint4 sb[3][16*NT][LK/8];        // 3 taps x 16 NT outputs x 32 inputs
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -185,7 +193,7 @@ conv3_fwd_lds_kernel(const uint16_t * __restrict__ x, const uint16_t * __restric
 	const int gx = wrapi(x0 + hx - 1, nx), gy = wrapi(y0 + hy - 1, ny), gz = wrapi(z0 + hz - 1, nz);
 	v = *reinterpret_cast<const uint4 *>(x + ((((int64_t) b*nx + gx)*ny + gy)*nz + gz)*cin + c0 + 8*part);
       }
-      sa[i] = v;
+      sa[4*h + lds_part(h, part, swz)] = v;
     }
     // weights of taps 3s .. 3s+2: [tap][n][part]
     uint4 pre[NBT];
@@ -207,7 +215,7 @@ conv3_fwd_lds_kernel(const uint16_t * __restrict__ x, const uint16_t * __restric
 #pragma unroll
       for (int k = 0; k < NBT; k++) {
 	const int i = threadIdx.x + 256*k;
-	if (i < NB) (&sb[0][0][0])[i] = pre[k];
+	if (i < NB) (&sb[0][0][0])[4*(i >> 2) + lds_part(i >> 2, i & 3, swz)] = pre[k];
       }
       __syncthreads();                        // halo (first stage) and weights in place
       if (st + 1 < 9) fetch(st + 1);
@@ -216,11 +224,14 @@ conv3_fwd_lds_kernel(const uint16_t * __restrict__ x, const uint16_t * __restric
       for (int dz = 0; dz < 3; dz++) {
 	s8 bv[NT];
 #pragma unroll
-	for (int n = 0; n < NT; n++) bv[n] = *reinterpret_cast<const s8 *>(&sb[dz][16*n + r][q]);
+	for (int n = 0; n < NT; n++) {
+	  const int row = dz*16*NT + 16*n + r;
+	  bv[n] = *reinterpret_cast<const s8 *>(&sb[dz][16*n + r][lds_part(row, q, swz)]);
+	}
 #pragma unroll
 	for (int j = 0; j < 4; j++) {
 	  const int h = ((wave + dx)*LH_Y + (j + dy))*LH_Z + (r + dz);
-	  const s8 a = *reinterpret_cast<const s8 *>(&sa[h*4 + q]);
+	  const s8 a = *reinterpret_cast<const s8 *>(&sa[h*4 + lds_part(h, q, swz)]);
 #pragma unroll
 	  for (int n = 0; n < NT; n++)
 	    acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a),
@@ -364,6 +375,16 @@ __global__ void to_bf16_kernel(const float * __restrict__ a, uint16_t * __restri
     b[i] = to_bf16(a[i]);
 }
 
+// LDS swizzle shift of the tiled kernel (LDW_CONV_SWZ; 16 = no swizzle)
+int conv_swz() {
+  static int v = -1;
+  if (v < 0) {
+    const char * e = getenv("LDW_CONV_SWZ");
+    v = e ? atoi(e) : 1;
+  }
+  return v;
+}
+
 // LDW_CONV_LDS=0 selects the fragment-direct kernel everywhere (for A/B timing)
 int conv_lds_enabled() {
   static int on = -1;
@@ -392,7 +413,7 @@ hipError_t launch_conv3_fwd(const uint16_t * x, const uint16_t * w, const float 
     const int ntx = (nx + LT_X - 1)/LT_X, nty = (ny + LT_Y - 1)/LT_Y, ntz = (nz + LT_Z - 1)/LT_Z;
     dim3 grid((unsigned) ((int64_t) batch*ntx*nty*ntz), nb16/nt), block(256);
 #define LDS_(NT) hipLaunchKernelGGL(conv3_fwd_lds_kernel<NT>, grid, block, 0, s, x, w, bias, y, act, nx, ny, nz, \
-				    cin, cout, accumulate, ntx, nty, ntz)
+				    cin, cout, accumulate, ntx, nty, ntz, conv_swz())
     switch (nt) {
     case 6: LDS_(6); break;
     case 5: LDS_(5); break;
