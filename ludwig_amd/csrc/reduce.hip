@@ -131,11 +131,12 @@ int tune_step_tile(int precision, int64_t nsite) {
     else if (e && strcmp(e, "128x4") == 0) v = 2;
   }
   if (v >= 0) return v;
-  // measured on MI355X (C2 physics): FP64 is fastest with 64 x 8 tiles at
-  // 128^3 and 256^3; FP32 with 128 x 4 tiles up to 128^3 (-17 %) and with
-  // 64 x 4 tiles at 256^3
+  // measured on MI355X (C2 physics, planes per block from the launch cost
+  // model): FP64 is fastest with 64 x 8 tiles at 128^3 and 256^3; FP32 with
+  // 128 x 4 tiles up to 128^3 and with 64 x 8 tiles at 256^3 (660 us
+  // against 686 us for 64 x 4)
   if (precision == 64) return 1;
-  return (nsite <= ((int64_t) 1 << 22)) ? 2 : 0;
+  return (nsite <= ((int64_t) 1 << 22)) ? 2 : 1;
 }
 
 int sum_blocks(int64_t n) {
