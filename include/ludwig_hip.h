@@ -106,6 +106,12 @@ int physics_eta_shear(physics_t * phys, double * eta);
 int physics_eta_bulk(physics_t * phys, double * eta);
 int physics_mobility(physics_t * phys, double * mobility);
 int physics_fbody(physics_t * phys, double f[3]);
+/* the time step counter (ludwig.c:528-531); the Lees-Edwards planes move
+ * with physics_control_time() = t_start + t_current - 1 */
+int physics_control_init_time(physics_t * phys, int nstart, int nsteps); /* physics.h:34 */
+int physics_control_next_step(physics_t * phys);              /* physics.h:33 */
+int physics_control_timestep(physics_t * phys);               /* physics.h:51 */
+int physics_control_time(physics_t * phys, double * t);       /* physics.h:52 */
 
 /* ------------------------------------------------------------------------ */
 /* Host/device copies (target/target.h tdpMemcpyKind)                       */
@@ -228,21 +234,40 @@ int lb_io_write(lb_t * lb, int timestep, io_event_t * event); /* lb_data.c:1716 
 int lb_io_read(lb_t * lb, int timestep, io_event_t * event);  /* lb_data.c:1779 */
 
 /* ------------------------------------------------------------------------ */
-/* Lees-Edwards, map, wall: the periodic, all-fluid, wall-free case only    */
+/* Lees-Edwards planes (steady shear, one rank); map and wall: the          */
+/* all-fluid, wall-free case only                                           */
 /* ------------------------------------------------------------------------ */
 
 typedef struct lees_edw_s lees_edw_t;
+typedef enum lees_edw_enum {                                  /* lees_edwards_options.h:26 */
+  LE_SHEAR_TYPE_INVALID,
+  LE_SHEAR_TYPE_STEADY,
+  LE_SHEAR_TYPE_OSCILLATORY
+} lees_edw_enum_t;
 typedef struct lees_edw_options_s {                           /* lees_edwards_options.h:32 */
   int nplanes;
-  int type;
+  int type;                   /* lees_edw_enum_t; planes need LE_SHEAR_TYPE_STEADY */
   int period;
   int nt0;
   double uy;
 } lees_edw_options_t;
 
+/* nplanes equally spaced planes, plane p between x = loc and loc + 1 with
+ * loc = (int) (L/2n + p L/n); the FP64 storage only; fails for planes
+ * within nhalo of the x boundaries, as the reference does */
 int lees_edw_create(pe_t * pe, cs_t * cs, const lees_edw_options_t * opts,
 		    lees_edw_t ** le);                        /* leesedwards.h:26 */
 int lees_edw_free(lees_edw_t * le);                           /* leesedwards.h:29 */
+int lees_edw_nplane_total(lees_edw_t * le);                   /* leesedwards.h:57 */
+int lees_edw_shear_rate(lees_edw_t * le, double * gammadot);  /* leesedwards.h:63 */
+int lees_edw_plane_location(lees_edw_t * le, int np);         /* leesedwards.h:65 */
+
+/* after lb_collide(), before lb_propagation(): reprojection, displacement
+ * and interpolation of the plane-crossing distributions; no-op without
+ * planes */
+int lb_data_apply_le_boundary_conditions(lb_t * lb, lees_edw_t * le); /* model_le.h:19 */
+/* steady linear shear profile f for rho0 and the plane speed */
+int lb_le_init_shear_profile(lb_t * lb, lees_edw_t * le);     /* model_le.h:20 */
 
 typedef struct map_options_s {int ndata; int is_porous_media;} map_options_t;
 map_options_t map_options_default(void);                      /* map_options.h:29 */

@@ -28,6 +28,7 @@ HEADER = os.path.join(REPO, "include", "ludwig_hip.h")
 M10, BGK, TRT = 0, 1, 2
 H2D, D2H = 1, 2
 FORCE_NO_FORCE, FORCE_STRESS_DIVERGENCE, FORCE_PHI_GRADMU = 1, 2, 3
+LE_SHEAR_TYPE_STEADY = 1                    # lees_edwards_options.h:26
 
 _lib = None
 
@@ -130,6 +131,12 @@ def lib():
         "lb_collision_relaxation_set": [vp, ctypes.c_int],
         "lb_collide": [vp, vp, vp, vp, vp, vp], "lb_propagation": [vp],
         "lees_edw_create": [vp, vp, ctypes.POINTER(LeOptions), pvp], "lees_edw_free": [vp],
+        "lees_edw_nplane_total": [vp], "lees_edw_plane_location": [vp, ctypes.c_int],
+        "lees_edw_shear_rate": [vp, ctypes.POINTER(ctypes.c_double)],
+        "lb_data_apply_le_boundary_conditions": [vp, vp], "lb_le_init_shear_profile": [vp, vp],
+        "physics_control_init_time": [vp, ctypes.c_int, ctypes.c_int],
+        "physics_control_next_step": [vp], "physics_control_timestep": [vp],
+        "physics_control_time": [vp, ctypes.POINTER(ctypes.c_double)],
         "map_create": [vp, vp, ctypes.POINTER(MapOptions), pvp], "map_free": [pvp],
         "wall_create": [vp, vp, vp, vp, pvp], "wall_free": [vp],
         "field_create": [vp, vp, vp, ctypes.c_char_p, ctypes.POINTER(FieldOptions), pvp],
@@ -281,9 +288,11 @@ class _Base:
 
     def __init__(self, shape, nhalo, tau=None, eta=None, eta_bulk=None, rho0=1.0,
                  relax=M10, fbody=(0.0, 0.0, 0.0), precision=64, fatal=False,
-                 io_format=IO_RECORD_BINARY):
+                 io_format=IO_RECORD_BINARY, le_planes=0, le_uy=0.0, le_nt0=0):
         """io_format: record format of configuration files, IO_RECORD_BINARY
-        or IO_RECORD_ASCII, or an (input, output) pair."""
+        or IO_RECORD_ASCII, or an (input, output) pair.  le_planes, le_uy,
+        le_nt0: Lees-Edwards planes (N_LE_plane, LE_plane_vel,
+        LE_time_offset; steady shear, FP64)."""
         L = lib()
         L.ldw_error_mode_set(1 if fatal else 0)
         set_precision(precision)
@@ -309,8 +318,10 @@ class _Base:
         _ck(L.cs_ntotal_set(self.cs, nt), "cs_ntotal_set")
         _ck(L.cs_init(self.cs), "cs_init")
         self.le = vp()
-        _ck(L.lees_edw_create(self.pe, self.cs, ctypes.byref(LeOptions()), ctypes.byref(self.le)),
+        leo = LeOptions(le_planes, LE_SHEAR_TYPE_STEADY if le_planes else 0, 0, le_nt0, le_uy)
+        _ck(L.lees_edw_create(self.pe, self.cs, ctypes.byref(leo), ctypes.byref(self.le)),
             "lees_edw_create")
+        self.le_planes = le_planes
         opts = L.lb_data_options_default()
         opts.nrelax = relax
         io = io_format if isinstance(io_format, tuple) else (io_format, io_format)
@@ -369,6 +380,27 @@ class _Base:
         _ck(lib().lb_halo(self.lb), "lb_halo")
         _ck(lib().lb_propagation(self.lb), "lb_propagation")
 
+    def init_rest(self, rho0=1.0):
+        _ck(lib().lb_init_rest_f(self.lb, rho0), "lb_init_rest_f")
+
+    def le_boundary_conditions(self):
+        """lb_data_apply_le_boundary_conditions, between collide and propagate."""
+        _ck(lib().lb_data_apply_le_boundary_conditions(self.lb, self.le),
+            "lb_data_apply_le_boundary_conditions")
+
+    def le_init_shear_profile(self):
+        _ck(lib().lb_le_init_shear_profile(self.lb, self.le), "lb_le_init_shear_profile")
+
+    # the time step counter the Lees-Edwards planes move with
+    def next_step(self, n=1):
+        for _ in range(n):
+            lib().physics_control_next_step(self.phys)
+
+    def time(self):
+        t = ctypes.c_double()
+        _ck(lib().physics_control_time(self.phys, ctypes.byref(t)), "physics_control_time")
+        return t.value
+
     # configuration files of time step t in the working directory
     # (lb_io_write / hydro_io_write; lb_io_read / hydro_io_read)
     def write_config(self, t):
@@ -413,9 +445,6 @@ class SingleFluid(_Base):
 
     def __init__(self, shape, **kw):
         super().__init__(shape, 1, **kw)
-
-    def init_rest(self, rho0=1.0):
-        _ck(lib().lb_init_rest_f(self.lb, rho0), "lb_init_rest_f")
 
     def init_random_velocity(self, seed, amp=1.0e-3):
         _ck(lib().ldw_lb_init_random_velocity(self.lb, seed, amp), "ldw_lb_init_random_velocity")
@@ -500,6 +529,10 @@ class BinaryFluid(_Base):
                                   self.pch, nsteps), "ldw_step_binary")
 
     # individual stages, as the reference's time loop calls them
+    def gradient_stage(self):
+        _ck(lib().field_halo(self.phi), "field_halo")
+        _ck(lib().field_grad_compute(self.dphi), "field_grad_compute")
+
     def force_stage(self):
         _ck(lib().field_halo(self.phi), "field_halo")
         _ck(lib().field_grad_compute(self.dphi), "field_grad_compute")

@@ -268,6 +268,67 @@ void launch_subtract_drift_f64(double * a, int64_t n, const double * sum, double
 void launch_subtract_drift_f32(float * a, int64_t n, const double * sum, double target,
 			       hipStream_t s);
 
+/* ------------------------------------------------------------------------ */
+/* Lees-Edwards sliding planes (lees_edwards.hip), FP64                      */
+/* ------------------------------------------------------------------------ */
+
+constexpr int LE_MAX_PLANES = 16;
+constexpr int LE_MAX_NH = 4;
+constexpr int LE_MAX_BUF = 2*LE_MAX_NH*LE_MAX_PLANES;
+
+// Plane geometry of one rank (leesedwards.c:430-455): nxbuf = 2 nh nplane
+// buffer planes follow the nx real planes in the "virtual" x index; buffer
+// plane b holds real plane loc[b/(2 nh)] - (nh - 1) + b%(2 nh) displaced
+// in y, downwards for b%(2 nh) < nh and upwards otherwise.
+struct LeGeom {
+  int nplane;
+  int nh;                       // coordinate halo depth nhalo
+  int nxbuf;
+  int sep;                      // nx/nplane (integer division)
+  int loc[LE_MAX_PLANES];       // 1-based x of the site just below plane p
+};
+
+// A y displacement, split as the reference splits it: integer part jdy
+// and fraction fr, per buffer plane (or per plane side)
+struct LeShift {
+  int jdy[LE_MAX_BUF];
+  double fr[LE_MAX_BUF];
+};
+
+// phi of the buffer planes, four-point Lagrange in y (field.c:459-500)
+void launch_le_phi_buffer(const Geom & g, const LeGeom & le, const LeShift & sh,
+			  const double * phi, double * phib, hipStream_t s);
+// gradient and Laplacian at the real sites (x neighbours across a plane
+// from the buffer) and at the buffer sites the reference fills
+// (gradient_3d_7pt_fluid.c:231-310, 330-430; gradient_3d_27pt_fluid.c)
+void launch_le_grad(const Geom & g, const LeGeom & le, int stencil, const double * phi,
+		    const double * phib, double * gr, double * dr, double * gb, double * db,
+		    hipStream_t s);
+// velocity of the buffer planes, linear in y plus the plane jump
+// (hydro.c:350-437)
+void launch_le_u_buffer(const Geom & g, const LeGeom & le, const LeShift & sh, double uy,
+			const double * u, double * ub, hipStream_t s);
+// the flux-form thermodynamic force (phi_force.c:289-520): raw x-face
+// fluxes at the planes, per-plane correction sums, then the force
+void launch_le_force(const Geom & g, const LeGeom & le, const FeParams & fe,
+		     const double * phi, const double * phib, const double * gr,
+		     const double * dr, const double * gb, const double * db,
+		     double * xface, double * fcor, double ra, double * force,
+		     int force_mode, const double fzero[3], hipStream_t s);
+// Cahn-Hilliard fluxes with the plane flux averaging
+// (phi_cahn_hilliard.c:213-290, 350-413, 613-730) and the update;
+// shw/she: the +dy and -dy splits of lees_edw_plane_dy
+void launch_le_cahn_hilliard(const Geom & g, const LeGeom & le, const FeParams & fe,
+			     const double * phi, const double * phib, const double * dr,
+			     const double * db, const double * u, const double * ub,
+			     double * xface, int jdy_w, double fr_w, int jdy_e, double fr_e,
+			     double * phi_out, double * csum, hipStream_t s);
+// lb_data_apply_le_boundary_conditions (model_le.c:78-560) on the
+// post-collision distributions g: side 0 reprojects cx = +1 at plane
+// sites, side 1 cx = -1 above; dj/fr per side, duy the velocity jump
+void launch_le_lb_bc(const Geom & g, const LeGeom & le, const int dj[2], const double fr[2],
+		     const double duy[2], double * gdist, double * sbuf, hipStream_t s);
+
 }  // namespace ldw
 
 #endif

@@ -346,6 +346,7 @@ static Geom geom_of(cs_t * cs) {
 
 struct physics_s {
   double rho0, phi0, eta_shear, eta_bulk, mobility, fbody[3];
+  int t_start, t_current, nsteps;   // time step control (physics.c:600-670)
 };
 
 static physics_t * g_phys = nullptr;
@@ -385,6 +386,27 @@ extern "C" int physics_eta_shear(physics_t * p, double * v) { *v = p->eta_shear;
 extern "C" int physics_eta_bulk(physics_t * p, double * v) { *v = p->eta_bulk; return 0; }
 extern "C" int physics_mobility(physics_t * p, double * v) { *v = p->mobility; return 0; }
 extern "C" int physics_fbody(physics_t * p, double f[3]) { for (int a = 0; a < 3; a++) f[a] = p->fbody[a]; return 0; }
+
+/* physics.c:600-670: the step counter runs nstart + 1 .. nstart + nsteps;
+ * the "time" of step t_current is t_start + t_current - 1 */
+extern "C" int physics_control_init_time(physics_t * p, int nstart, int nsteps) {
+  p->t_start = nstart;
+  p->nsteps = nsteps;
+  p->t_current = nstart;
+  return 0;
+}
+
+extern "C" int physics_control_next_step(physics_t * p) {
+  p->t_current += 1;
+  return (p->t_start + p->nsteps - p->t_current + 1);
+}
+
+extern "C" int physics_control_timestep(physics_t * p) { return p->t_current; }
+
+extern "C" int physics_control_time(physics_t * p, double * t) {
+  *t = 1.0*(p->t_start + p->t_current - 1.0);
+  return 0;
+}
 
 /* ------------------------------------------------------------------------ */
 /* Host <-> device layout                                                   */
@@ -669,15 +691,143 @@ extern "C" int lb_collision_ghost_modes_off(lb_t * lb) { lb->isghost = 0; return
 /* le, map, wall, noise, visc                                               */
 /* ------------------------------------------------------------------------ */
 
-struct lees_edw_s { int nplanes; };
+/* Lees-Edwards planes (leesedwards.c), one rank, steady shear.  Besides
+ * the plane geometry the object owns the device scratch of the plane
+ * kernels: x-face fluxes at the planes, the per-plane force corrections
+ * and the reprojected distributions. */
+struct lees_edw_s {
+  int nplanes;
+  lees_edw_options_t opts;
+  cs_t * cs;
+  int nh;
+  int n[3];
+  double dx_min, dx_sep, time0;
+  ldw::LeGeom geo;
+  double * d_xface;        // nplane*2*3*ny*nz
+  double * d_fcor;         // nplane*3
+  double * d_sbuf;         // nplane*2*5*ny*nz
+};
 
-extern "C" int lees_edw_create(pe_t *, cs_t *, const lees_edw_options_t * opts, lees_edw_t ** le) {
-  if (opts && opts->nplanes != 0) FAIL("lees_edw_create: Lees-Edwards planes are not supported");
-  *le = (lees_edw_t *) calloc(1, sizeof(lees_edw_t));
+static int le_active(const lees_edw_t * le) { return le != nullptr && le->nplanes > 0; }
+
+/* leesedwards.c:81-135, 233-275, 417-455, 464-510 */
+extern "C" int lees_edw_create(pe_t *, cs_t * cs, const lees_edw_options_t * opts, lees_edw_t ** ple) {
+  if (!ple) FAIL("lees_edw_create: null argument");
+  lees_edw_t * le = (lees_edw_t *) calloc(1, sizeof(lees_edw_t));
+  *ple = le;
+  le->cs = cs;
+  if (opts == nullptr || opts->nplanes == 0) return 0;
+  if (!cs || !cs->init) FAIL("lees_edw_create: the coordinate system must be initialised first");
+  le->opts = *opts;
+  le->nplanes = opts->nplanes;
+  le->nh = cs->nhalo;
+  for (int a = 0; a < 3; a++) le->n[a] = cs->ntotal[a];
+  if (opts->nplanes < 0) FAIL("lees_edw_create: %d planes", opts->nplanes);
+  if (le->n[0] % opts->nplanes) {
+    FAIL("lees_edw_create: Number of planes must divide system size (%d planes, x size %d)",
+	 opts->nplanes, le->n[0]);
+  }
+  if (opts->type != LE_SHEAR_TYPE_STEADY) {
+    FAIL("lees_edw_create: only steady shear (LE_SHEAR_TYPE_STEADY) is built, type %d", opts->type);
+  }
+  if (opts->nplanes > ldw::LE_MAX_PLANES) FAIL("lees_edw_create: at most %d planes", ldw::LE_MAX_PLANES);
+  if (!std::isfinite(opts->uy)) FAIL("lees_edw_create: plane speed %g", opts->uy);
+  if (le->nh > ldw::LE_MAX_NH) FAIL("lees_edw_create: halo %d above %d", le->nh, ldw::LE_MAX_NH);
+  le->dx_sep = 1.0*le->n[0]/opts->nplanes;
+  le->dx_min = 0.5*le->dx_sep;
+  le->time0 = 1.0*opts->nt0;
+  ldw::LeGeom & g = le->geo;
+  g.nplane = opts->nplanes;
+  g.nh = le->nh;
+  g.nxbuf = 2*le->nh*opts->nplanes;
+  g.sep = le->n[0]/opts->nplanes;
+  for (int p = 0; p < opts->nplanes; p++) {
+    g.loc[p] = (int) (le->dx_min + p*le->dx_sep);     // lees_edw_plane_location
+    // lees_edw_checks: no plane within nhalo of the x boundaries
+    if (g.loc[p] <= le->nh || g.loc[p] > le->n[0] - le->nh) {
+      FAIL("lees_edw_create: Wall at domain boundary (plane %d at x = %d, halo %d)", p + 1,
+	   g.loc[p], le->nh);
+    }
+  }
+  const size_t nyz = (size_t) le->n[1]*le->n[2];
+  if (dmalloc((void **) &le->d_xface, 8*(size_t) opts->nplanes*2*3*nyz)) return -1;
+  if (dmalloc((void **) &le->d_fcor, 8*(size_t) opts->nplanes*3)) return -1;
+  if (dmalloc((void **) &le->d_sbuf, 8*(size_t) opts->nplanes*2*5*nyz)) return -1;
   return 0;
 }
 
-extern "C" int lees_edw_free(lees_edw_t * le) { free(le); return 0; }
+extern "C" int lees_edw_free(lees_edw_t * le) {
+  if (!le) return 0;
+  if (g_ready) (void) hipStreamSynchronize(g_stream);
+  if (le->d_xface) (void) hipFree(le->d_xface);
+  if (le->d_fcor) (void) hipFree(le->d_fcor);
+  if (le->d_sbuf) (void) hipFree(le->d_sbuf);
+  free(le);
+  return 0;
+}
+
+extern "C" int lees_edw_nplane_total(lees_edw_t * le) { return le ? le->nplanes : 0; }
+
+/* lees_edw_shear_rate (leesedwards.c:700-712) */
+extern "C" int lees_edw_shear_rate(lees_edw_t * le, double * gammadot) {
+  *gammadot = le->nplanes ? le->opts.uy*le->nplanes/(double) le->n[0] : 0.0;
+  return 0;
+}
+
+/* lees_edw_plane_location (leesedwards.c:617-636), 1-based local x */
+extern "C" int lees_edw_plane_location(lees_edw_t * le, int np) {
+  return (le && np >= 0 && np < le->nplanes) ? le->geo.loc[np] : -1;
+}
+
+static double phys_time(void) {
+  physics_t * p = nullptr;
+  physics_ref(&p);
+  double t;
+  physics_control_time(p, &t);
+  return t;
+}
+
+static int phys_timestep(void) {
+  physics_t * p = nullptr;
+  physics_ref(&p);
+  return physics_control_timestep(p);
+}
+
+/* lees_edw_buffer_displacement (leesedwards.c:645-670), steady shear */
+static double le_buffer_displacement(const lees_edw_t * le, int ib, double t) {
+  if (t < 0.0) t = 0.0;
+  const double tle = t - le->time0;
+  const int duy = (ib % (2*le->nh) < le->nh) ? -1 : +1;   // lees_edw_buffer_duy
+  return tle*le->opts.uy*duy;
+}
+
+/* the split of each buffer plane's displacement for field_leesedwards
+ * (field.c:470-475: t0 = 0, fr = 1 - fraction) or hydro_lees_edwards
+ * (hydro.c:398-403: t0 = 1, fr = fraction) */
+/* integer part of a displacement already reduced modulo ly: in [-ly, ly]
+ * for any finite input, which the kernels' periodic indexing relies on */
+static int le_floor(double dy, double ly) {
+  if (!(fabs(dy) <= ly)) {
+    fprintf(stderr, "libludwig_hip: fatal: Lees-Edwards displacement %g outside [-%g, %g]\n", dy, ly, ly);
+    abort();
+  }
+  return (int) floor(dy);
+}
+
+static ldw::LeShift le_buffer_shift(const lees_edw_t * le, double t0, int one_minus) {
+  ldw::LeShift sh;
+  memset(&sh, 0, sizeof(sh));
+  const double ly = (double) le->n[1];
+  const double t = phys_time();
+  for (int ib = 0; ib < le->geo.nxbuf; ib++) {
+    double dy = le_buffer_displacement(le, ib, t + t0);
+    dy = fmod(dy, ly);
+    const int jdy = le_floor(dy, ly);
+    sh.jdy[ib] = jdy;
+    sh.fr[ib] = one_minus ? 1.0 - (dy - jdy) : dy - jdy;
+  }
+  return sh;
+}
 
 struct map_s { int ndata; };
 
@@ -724,6 +874,8 @@ struct field_s {
   io_info_args_t iodata;   // data file options
   int meta_written;
   double init_sum;         // field_init_sum (cahn_hilliard_stats_time0)
+  lees_edw_t * le;         // planes: phi of the buffer planes in d_phib
+  double * d_phib;
 };
 
 struct field_grad_s {
@@ -738,6 +890,10 @@ struct field_grad_s {
   grad_ft d2;
   long grad_version;       // phi version at the last field_grad_compute()
   int grad_buf;            // phi buffer holding that version
+  // with Lees-Edwards planes the gradient is formed by field_grad_compute()
+  // itself: real sites (d_gr, d_dr) and buffer sites (d_gb, d_db)
+  double * d_gr, * d_dr, * d_gb, * d_db;
+  long le_version;
 };
 
 extern "C" field_options_t field_options_ndata_nhalo(int ndata, int nhalo) {
@@ -745,7 +901,7 @@ extern "C" field_options_t field_options_ndata_nhalo(int ndata, int nhalo) {
   return o;
 }
 
-extern "C" int field_create(pe_t * pe, cs_t * cs, lees_edw_t *, const char * name,
+extern "C" int field_create(pe_t * pe, cs_t * cs, lees_edw_t * le, const char * name,
 			    const field_options_t * opts, field_t ** pobj) {
   if (!cs || !cs->init || !opts || !pobj) FAIL("field_create: bad arguments");
   if (opts->ndata != 1) FAIL("field_create: only scalar fields (ndata 1) are on the path, got %d", opts->ndata);
@@ -768,6 +924,17 @@ extern "C" int field_create(pe_t * pe, cs_t * cs, lees_edw_t *, const char * nam
   f->version = 1;
   f->iodata = opts->iodata;
   *pobj = f;
+  if (le_active(le)) {
+    if (f->prec != 64) FAIL("field_create: Lees-Edwards planes are built for the FP64 storage");
+    // stencils reach 2 sites across a plane (third-order advection) and the
+    // gradient is needed one buffer plane deep (lees_edw_ic_to_buff asserts
+    // |di| <= nhalo)
+    if (cs->nhalo < 2) FAIL("field_create: a field with Lees-Edwards planes needs nhalo >= 2, got %d", cs->nhalo);
+    f->le = le;
+    const size_t nb = (size_t) le->geo.nxbuf*f->geo.ny*f->geo.nz;
+    if (dmalloc((void **) &f->d_phib, 8*nb)) return -1;
+    HIP_CHECK(hipMemsetAsync(f->d_phib, 0, 8*nb, g_stream));
+  }
   return 0;
 }
 
@@ -776,6 +943,7 @@ extern "C" int field_free(field_t * f) {
   pending_drop_field(f);
   if (g_ready) (void) hipStreamSynchronize(g_stream);
   for (int b = 0; b < 2; b++) (void) hipFree(f->d_data[b]);
+  if (f->d_phib) (void) hipFree(f->d_phib);
   free(f->data);
   free(f);
   return 0;
@@ -848,6 +1016,14 @@ extern "C" int field_grad_create(pe_t *, field_t * f, int level, field_grad_t **
   g->mu_version = 0;
   g->d2 = grad_3d_7pt_fluid_d2;
   *pobj = g;
+  if (le_active(f->le)) {
+    const size_t n = (size_t) f->geo.nsite;
+    const size_t nb = (size_t) f->le->geo.nxbuf*f->geo.ny*f->geo.nz;
+    if (dmalloc((void **) &g->d_gr, 8*3*n) || dmalloc((void **) &g->d_dr, 8*n)) return -1;
+    if (dmalloc((void **) &g->d_gb, 8*3*nb) || dmalloc((void **) &g->d_db, 8*nb)) return -1;
+    HIP_CHECK(hipMemsetAsync(g->d_gb, 0, 8*3*nb, g_stream));
+    HIP_CHECK(hipMemsetAsync(g->d_db, 0, 8*nb, g_stream));
+  }
   return 0;
 }
 
@@ -857,6 +1033,8 @@ extern "C" void field_grad_free(field_grad_t * g) {
   (void) hipFree(g->d_mu);
   if (g->d_grad) (void) hipFree(g->d_grad);
   if (g->d_delsq) (void) hipFree(g->d_delsq);
+  if (g->d_gr) { (void) hipFree(g->d_gr); (void) hipFree(g->d_dr); }
+  if (g->d_gb) { (void) hipFree(g->d_gb); (void) hipFree(g->d_db); }
   free(g->grad);
   free(g->delsq);
   free(g);
@@ -880,6 +1058,20 @@ static int grad_stencil(const field_grad_t * g) {
 extern "C" int field_grad_compute(field_grad_t * g) {
   g->grad_version = g->field->version;
   g->grad_buf = g->field->cur;
+  field_t * f = g->field;
+  if (le_active(f->le)) {
+    // field_leesedwards() then the gradient (field_grad.c:319-325): the
+    // buffer planes at the current time, then real and buffer sites
+    if (flush_force() || materialise_derived()) return -1;
+    const ldw::LeShift sh = le_buffer_shift(f->le, 0.0, 1);
+    KTimer t("le_gradient");
+    ldw::launch_le_phi_buffer(f->geo, f->le->geo, sh, (const double *) f->d_data[f->cur],
+			      f->d_phib, g_stream);
+    ldw::launch_le_grad(f->geo, f->le->geo, grad_stencil(g), (const double *) f->d_data[f->cur],
+			f->d_phib, g->d_gr, g->d_dr, g->d_gb, g->d_db, g_stream);
+    CHECK_LAUNCH();
+    g->le_version = f->version;
+  }
   return 0;
 }
 
@@ -888,6 +1080,16 @@ extern "C" int field_grad_memcpy(field_grad_t * g, tdpMemcpyKind flag) {
   field_t * f = g->field;
   const Geom geo = f->geo;
   if (f->prec != 64) FAIL("field_grad_memcpy: gradient read-back is built for FP64 only");
+  if (le_active(f->le)) {
+    // the gradient the last field_grad_compute() formed
+    std::vector<double> b1(3*geo.nsite), b2(geo.nsite);
+    HIP_CHECK(hipMemcpyAsync(b1.data(), g->d_gr, 8*3*geo.nsite, hipMemcpyDeviceToHost, g_stream));
+    HIP_CHECK(hipMemcpyAsync(b2.data(), g->d_dr, 8*geo.nsite, hipMemcpyDeviceToHost, g_stream));
+    HIP_CHECK(hipStreamSynchronize(g_stream));
+    unpack_host<double>(f->cs, b1.data(), 3, g->grad);
+    unpack_host<double>(f->cs, b2.data(), 1, g->delsq);
+    return 0;
+  }
   if (!g->d_grad) {
     if (dmalloc(&g->d_grad, 8*3*(size_t) geo.nsite)) return -1;
     if (dmalloc(&g->d_delsq, 8*(size_t) geo.nsite)) return -1;
@@ -953,6 +1155,8 @@ struct hydro_s {
   field_grad_t * fd_dphi;
   io_info_args_t rho_io, u_io;     // "rho" and "vel" file options
   int rho_meta_written, u_meta_written;
+  lees_edw_t * le;                 // planes: velocity of the buffer planes
+  double * d_ub;
 };
 
 static hydro_t * g_pending = nullptr;   // at most one deferred force
@@ -980,7 +1184,7 @@ extern "C" hydro_options_t hydro_options_default(void) {
   return o;
 }
 
-extern "C" int hydro_create(pe_t * pe, cs_t * cs, lees_edw_t *, const hydro_options_t * opts,
+extern "C" int hydro_create(pe_t * pe, cs_t * cs, lees_edw_t * le, const hydro_options_t * opts,
 			    hydro_t ** pobj) {
   if (!cs || !cs->init || !pobj) FAIL("hydro_create: bad arguments");
   const hydro_options_t dflt = hydro_options_default();
@@ -1008,6 +1212,11 @@ extern "C" int hydro_create(pe_t * pe, cs_t * cs, lees_edw_t *, const hydro_opti
   h->u_uniform = 1;
   h->force_uniform = 1;
   *pobj = h;
+  if (le_active(le)) {
+    if (h->prec != 64) FAIL("hydro_create: Lees-Edwards planes are built for the FP64 storage");
+    h->le = le;
+    if (dmalloc((void **) &h->d_ub, 8*3*(size_t) le->geo.nxbuf*h->geo.ny*h->geo.nz)) return -1;
+  }
   return 0;
 }
 
@@ -1017,6 +1226,7 @@ extern "C" int hydro_free(hydro_t * h) {
   if (g_derived == h) g_derived = nullptr;
   if (g_ready) (void) hipStreamSynchronize(g_stream);
   if (h->d_u2) (void) hipFree(h->d_u2);
+  if (h->d_ub) (void) hipFree(h->d_ub);
   (void) hipFree(h->d_rho); (void) hipFree(h->d_u); (void) hipFree(h->d_force);
   free(h->rho); free(h->u); free(h->force);
   free(h);
@@ -1261,12 +1471,32 @@ extern "C" int phi_force_calculation(pe_t *, cs_t *, lees_edw_t * le, wall_t *, 
 				     fe_t * fe, map_t *, field_t * phi, hydro_t * hydro) {
   if (hydro == nullptr) return 0;
   if (pth == nullptr || pth->method == FE_FORCE_METHOD_NO_FORCE) return 0;
-  if (le && le->nplanes > 0) FAIL("phi_force_calculation: Lees-Edwards planes not supported");
   fe_symm_t * fs = as_symm(fe);
   if (!fs) FAIL("phi_force_calculation: only the symmetric free energy is built");
   if (phi->prec != hydro->prec) FAIL("phi_force_calculation: precision mismatch");
   if (flush_force()) return -1;
   if (materialise_derived()) return -1;
+  if (le_active(le)) {
+    // with planes the reference always takes the flux form (phi_force.c:89-97)
+    if (!le_active(phi->le) || phi->le != le) FAIL("phi_force_calculation: phi was not created with these Lees-Edwards planes");
+    field_grad_t * dphi = fs->dphi;
+    if (dphi->le_version != phi->version) {
+      FAIL("phi_force_calculation: with Lees-Edwards planes field_grad_compute() of the current phi must come first");
+    }
+    const Geom g = phi->geo;
+    const FeParams p = fe_params_of(fs, pth->method);
+    const double ra = 0.5/((double) g.ny*(double) g.nz);
+    const int mode = hydro->force_uniform ? 0 : 1;
+    {
+      KTimer t("le_force");
+      ldw::launch_le_force(g, le->geo, p, (const double *) phi->d_data[phi->cur], phi->d_phib,
+			   dphi->d_gr, dphi->d_dr, dphi->d_gb, dphi->d_db, le->d_xface, le->d_fcor,
+			   ra, (double *) hydro->d_force, mode, hydro->fzero, g_stream);
+    }
+    CHECK_LAUNCH();
+    hydro->force_uniform = 0;
+    return 0;
+  }
   hydro->fp_pending = 1;
   hydro->fp_phi = phi;
   hydro->fp_version = phi->version;
@@ -1460,6 +1690,43 @@ extern "C" int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi, hydro
   const Geom g = phi->geo;
   if (materialise_derived()) return -1;
 
+  if (le_active(phi->le)) {
+    // hydro_lees_edwards(), advection_x() and the diffusive fluxes with
+    // the planes, phi_ch_le_fix_fluxes(), the update (phi_cahn_hilliard.c:
+    // 213-290)
+    lees_edw_t * le = phi->le;
+    if (dphi->le_version != phi->version) {
+      FAIL("phi_cahn_hilliard: with Lees-Edwards planes field_grad_compute() of the current phi must come first");
+    }
+    if (flush_force()) return -1;
+    const double * u = nullptr;
+    double * ub = nullptr;
+    if (hydro) {
+      if (hydro->le != le) FAIL("phi_cahn_hilliard: hydro was not created with the planes of phi");
+      if (hydro_materialise(hydro)) return -1;
+      u = (const double *) hydro->d_u;
+      ub = hydro->d_ub;
+      const ldw::LeShift sh = le_buffer_shift(le, 1.0, 0);
+      ldw::launch_le_u_buffer(g, le->geo, sh, le->opts.uy, u, ub, g_stream);
+      CHECK_LAUNCH();
+    }
+    // lees_edw_plane_dy (leesedwards.c:880-893): t uy, no time offset
+    const double dy = phys_time()*le->opts.uy;
+    const double ly = (double) g.ny;
+    const double dyw = fmod(+dy, ly), dye = fmod(-dy, ly);
+    const int jw = le_floor(dyw, ly), je = le_floor(dye, ly);
+    {
+      KTimer t("le_cahn_hilliard");
+      ldw::launch_le_cahn_hilliard(g, le->geo, p, (const double *) phi->d_data[phi->cur], phi->d_phib,
+				   dphi->d_dr, dphi->d_db, u, ub, le->d_xface, jw, dyw - jw, je, dye - je,
+				   (double *) phi->d_data[1 - phi->cur], (double *) pch->d_csum, g_stream);
+    }
+    CHECK_LAUNCH();
+    phi->cur = 1 - phi->cur;
+    phi->version++;
+    return ch_subtract_drift(pch, phi);
+  }
+
   if (hydro && hydro->fp_pending && g_pending == hydro && hydro->fp_phi == phi
       && hydro->fp_version == phi->version) {
     // force and Cahn-Hilliard update in one pass (K23); a uniform zero
@@ -1642,6 +1909,74 @@ extern "C" int lb_collide(lb_t * lb, hydro_t * hydro, map_t *, noise_t * noise, 
   return 0;
 }
 
+/* model_le.c:78-170: on the post-collision distributions of lb_collide();
+ * the displacement is that of time step t = physics_control_timestep() */
+extern "C" int lb_data_apply_le_boundary_conditions(lb_t * lb, lees_edw_t * le) {
+  if (!lb || !le) FAIL("lb_data_apply_le_boundary_conditions: null argument");
+  if (!le_active(le)) return 0;
+  if (lb->prec != 64) FAIL("lb_data_apply_le_boundary_conditions: Lees-Edwards planes are built for FP64");
+  if (lb->streamed) FAIL("lb_data_apply_le_boundary_conditions: needs the post-collision distributions (call it between lb_collide and lb_propagation)");
+  const double t = 1.0*phys_timestep();
+  const double ly = (double) lb->geo.ny;
+  // lees_edw_buffer_displacement at ib = nhalo (sign +1); plane_uy_now
+  const double dy = le_buffer_displacement(le, le->nh, t);
+  int dj[2];
+  double fr[2], duy[2];
+  for (int side = 0; side < 2; side++) {
+    const int cx = 1 - 2*side;
+    dj[side] = le_floor(fmod(dy*cx, ly), ly);
+    const double d = fmod(dy*cx, ly);
+    const int jdy = le_floor(d, ly);
+    fr[side] = d - jdy;
+    duy[side] = -1.0*cx*le->opts.uy;
+  }
+  {
+    KTimer tm("le_lb_bc");
+    ldw::launch_le_lb_bc(lb->geo, le->geo, dj, fr, duy, (double *) lb->d_g[lb->cur], le->d_sbuf, g_stream);
+  }
+  CHECK_LAUNCH();
+  return 0;
+}
+
+/* model_le.c:649-715: f = w (rho0 + rho0 c.u/cs2 + (rho0 u u - eta grad u):Q
+ * / 2 cs2^2) with u_y = lees_edw_steady_uy(x), on the host copy */
+extern "C" int lb_le_init_shear_profile(lb_t * lb, lees_edw_t * le) {
+  if (!lb || !le) FAIL("lb_le_init_shear_profile: null argument");
+  if (!le_active(le)) FAIL("lb_le_init_shear_profile: Cannot use LE_init_profile with no planes");
+  physics_t * phys = nullptr;
+  physics_ref(&phys);
+  const double rho0 = phys->rho0, eta = phys->eta_shear;
+  double u[3] = {0.0, 0.0, 0.0}, gradu[3][3] = {{0.0}};
+  lees_edw_shear_rate(le, &gradu[0][1]);
+  const Geom g = lb->geo;
+  const double cs2 = (1.0/3.0);
+  const double rcs2 = 1.0/cs2;
+  for (int ic = 1; ic <= g.nx; ic++) {
+    // lees_edw_steady_uy (leesedwards.c:520-545)
+    const double xglobal = 0 + (double) ic - 0.5;
+    const int nplane = (int) ((le->dx_min + xglobal)/le->dx_sep);
+    u[1] = xglobal*gradu[0][1] - le->opts.uy*nplane;
+    for (int jc = 1; jc <= g.ny; jc++)
+      for (int kc = 1; kc <= g.nz; kc++) {
+	const int index = cs_index(lb->cs, ic, jc, kc);
+	for (int p = 0; p < 19; p++) {
+	  double cdotu = 0.0, sdotq = 0.0;
+	  for (int i = 0; i < 3; i++) {
+	    cdotu += ldw::CV.v[p][i]*u[i];
+	    for (int j = 0; j < 3; j++) {
+	      const double dij = (i == j);
+	      const double qij = ldw::CV.v[p][i]*ldw::CV.v[p][j] - cs2*dij;
+	      sdotq += (rho0*u[i]*u[j] - eta*gradu[i][j])*qij;
+	    }
+	  }
+	  lb->f[(int64_t) p*lb->nsite + index]
+	    = ldw::weight(p)*(rho0 + rcs2*rho0*cdotu + 0.5*rcs2*rcs2*sdotq);
+	}
+      }
+  }
+  return lb_memcpy(lb, tdpMemcpyHostToDevice);
+}
+
 /* propagation.c:43-105: the streaming is deferred into the next collision
  * (pull); two propagations in a row materialise the first. */
 extern "C" int lb_propagation(lb_t * lb) {
@@ -1664,13 +1999,15 @@ static int step_staged(lb_t * lb, hydro_t * hydro, field_t * phi, field_grad_t *
 		       fe_t * fe, pth_t * pth, phi_ch_t * pch) {
   const double fzero[3] = {0.0, 0.0, 0.0};
   const double uzero[3] = {0.0, 0.0, 0.0};
+  lees_edw_t * le = phi->le;
   if (hydro_f_zero(hydro, fzero)) return -1;
   if (field_halo(phi)) return -1;
   if (field_grad_compute(dphi)) return -1;
-  if (phi_force_calculation(nullptr, nullptr, nullptr, nullptr, pth, fe, nullptr, phi, hydro)) return -1;
+  if (phi_force_calculation(nullptr, nullptr, le, nullptr, pth, fe, nullptr, phi, hydro)) return -1;
   if (phi_cahn_hilliard(pch, fe, phi, hydro, nullptr, nullptr)) return -1;
   if (hydro_u_zero(hydro, uzero)) return -1;
   if (lb_collide(lb, hydro, nullptr, nullptr, fe, nullptr)) return -1;
+  if (le_active(le) && lb_data_apply_le_boundary_conditions(lb, le)) return -1;
   if (lb_halo(lb)) return -1;
   if (lb_propagation(lb)) return -1;
   return 0;
@@ -1736,12 +2073,15 @@ extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_
     g_fused_step = (e && atoi(e) == 0) ? 0 : 1;
   }
   fe_symm_t * fs = as_symm(fe);
+  physics_t * phys = nullptr;
+  physics_ref(&phys);
   for (int step = 0; step < nsteps; step++) {
+    physics_control_next_step(phys);
     const bool fusable = g_fused_step && fs && pth && hydro
       && (pth->method == FE_FORCE_METHOD_STRESS_DIVERGENCE || pth->method == FE_FORCE_METHOD_PHI_GRADMU)
       && lb->streamed && !hydro->u_uniform && !hydro->fp_pending
       && lb->prec == hydro->prec && phi->prec == lb->prec && fs->phi == phi
-      && pch && pch->info.conserve == 0;
+      && pch && pch->info.conserve == 0 && !le_active(phi->le) && !le_active(hydro->le);
     if (fusable) {
       if (step_fused(lb, hydro, phi, fs, pth)) return -1;
     }
@@ -1755,10 +2095,14 @@ extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_
 extern "C" int ldw_step_single(lb_t * lb, hydro_t * hydro, int nsteps) {
   const double fzero[3] = {0.0, 0.0, 0.0};
   const double uzero[3] = {0.0, 0.0, 0.0};
+  physics_t * phys = nullptr;
+  physics_ref(&phys);
   for (int step = 0; step < nsteps; step++) {
+    physics_control_next_step(phys);
     if (hydro_f_zero(hydro, fzero)) return -1;
     if (hydro_u_zero(hydro, uzero)) return -1;
     if (lb_collide(lb, hydro, nullptr, nullptr, nullptr, nullptr)) return -1;
+    if (le_active(hydro->le) && lb_data_apply_le_boundary_conditions(lb, hydro->le)) return -1;
     if (lb_propagation(lb)) return -1;
   }
   return 0;

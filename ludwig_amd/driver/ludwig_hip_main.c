@@ -414,8 +414,28 @@ int main(int argc, char ** argv) {
   cs_nhalo_set(cs, symmetric ? 2 : 1);     /* ludwig.c:1183,1198 */
   cs_ntotal_set(cs, size);
   cs_init(cs);
+  /* leesedwards_rt.c:18-37, leesedwards.c:330-360 */
   lees_edw_options_t leo = {0};
+  leo.type = LE_SHEAR_TYPE_STEADY;
+  get_int("N_LE_plane", &leo.nplanes);
+  get_double("LE_plane_vel", &leo.uy);
+  if (get_int("LE_oscillation_period", &leo.period)) {
+    fatal("oscillatory Lees-Edwards shear is not supported by this build", NULL);
+  }
+  get_int("LE_time_offset", &leo.nt0);
   lees_edw_create(pe, cs, &leo, &le);
+  if (leo.nplanes > 0) {
+    double gammadot = 0.0;
+    lees_edw_shear_rate(le, &gammadot);
+    printf("\nLees-Edwards boundary conditions are active:\n");
+    for (int np = 0; np < leo.nplanes; np++) {
+      printf("LE plane %d is at x = %d with speed %f\n", np + 1, lees_edw_plane_location(le, np),
+	     leo.uy);
+    }
+    printf("Overall shear rate = %f\n", gammadot);
+    printf("\nLees-Edwards time offset (time steps): %8d\n", leo.nt0);
+  }
+  physics_control_init_time(phys, nstart, ncycles);
 
   lb_data_options_t lbo = lb_data_options_default();
   lbo.iodata = io_args_keys("lb");
@@ -525,6 +545,12 @@ int main(int argc, char ** argv) {
   }
   else {
     lb_init_rest_f(lb, rho0);
+  }
+  /* ludwig.c:305-312 */
+  if (nstart == 0 && switch_on("LE_init_profile")) {
+    if (leo.nplanes == 0) fatal("Cannot use LE_init_profile with no planes", NULL);
+    printf("Initialising shear profile\n");
+    lb_le_init_shear_profile(lb, le);
   }
 
   /* Restart (ludwig.c:313-364): distributions, phi, then rho and vel */

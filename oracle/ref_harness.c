@@ -25,6 +25,16 @@
  *                                        (ludwig.c:528-860 order)
  *    single   N steps seed tau           plain fluid, random-velocity
  *                                        equilibrium start
+ *    lebc     N seed                     lb_data_apply_le_boundary_conditions
+ *                                        on random post-collision f
+ *
+ *  Lees-Edwards planes: LDW_HARNESS_LE="nplanes uy" creates the planes
+ *  (steady shear, lees_edw_create leesedwards.c:81); LDW_HARNESS_T0=k
+ *  advances the physics time by k steps before a single-call case runs
+ *  (the plane displacement depends on it); LDW_HARNESS_LE_PROFILE=1 starts
+ *  the binary case from lb_le_init_shear_profile (model_le.c:649).  The
+ *  binary case applies lb_data_apply_le_boundary_conditions (model_le.c:78)
+ *  after each collision, as ludwig.c:810-812 does.
  *
  *  Reference functions used (file:line in the reference tree):
  *    lb_collide          collision.c:143
@@ -72,6 +82,7 @@
 #include "wall.h"
 #include "fe_force_method.h"
 #include "field_phi_init.h"
+#include "model_le.h"
 
 /* splitmix64: input generator for the harness only */
 
@@ -89,6 +100,15 @@ static double sm_uniform(void) {            /* [0, 1) with 53 bits */
 }
 
 static const char * outdir = ".";
+
+static void usage(void);
+
+/* LDW_HARNESS_T0=k: the physics time of a single-call case */
+static void advance_time(physics_t * phys) {
+  const char * t0 = getenv("LDW_HARNESS_T0");
+  int k = t0 ? atoi(t0) : 0;
+  for (int i = 0; i < k; i++) physics_control_next_step(phys);
+}
 
 static void dump(const char * name, const double * a, size_t n) {
   char path[4096];
@@ -144,6 +164,11 @@ static void sys_create(sys_t * s, int n, int nhalo, lb_relaxation_enum_t relax,
 
   {
     lees_edw_options_t opts = {0};
+    const char * le = getenv("LDW_HARNESS_LE");
+    if (le) {
+      opts.type = LE_SHEAR_TYPE_STEADY;
+      if (sscanf(le, "%d %lf", &opts.nplanes, &opts.uy) != 2) usage();
+    }
     lees_edw_create(s->pe, s->cs, &opts, &s->le);
   }
   {
@@ -391,6 +416,7 @@ int main(int argc, char ** argv) {
       double fzero[3] = {0};
       hydro_f_zero(s.hydro, fzero);
     }
+    advance_time(s.phys);
     field_halo(s.phi);
     field_grad_compute(s.dphi);
     phi_force_calculation(s.pe, s.cs, s.le, s.wall, s.pth, (fe_t *) s.fe,
@@ -421,6 +447,7 @@ int main(int argc, char ** argv) {
     double * u = random_field(&s, 3, 0.0, 0.02);
     set_field(&s, s.phi, 1, phi);
     set_field(&s, s.hydro->u, 3, u);
+    advance_time(s.phys);
     field_halo(s.phi);
     field_grad_compute(s.dphi);
     phi_cahn_hilliard(s.pch, (fe_t *) s.fe, s.phi, s.hydro, s.map, NULL);
@@ -449,6 +476,10 @@ int main(int argc, char ** argv) {
     field_phi_init_spinodal(s.phi, seed, phi0, amp);
     lb_init_rest_f(s.lb, 1.0);
     {
+      const char * pr = getenv("LDW_HARNESS_LE_PROFILE");
+      if (pr && atoi(pr) == 1) lb_le_init_shear_profile(s.lb, s.le);
+    }
+    {
       double * p0 = get_field(&s, s.phi, 1);
       dump("phi_init", p0, s.nsites);
       free(p0);
@@ -469,6 +500,7 @@ int main(int argc, char ** argv) {
       phi_cahn_hilliard(s.pch, (fe_t *) s.fe, s.phi, s.hydro, s.map, NULL);
       hydro_u_zero(s.hydro, uzero);
       lb_collide(s.lb, s.hydro, s.map, NULL, (fe_t *) s.fe, NULL);
+      lb_data_apply_le_boundary_conditions(s.lb, s.le);
       lb_halo(s.lb);
       lb_propagation(s.lb);
       if (dumpevery > 0 && (step % dumpevery) == 0 && step != nsteps) {
@@ -549,6 +581,19 @@ int main(int argc, char ** argv) {
     double * f = get_f(&s);
     dump("f", f, 19*(size_t) s.nsites);
     dump_hydro(&s, "");
+  }
+  else if (strcmp(which, "lebc") == 0) {
+    /* N seed */
+    if (argc < 5) usage();
+    int n = atoi(argv[3]);
+    sm_state = strtoull(argv[4], NULL, 10);
+    sys_create(&s, n, 2, LB_RELAXATION_M10, 0.1, 0, 0);
+    double * f0 = random_f(&s);
+    advance_time(s.phys);
+    lb_data_apply_le_boundary_conditions(s.lb, s.le);
+    double * f1 = get_f(&s);
+    dump("f_in", f0, 19*(size_t) s.nsites);
+    dump("f_out", f1, 19*(size_t) s.nsites);
   }
   else {
     usage();

@@ -95,6 +95,34 @@ CASES = {
     "binary16_c2_o2_s10": (["binary", 16, 10, 616, 2, A, B, K, MS, 0.8, -0.1, 0.2],
                            16, ["phi_init_sum", "phi", "rho", "u"], ["f"],
                            {"LDW_HARNESS_CONSERVE": "2", "OMP_NUM_THREADS": "1"}),
+    # Lees-Edwards planes (N_LE_plane 2, steady shear): single calls at
+    # time step 9 (buffer displacement 8 uy), the LB plane conditions, and
+    # the whole step loop with the planes, as ludwig.c runs it
+    "le_force16": (["force", 16, 21, 0, A, B, K], 16, ["phi", "grad", "delsq", "mu", "force"], [],
+                   {"LDW_HARNESS_LE": "2 0.37", "LDW_HARNESS_T0": "9"}),
+    "le_force16_g27": (["force", 16, 22, 0, A, B, K], 16, ["phi", "grad", "delsq", "mu", "force"], [],
+                       {"LDW_HARNESS_LE": "2 0.37", "LDW_HARNESS_T0": "9", "LDW_HARNESS_STENCIL": "27"}),
+    "le_ch16_o1": (["ch", 16, 23, 1, A, B, K, M], 16, ["phi_in", "u", "phi_out"], [],
+                   {"LDW_HARNESS_LE": "2 0.37", "LDW_HARNESS_T0": "9"}),
+    "le_ch16_o2": (["ch", 16, 24, 2, A, B, K, M], 16, ["phi_in", "u", "phi_out"], [],
+                   {"LDW_HARNESS_LE": "2 0.37", "LDW_HARNESS_T0": "9"}),
+    "le_ch16_o3": (["ch", 16, 25, 3, A, B, K, M], 16, ["phi_in", "u", "phi_out"], [],
+                   {"LDW_HARNESS_LE": "2 0.37", "LDW_HARNESS_T0": "9"}),
+    "le_lbbc16": (["lebc", 16, 26], 16, ["f_in", "f_out"], [],
+                  {"LDW_HARNESS_LE": "2 0.37", "LDW_HARNESS_T0": "9"}),
+    "le_binary16_o1_s20": (["binary", 16, 20, 8361235, 1, A, B, K, MS, 0.8, 0.0, 0.1], 16,
+                           ["phi", "rho", "u"], ["f"],
+                           {"LDW_HARNESS_LE": "2 0.05", "LDW_HARNESS_LE_PROFILE": "1"}),
+    "le_binary16_o3_g27_s10": (["binary", 16, 10, 4242, 3, A, B, K, MS, 1.0, 0.0, 0.1], 16,
+                               ["phi", "rho", "u"], ["f"],
+                               {"LDW_HARNESS_LE": "2 0.1", "LDW_HARNESS_STENCIL": "27"}),
+    "le_binary24_p3_o2_s12": (["binary", 24, 12, 77, 2, A, B, K, MS, 0.8, 0.1, 0.2], 24,
+                              ["phi", "rho", "u"], ["f"],
+                              {"LDW_HARNESS_LE": "3 0.2", "LDW_HARNESS_LE_PROFILE": "1"}),
+    "le_binary16_c1_s10": (["binary", 16, 10, 515, 1, A, B, K, MS, 1.0, 0.05, 0.1], 16,
+                           ["phi", "rho", "u", "csum"], ["f"],
+                           {"LDW_HARNESS_LE": "2 0.05", "LDW_HARNESS_CONSERVE": "1",
+                            "OMP_NUM_THREADS": "1"}),
 }
 
 NSAMPLE = 256
@@ -126,6 +154,12 @@ def main():
             if len(spec) > 4:
                 out["stencil"] = np.array(int(spec[4].get("LDW_HARNESS_STENCIL", 7)))
                 out["conserve"] = np.array(int(spec[4].get("LDW_HARNESS_CONSERVE", 0)))
+                if "LDW_HARNESS_LE" in spec[4]:
+                    nplane, uy = spec[4]["LDW_HARNESS_LE"].split()
+                    out["le_planes"] = np.array(int(nplane))
+                    out["le_uy"] = np.array(float(uy))
+                    out["le_t0"] = np.array(int(spec[4].get("LDW_HARNESS_T0", 0)))
+                    out["le_profile"] = np.array(int(spec[4].get("LDW_HARNESS_LE_PROFILE", 0)))
             out["sample_sites"] = samples
             for key in full:
                 out[key] = np.fromfile(os.path.join(tmp, key + ".f64"))

@@ -6,6 +6,11 @@ kept here as .ref files with their unchanged .inp inputs; serial-spin-fd1
 uses the 27-point gradient), token by token with the
 absolute tolerance 1e-12 of the reference's tests/awk-fp-diff.sh.
 
+Lees-Edwards planes: the reference's regression inputs
+tests/regression/d3q19-short/serial-le3d-st{1,2,3,5,6,7} (two planes of
+steady shear on 32^3 from the shear-profile start, advection orders 1-3,
+the 7- and 27-point gradients) and their committed logs.
+
 Restart: the reference's tests/regression/d3q19-io/iodrop-mpi1-io{1,2,3}
 run a droplet 0 -> 20 in one go, 0 -> 10 writing a configuration, and
 10 -> 20 from that configuration.  The driver runs the same chain; its logs
@@ -67,7 +72,9 @@ def run_driver(name, tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["serial-dist-3du", "serial-spin-fd1", "spin7-o1", "spin7-o3",
-                                  "c2-m025", "gradmu-trt", "single-bgk", "spin7-c1", "spin7-c2"])
+                                  "c2-m025", "gradmu-trt", "single-bgk", "spin7-c1", "spin7-c2",
+                                  "serial-le3d-st1", "serial-le3d-st2", "serial-le3d-st3",
+                                  "serial-le3d-st5", "serial-le3d-st6", "serial-le3d-st7"])
 def test_driver_matches_reference_log(name, tmp_path):
     got = run_driver(name, tmp_path)
     want = open(os.path.join(LOGS, name + ".ref")).read()

@@ -1,0 +1,156 @@
+"""Lees-Edwards sliding planes on the GPU against the reference.
+
+The golden vectors come from the reference's own build driven through its
+public API (oracle/ref_harness.c, tests/golden/make_golden.py) with
+N_LE_plane planes of steady shear: the gradient and the flux-form force at
+a time step where the planes have slid 2.96 sites, one Cahn-Hilliard update
+for advection orders 1-3, the LB plane conditions on random distributions,
+and whole step loops (shear-profile start, 27-point gradient, three planes,
+conserve 1).  The bar is bit-exact equality.
+
+At larger sizes, where no golden run exists, the tests check what the
+mathematics fixes: the plane conditions conserve mass and the Cahn-Hilliard
+update conserves phi (the face-flux averaging only redistributes), and a
+uniform fluid started on the steady shear profile stays on it.
+"""
+
+import hashlib
+
+import numpy as np
+import pytest
+
+import ludwig_amd as L
+
+pytestmark = pytest.mark.gpu
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def assert_exact(got, want, what):
+    got, want = np.asarray(got), np.asarray(want)
+    assert got.shape == want.shape, what
+    bad = np.flatnonzero(got != want)
+    if bad.size:
+        i = bad[0]
+        rel = np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-300))
+        pytest.fail(f"{what}: {bad.size}/{got.size} differ, first {i}: {got[i]!r} vs {want[i]!r}; "
+                    f"max rel {rel:.3e}")
+
+
+def _le(g):
+    return dict(le_planes=int(g["le_planes"]), le_uy=float(g["le_uy"]))
+
+
+@pytest.mark.parametrize("case,stencil", [("le_force16", 7), ("le_force16_g27", 27)])
+def test_le_force_golden(golden, case, stencil):
+    g = golden(case)
+    n = int(g["n"])
+    a, b_, k = map(float, [str(x) for x in g["args"]][4:7])
+    print(f"case {case}", flush=True)
+    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, tau=1.0, stencil=stencil, **_le(g))
+    b.set_phi(g["phi"])
+    b.next_step(int(g["le_t0"]))
+    b.f_zero()
+    b.force_stage()
+    _, _, F = b.hydro_fields()
+    assert_exact(F, g["force"], "force")
+    b.close()
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_le_cahn_hilliard_golden(golden, order):
+    g = golden(f"le_ch16_o{order}")
+    n = int(g["n"])
+    a, b_, k, m = map(float, [str(x) for x in g["args"]][4:8])
+    print(f"case le_ch16_o{order}", flush=True)
+    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, tau=1.0, order=order, **_le(g))
+    b.set_phi(g["phi_in"])
+    b.set_hydro(u=g["u"])
+    b.next_step(int(g["le_t0"]))
+    b.gradient_stage()
+    b.cahn_hilliard_stage()
+    assert_exact(b.phi_field(), g["phi_out"], "phi")
+    b.close()
+
+
+def test_le_lb_boundary_conditions_golden(golden):
+    g = golden("le_lbbc16")
+    n = int(g["n"])
+    print("case le_lbbc16", flush=True)
+    s = L.SingleFluid((n, n, n), tau=0.8, **_le(g))
+    s.set_f(g["f_in"])
+    s.next_step(int(g["le_t0"]))
+    s.le_boundary_conditions()
+    got = s.f()
+    assert_exact(got, g["f_out"], "f")
+    # only the plane-crossing populations of the two planes' sites change
+    changed = np.flatnonzero((got != g["f_in"]).reshape(19, n, -1).any(axis=2).any(axis=0))
+    assert set(changed) <= {3, 4, 11, 12}
+    s.close()
+
+
+@pytest.mark.parametrize("case", ["le_binary16_o1_s20", "le_binary16_o3_g27_s10",
+                                  "le_binary24_p3_o2_s12", "le_binary16_c1_s10"])
+def test_le_binary_step_golden(golden, case):
+    g = golden(case)
+    args = [str(a) for a in g["args"]]
+    n, nsteps, seed, order = int(args[1]), int(args[2]), int(args[3]), int(args[4])
+    a, b_, k, m, tau, phi0, amp = map(float, args[5:12])
+    print(f"case {case}", flush=True)
+    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, order=order, tau=tau,
+                      stencil=int(g["stencil"]), conserve=int(g["conserve"]), **_le(g))
+    b.init_spinodal(seed, phi0, amp)
+    if int(g["le_profile"]):
+        b.le_init_shear_profile()
+    b.step(nsteps)
+    assert_exact(b.phi_field(), g["phi"], "phi")
+    rho, u, _ = b.hydro_fields()
+    assert_exact(rho, g["rho"], "rho")
+    assert_exact(u, g["u"], "u")
+    assert sha(b.f()) == str(g["f_sha256"]), "f"
+    b.close()
+
+
+def test_le_conservation_and_shear_profile():
+    """64 x 32 x 32 with 4 planes: sums of rho and phi stay put to rounding
+    over 60 steps; with phi = 0 (no thermodynamic force) the steady shear
+    profile of lb_le_init_shear_profile is a fixed point of the dynamics:
+    u_y(x) stays linear between the planes, jumping by the plane speed."""
+    nx, ny, nz, uy = 64, 32, 32, 0.04
+    print("case le_conservation 64x32x32", flush=True)
+    b = L.BinaryFluid((nx, ny, nz), tau=0.8, mobility=0.15, le_planes=4, le_uy=uy)
+    b.init_spinodal(13, 0.0, 0.1)
+    b.le_init_shear_profile()
+    r0, p0 = b.conserved_sums()
+    b.step(60)
+    r1, p1 = b.conserved_sums()
+    assert abs(r1 - r0) < 1e-11 * nx * ny * nz
+    assert abs(p1 - p0) < 1e-11 * nx * ny * nz
+    b.close()
+
+    b = L.BinaryFluid((nx, ny, nz), tau=0.8, mobility=0.15, le_planes=4, le_uy=uy)
+    b.set_phi(np.zeros(nx * ny * nz))
+    b.init_rest()
+    b.le_init_shear_profile()
+    b.step(40)
+    _, u, _ = b.hydro_fields()
+    u = u.reshape(3, nx, ny, nz)
+    gammadot = uy * 4 / nx
+    x = np.arange(1, nx + 1) - 0.5
+    want = x * gammadot - uy * np.floor((nx / 8 + x) / (nx / 4))
+    got = u[1].mean(axis=(1, 2))
+    assert np.abs(got - want).max() < 1e-6, np.abs(got - want).max()
+    assert np.abs(u[1] - u[1].mean(axis=(1, 2), keepdims=True)).max() < 1e-9
+    assert np.abs(u[0]).max() < 1e-9 and np.abs(u[2]).max() < 1e-9
+    b.close()
+
+
+def test_le_errors():
+    """Planes the reference refuses are refused, and FP32 storage with
+    planes fails loudly."""
+    with pytest.raises(RuntimeError, match="Wall at domain boundary|divide"):
+        L.BinaryFluid((16, 16, 16), tau=1.0, le_planes=3, le_uy=0.1)
+    with pytest.raises(RuntimeError, match="FP64"):
+        L.BinaryFluid((32, 32, 32), tau=1.0, le_planes=2, le_uy=0.1, precision=32)
