@@ -18,6 +18,7 @@ Prints one JSON line:
                compiled from the reference sources) on the host cores, or the
                CPU oracle when that binary is absent
   extra        256^3 FP64 and FP32-shifted throughput, per-kernel times; the
+               binary fluid at 128^3 with two Lees-Edwards planes; the
                256-replica uneven-size ensemble (FP32) throughput; the
                surrogate U-Net's BF16 MFMA inference rate (TFLOP/s)
 
@@ -122,6 +123,54 @@ def run_ensemble(nrep, steps, warmup, precision=32, seed=2026):
             "hbm_frac_step": mlups * 1e6 * BYTES_SITE[precision]["step"] / (HBM_PEAK_GBS * 1e9),
             "tuned": cfg, "tuner_trials": rows.tolist(), "init_s": t_init,
             "mass_first_last": mass}
+
+
+# Lees-Edwards path (FP64, one kernel per stage): algorithmic bytes per site
+# of each kernel; the plane-only kernels (faces, sums, LB plane conditions)
+# touch O(N^2) sites and are left out
+LE_BYTES_SITE = {"le_gradient": 40,        # phi in; grad (3) and delsq out
+                 "le_force": 64,           # phi, grad, delsq in; force out
+                 "le_cahn_hilliard": 48,   # phi, delsq, u in; phi out
+                 "collide": 360}           # f in and out, force in, rho and u out
+
+
+def run_le(n=128, steps=50, warmup=5, nplanes=2, uy=0.05):
+    """The binary fluid with Lees-Edwards planes (steady shear from the
+    shear-profile start): whole-step MLUPS and each kernel's HBM rate."""
+    import ctypes
+    import ludwig_amd as L
+    b = L.BinaryFluid((n, n, n), fatal=True, le_planes=nplanes, le_uy=uy, **C2)
+    b.init_spinodal(SEED, 0.0, 0.1)
+    b.le_init_shear_profile()
+    lib = L.lib()
+    b.step(warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    lib.ldw_timer_reset()
+    lib.ldw_timer_enable(1)
+    b.step(min(steps, 20))
+    b.synchronize()
+    lib.ldw_timer_enable(0)
+    kt, rate = {}, {}
+    for name in ("le_gradient", "le_force", "le_cahn_hilliard", "collide", "le_lb_bc"):
+        ms, cnt = ctypes.c_double(), ctypes.c_int()
+        lib.ldw_timer_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+        if cnt.value:
+            kt[name] = ms.value / cnt.value
+            if name in LE_BYTES_SITE:
+                rate[name] = LE_BYTES_SITE[name] * n ** 3 / (kt[name] * 1e-3) / 1e9
+    r, _ = b.conserved_sums()
+    b.close()
+    mlups = n ** 3 * steps / wall / 1e6
+    return {"n": n, "planes": nplanes, "plane_speed": uy, "steps": steps,
+            "ms_per_step": 1e3 * wall / steps, "mlups": mlups,
+            "bytes_per_site_algorithmic": sum(LE_BYTES_SITE.values()),
+            "hbm_frac_step": mlups * 1e6 * sum(LE_BYTES_SITE.values()) / (HBM_PEAK_GBS * 1e9),
+            "kernel_ms": kt, "kernel_GBps": rate,
+            "mass_ok": bool(abs(r - n ** 3) / n ** 3 < 1e-6)}
 
 
 BF16_PEAK_TFLOPS = 2500.0      # MI355X dense BF16 MFMA
@@ -233,6 +282,8 @@ def main():
                     help="only run the plain copy kernel (1 GiB, 20 launches)")
     ap.add_argument("--profile-ensemble", type=int, default=0,
                     help="only run this many steps of the 256-replica ensemble (for rocprofv3)")
+    ap.add_argument("--profile-le", type=int, default=0,
+                    help="only run this many steps of the Lees-Edwards workload (for rocprofv3)")
     ap.add_argument("--profile-surrogate", type=int, default=0,
                     help="only run this many U-Net inferences on 8 x 64^3 (for rocprofv3)")
     args = ap.parse_args()
@@ -243,6 +294,10 @@ def main():
         e.step(args.profile_ensemble)
         e.close()
         print(f"profile run: ensemble of 256 replicas, {args.profile_ensemble} steps")
+        return
+    if args.profile_le:
+        r = run_le(steps=args.profile_le, warmup=1)
+        print(f"profile run: lees-edwards {r}")
         return
     if args.profile_surrogate:
         r = run_surrogate_conv(reps=args.profile_surrogate)
@@ -283,6 +338,9 @@ def main():
                                                           "kernel_ms", "mass_ok")}
             print(f"# extra {n} fp{prec}: {r}", file=sys.stderr, flush=True)
 
+        r = run_le()
+        extra["lees_edwards_128^3_2planes_fp64"] = r
+        print(f"# lees-edwards: {r}", file=sys.stderr, flush=True)
         r = run_ensemble(256, max(20, args.steps // 4), 5)
         extra["ensemble_256_replicas_fp32"] = r
         print(f"# ensemble: {r}", file=sys.stderr, flush=True)

@@ -34,7 +34,8 @@
 // contraction, the results are the reference's bits.  The per-plane
 // correction of the force is a sum in site order, which one thread of a
 // block per plane and component forms serially, as the reference's loop
-// does.
+// does; those blocks run inside the launch that forms the force off the
+// planes, and a small launch then forms the force at the plane sites.
 
 #include <hip/hip_runtime.h>
 
@@ -257,38 +258,54 @@ le_force_faces_kernel(Geom g, LeGeom le, FeParams fe, GradFields F, double * __r
 }
 
 // fbar_a = sum over the plane, in site order, of -fluxe + fluxw
-// (phi_force.c:470-500); one block per plane and component
-constexpr int PS_TILE = 2048;
+// (phi_force.c:470-500), by one block per plane and component: each
+// addition needs the previous one, so thread 0 adds alone, reading 16
+// values from LDS ahead of the 16 dependent additions, while waves 1-3
+// stage the next tile (double-buffered).
+constexpr int PS_TILE = 1024;
 
-__global__ void __launch_bounds__(LB)
-le_plane_sum_kernel(const double * __restrict__ xface, int64_t nyz, double * __restrict__ fcor) {
-  __shared__ double buf[PS_TILE];
-  const int p = blockIdx.x/3, a = blockIdx.x%3;
+__device__ __forceinline__ void plane_sum_block(const double * __restrict__ xface, int64_t nyz,
+						double * __restrict__ fcor, int blk) {
+  __shared__ double buf[2][PS_TILE];
+  const int p = blk/3, a = blk%3;
   const double * fe = xface + ((int64_t) (2*p + 0)*3 + a)*nyz;
   const double * fw = xface + ((int64_t) (2*p + 1)*3 + a)*nyz;
+  const int tid = threadIdx.x;
+  const int64_t ntile = (nyz + PS_TILE - 1)/PS_TILE;
+  for (int k = tid; k < PS_TILE; k += blockDim.x) buf[0][k] = (k < nyz) ? - fe[k] + fw[k] : 0.0;
+  __syncthreads();
   double s = 0.0;
-  for (int64_t t0 = 0; t0 < nyz; t0 += PS_TILE) {
-    const int m = (int) ((nyz - t0 < PS_TILE) ? nyz - t0 : PS_TILE);
-    for (int k = threadIdx.x; k < m; k += blockDim.x) buf[k] = - fe[t0 + k] + fw[t0 + k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int k = 0; k < m; k++) s += buf[k];
+  for (int64_t t = 0; t < ntile; t++) {
+    const int cur = (int) (t & 1);
+    if (tid == 0) {
+      const int m = (int) ((nyz - t*PS_TILE < PS_TILE) ? nyz - t*PS_TILE : PS_TILE);
+      int k = 0;
+      for (; k + 16 <= m; k += 16) {
+	double v[16];
+#pragma unroll
+	for (int q = 0; q < 16; q++) v[q] = buf[cur][k + q];
+#pragma unroll
+	for (int q = 0; q < 16; q++) s += v[q];
+      }
+      for (; k < m; k++) s += buf[cur][k];
+    }
+    else if (tid >= 64 && t + 1 < ntile) {
+      const int64_t base = (t + 1)*PS_TILE;
+      for (int k = tid - 64; k < PS_TILE; k += blockDim.x - 64) {
+	buf[1 - cur][k] = (base + k < nyz) ? - fe[base + k] + fw[base + k] : 0.0;
+      }
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) fcor[3*p + a] = s;
+  if (tid == 0) fcor[3*p + a] = s;
 }
 
-// force = -(fluxe - fluxw + fluxy - fluxy(y-1) + fluxz - fluxz(z-1)),
-// the plane fluxes corrected by +/- ra fcor (phi_force.c:412-448, 502-520)
-__global__ void __launch_bounds__(LB)
-le_force_kernel(Geom g, LeGeom le, FeParams fe, GradFields F, const double * __restrict__ fcor,
-		double ra, double * __restrict__ force, int force_mode, double fz0, double fz1,
-		double fz2) {
-  const int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x;
-  if (i >= g.nsite) return;
-  const Coord c = coord_of(i, g.ny, g.nz);
-  const int x = c.v, y = c.y, z = c.z;
+// force = -(fluxe - fluxw + fluxy - fluxy(y-1) + fluxz - fluxz(z-1)) at
+// real site (x, y, z); at the plane sites fluxe (x = L) or fluxw
+// (x = L + 1) carry the correction +/- ra fcor (phi_force.c:412-448, 502-520)
+__device__ __forceinline__ void force_at(const Geom & g, const LeGeom & le, const FeParams & fe,
+					 const GradFields & F, const double * __restrict__ fcor,
+					 double ra, int x, int y, int z, double fo[3]) {
   const int vm = xnb(le, g.nx, x, -1), vp = xnb(le, g.nx, x, +1);
   double s0x[3], sxm[3], sxp[3], s0y[3], syp[3], sym[3], s0z[3], szp[3], szm[3];
   stress_col(fe, F, x, y, z, 0, s0x);
@@ -301,7 +318,6 @@ le_force_kernel(Geom g, LeGeom le, FeParams fe, GradFields F, const double * __r
   stress_col(fe, F, x, y, z + 1, 2, szp);
   stress_col(fe, F, x, y, z - 1, 2, szm);
   const int ic = x + 1;
-  double fo[3];
   for (int a = 0; a < 3; a++) {
     double fe_ = 0.5*(sxp[a] + s0x[a]);
     double fw = 0.5*(sxm[a] + s0x[a]);
@@ -309,22 +325,73 @@ le_force_kernel(Geom g, LeGeom le, FeParams fe, GradFields F, const double * __r
     const double fym = 0.5*(s0y[a] + sym[a]);
     const double fz = 0.5*(szp[a] + s0z[a]);
     const double fzm = 0.5*(s0z[a] + szm[a]);
-    for (int p = 0; p < le.nplane; p++) {
-      if (ic == le.loc[p]) fe_ += ra*fcor[3*p + a];
-      if (ic == le.loc[p] + 1) fw -= ra*fcor[3*p + a];
+    if (fcor != nullptr) {
+      for (int p = 0; p < le.nplane; p++) {
+	if (ic == le.loc[p]) fe_ += ra*fcor[3*p + a];
+	if (ic == le.loc[p] + 1) fw -= ra*fcor[3*p + a];
+      }
     }
     fo[a] = -(+ fe_ - fw + fy - fym + fz - fzm);
   }
+}
+
+__device__ __forceinline__ void force_store(double * __restrict__ force, int64_t nsite, int64_t i,
+					    int force_mode, const double fo[3], double fz0, double fz1,
+					    double fz2) {
   if (force_mode == 0) {
     force[i] = fz0 + fo[0];
-    force[g.nsite + i] = fz1 + fo[1];
-    force[2*g.nsite + i] = fz2 + fo[2];
+    force[nsite + i] = fz1 + fo[1];
+    force[2*nsite + i] = fz2 + fo[2];
   }
   else {
     force[i] += fo[0];
-    force[g.nsite + i] += fo[1];
-    force[2*g.nsite + i] += fo[2];
+    force[nsite + i] += fo[1];
+    force[2*nsite + i] += fo[2];
   }
+}
+
+__device__ __forceinline__ bool on_plane(const LeGeom & le, int x) {
+  for (int p = 0; p < le.nplane; p++) {
+    if (x == le.loc[p] - 1 || x == le.loc[p]) return true;
+  }
+  return false;
+}
+
+// The first 3 nplane blocks form the plane sums; the others the force at
+// every site off the planes, which needs no correction, concurrently.
+__global__ void __launch_bounds__(LB)
+le_force_kernel(Geom g, LeGeom le, FeParams fe, GradFields F, const double * __restrict__ xface,
+		double * __restrict__ fcor, double * __restrict__ force, int force_mode,
+		double fz0, double fz1, double fz2) {
+  const int nsum = 3*le.nplane;
+  if ((int) blockIdx.x < nsum) {
+    plane_sum_block(xface, (int64_t) g.ny*g.nz, fcor, blockIdx.x);
+    return;
+  }
+  const int64_t i = (int64_t) (blockIdx.x - nsum)*blockDim.x + threadIdx.x;
+  if (i >= g.nsite) return;
+  const Coord c = coord_of(i, g.ny, g.nz);
+  if (on_plane(le, c.v)) return;
+  double fo[3];
+  force_at(g, le, fe, F, nullptr, 0.0, c.v, c.y, c.z, fo);
+  force_store(force, g.nsite, i, force_mode, fo, fz0, fz1, fz2);
+}
+
+// the sites either side of each plane, once the sums are known
+__global__ void __launch_bounds__(LB)
+le_plane_force_kernel(Geom g, LeGeom le, FeParams fe, GradFields F, const double * __restrict__ fcor,
+		      double ra, double * __restrict__ force, int force_mode, double fz0, double fz1,
+		      double fz2) {
+  const int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x;
+  const int64_t nyz = (int64_t) g.ny*g.nz;
+  if (i >= (int64_t) le.nplane*2*nyz) return;
+  const int ps = (int) (i/nyz);
+  const int rem = (int) (i - (int64_t) ps*nyz);
+  const int x = le.loc[ps/2] - 1 + ps%2;
+  const int y = rem/g.nz, z = rem - (rem/g.nz)*g.nz;
+  double fo[3];
+  force_at(g, le, fe, F, fcor, ra, x, y, z, fo);
+  force_store(force, g.nsite, (int64_t) x*nyz + rem, force_mode, fo, fz0, fz1, fz2);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -627,8 +694,9 @@ void launch_le_force(const Geom & g, const LeGeom & le, const FeParams & fe,
   const int64_t nyz = (int64_t) g.ny*g.nz;
   const int64_t nf = (int64_t) le.nplane*2*nyz;
   hipLaunchKernelGGL(le_force_faces_kernel, dim3(nblk(nf)), dim3(LB), 0, s, g, le, fe, F, xface);
-  hipLaunchKernelGGL(le_plane_sum_kernel, dim3(3*le.nplane), dim3(LB), 0, s, xface, nyz, fcor);
-  hipLaunchKernelGGL(le_force_kernel, dim3(nblk(g.nsite)), dim3(LB), 0, s, g, le, fe, F, fcor, ra,
+  hipLaunchKernelGGL(le_force_kernel, dim3(3*le.nplane + nblk(g.nsite)), dim3(LB), 0, s, g, le, fe,
+		     F, xface, fcor, force, force_mode, fzero[0], fzero[1], fzero[2]);
+  hipLaunchKernelGGL(le_plane_force_kernel, dim3(nblk(nf)), dim3(LB), 0, s, g, le, fe, F, fcor, ra,
 		     force, force_mode, fzero[0], fzero[1], fzero[2]);
 }
 
