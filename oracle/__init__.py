@@ -38,6 +38,11 @@ class FeParam(ctypes.Structure):
                 ("stencil", ctypes.c_int), ("conserve", ctypes.c_int)]
 
 
+class LeParam(ctypes.Structure):
+    _fields_ = [("nplane", ctypes.c_int), ("nhalo", ctypes.c_int),
+                ("uy", ctypes.c_double), ("nt0", ctypes.c_int)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -51,6 +56,12 @@ def lib():
         _lib.ora_phi_init_sum.restype = ctypes.c_double
         _lib.ora_phi_init_sum.argtypes = [ctypes.c_int64, ctypes.c_void_p]
         _lib.ora_klein_sum.argtypes = [ctypes.c_int64, ctypes.c_void_p]
+        vp, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+        _lib.ora_le_force.argtypes = [i, i, i, vp, vp, d, vp, vp]
+        _lib.ora_le_cahn_hilliard.argtypes = [i, i, i, vp, vp, d, vp, vp, vp, d]
+        _lib.ora_le_lb_bc.argtypes = [i, i, i, vp, d, vp]
+        _lib.ora_le_shear_profile.argtypes = [i, i, i, vp, d, d, vp]
+        _lib.ora_le_binary_steps.argtypes = [i, i, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, d]
     return _lib
 
 
@@ -292,3 +303,61 @@ def conv3_bwd_weight(x, dy, shape, cin, cout, batch=1):
     lib().ora_conv3_bwd_weight(_p(x), _p(dy), _p(dw), *_conv_args(batch, nx, ny, nz, cin, cout))
     return dw
 
+
+
+# Lees-Edwards planes (le_oracle.c)
+
+def _le_ck(rc):
+    if rc != 0:
+        raise ValueError("plane geometry refused (planes must divide nx and sit more "
+                         "than nhalo sites from the x boundaries)")
+
+
+def le_force(shape, fe, le, t, phi):
+    """Gradient and flux-form force at physics time t; returns F (3 planes)."""
+    nx, ny, nz = shape
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    F = np.zeros(3 * phi.size)
+    _le_ck(lib().ora_le_force(nx, ny, nz, ctypes.byref(fe), ctypes.byref(le), t, _p(phi), _p(F)))
+    return F
+
+
+def le_cahn_hilliard(shape, fe, le, t, phi, u=None, csum=None, phi_init_sum=0.0):
+    """Gradient and one Cahn-Hilliard update at time t; returns (phi, csum)."""
+    nx, ny, nz = shape
+    out = np.ascontiguousarray(phi, dtype=np.float64).copy()
+    cs = np.zeros(out.size) if csum is None else np.ascontiguousarray(csum, dtype=np.float64).copy()
+    uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
+    _le_ck(lib().ora_le_cahn_hilliard(nx, ny, nz, ctypes.byref(fe), ctypes.byref(le), t,
+                                      _p(out), _p(uu), _p(cs), phi_init_sum))
+    return out, cs
+
+
+def le_lb_bc(shape, le, timestep, f):
+    nx, ny, nz = shape
+    f = np.ascontiguousarray(f, dtype=np.float64).copy()
+    _le_ck(lib().ora_le_lb_bc(nx, ny, nz, ctypes.byref(le), float(timestep), _p(f)))
+    return f
+
+
+def le_shear_profile(shape, le, rho0, eta):
+    nx, ny, nz = shape
+    f = np.zeros(19 * nx * ny * nz)
+    _le_ck(lib().ora_le_shear_profile(nx, ny, nz, ctypes.byref(le), rho0, eta, _p(f)))
+    return f
+
+
+def le_binary_steps(shape, nsteps, lb, fe, le, f, phi, rho=None, u=None, csum=None,
+                    phi_init_sum=0.0, t_start=0, t_current=0):
+    """nsteps of the step loop with planes; returns f, phi, rho, u, csum."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    f = np.ascontiguousarray(f, dtype=np.float64).copy()
+    phi = np.ascontiguousarray(phi, dtype=np.float64).copy()
+    rho = np.ones(n) if rho is None else rho.copy()
+    u = np.zeros(3 * n) if u is None else u.copy()
+    cs = np.zeros(n) if csum is None else np.ascontiguousarray(csum, dtype=np.float64).copy()
+    _le_ck(lib().ora_le_binary_steps(nx, ny, nz, nsteps, t_start, t_current, ctypes.byref(lb),
+                                     ctypes.byref(fe), ctypes.byref(le), _p(f), _p(phi), _p(rho),
+                                     _p(u), _p(cs), phi_init_sum))
+    return f, phi, rho, u, cs

@@ -149,6 +149,43 @@ void ora_random_velocity(int nx, int ny, int nz, int seed, double amp,
 double ora_kahan_sum(int64_t n, const double * a);
 double ora_klein_sum(int64_t n, const double * a);
 
+/* ------------------------------------------------------------------------ */
+/* Lees-Edwards planes (le_oracle.c), steady shear, one rank                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct ora_le_param_s {
+  int    nplane;           /* N_LE_plane */
+  int    nhalo;            /* coordinate halo (2 for the symmetric free energy) */
+  double uy;               /* LE_plane_vel */
+  int    nt0;              /* LE_time_offset */
+} ora_le_param_t;
+
+/* field_grad_compute + phi_force_calculation at physics time t (the
+ * flux form, phi_force.c:289-520): F = 0 + force.  Returns -1 for a plane
+ * geometry the reference refuses. */
+int ora_le_force(int nx, int ny, int nz, const ora_fe_param_t * fe, const ora_le_param_t * le,
+		 double t, const double * phi, double * F);
+
+/* field_grad_compute + phi_cahn_hilliard at time t; phi updated in place;
+ * u may be NULL (no hydrodynamics) */
+int ora_le_cahn_hilliard(int nx, int ny, int nz, const ora_fe_param_t * fe,
+			 const ora_le_param_t * le, double t, double * phi, const double * u,
+			 double * csum, double phi_init_sum);
+
+/* lb_data_apply_le_boundary_conditions at time step timestep, in place */
+int ora_le_lb_bc(int nx, int ny, int nz, const ora_le_param_t * le, double timestep, double * f);
+
+/* lb_le_init_shear_profile: f for rho0 and the shear viscosity eta */
+int ora_le_shear_profile(int nx, int ny, int nz, const ora_le_param_t * le, double rho0,
+			 double eta, double * f);
+
+/* nsteps of the ludwig.c loop with planes, from the time counter state
+ * (t_start, t_current) of physics_control */
+int ora_le_binary_steps(int nx, int ny, int nz, int nsteps, int t_start, int t_current,
+			const ora_lb_param_t * lb, const ora_fe_param_t * fe,
+			const ora_le_param_t * le, double * f, double * phi, double * rho,
+			double * u, double * csum, double phi_init_sum);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,8 +8,10 @@ for advection orders 1-3, the LB plane conditions on random distributions,
 and whole step loops (shear-profile start, 27-point gradient, three planes,
 conserve 1).  The bar is bit-exact equality.
 
-At larger sizes, where no golden run exists, the tests check what the
-mathematics fixes: the plane conditions conserve mass and the Cahn-Hilliard
+On non-cubic lattices with up to four planes the GPU step loop matches
+the CPU oracle's restatement of the plane code (oracle/le_oracle.c, itself
+pinned against the golden vectors) bit for bit.  At larger sizes the tests
+check what the mathematics fixes: the plane conditions conserve mass and the Cahn-Hilliard
 update conserves phi (the face-flux averaging only redistributes), and a
 uniform fluid started on the steady shear profile stays on it.
 """
@@ -110,6 +112,41 @@ def test_le_binary_step_golden(golden, case):
     assert_exact(rho, g["rho"], "rho")
     assert_exact(u, g["u"], "u")
     assert sha(b.f()) == str(g["f_sha256"]), "f"
+    b.close()
+
+
+@pytest.mark.parametrize("shape,planes,uy,order,stencil,conserve,profile,nsteps", [
+    ((48, 20, 12), 4, 0.3, 3, 27, 0, True, 15),
+    ((40, 24, 16), 2, 0.71, 2, 7, 1, False, 12),
+    ((36, 18, 10), 3, 0.05, 1, 7, 2, True, 10),
+])
+def test_le_binary_vs_oracle(shape, planes, uy, order, stencil, conserve, profile, nsteps):
+    """Non-cubic lattices, up to four planes, displacements that wrap in y,
+    conserve 1 and 2: the GPU step loop against the CPU oracle's
+    (oracle/le_oracle.c), bit for bit."""
+    import oracle as O
+    print(f"case {shape} planes={planes} uy={uy} order={order} stencil={stencil} "
+          f"conserve={conserve}", flush=True)
+    p = dict(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25)
+    tau = 0.9
+    b = L.BinaryFluid(shape, tau=tau, order=order, stencil=stencil, conserve=conserve,
+                      le_planes=planes, le_uy=uy, **p)
+    b.init_spinodal(2027, 0.02, 0.1)
+    if profile:
+        b.le_init_shear_profile()
+    b.step(nsteps)
+    le = O.LeParam(planes, 2, uy, 0)
+    fe = O.fe_param(order=order, stencil=stencil, conserve=conserve, **p)
+    phi0 = O.spinodal(shape, 2027, 0.02, 0.1)
+    f0 = (O.le_shear_profile(shape, le, 1.0, (tau - 0.5) / 3.0) if profile
+          else O.equilibrium(shape, rho_uniform=1.0))
+    f, phi, rho, u, _ = O.le_binary_steps(shape, nsteps, O.lb_param(tau=tau), fe, le, f0, phi0,
+                                          phi_init_sum=O.phi_init_sum(phi0))
+    assert_exact(b.phi_field(), phi, "phi")
+    r, uu, _ = b.hydro_fields()
+    assert_exact(r, rho, "rho")
+    assert_exact(uu, u, "u")
+    assert_exact(b.f(), f, "f")
     b.close()
 
 
