@@ -3,8 +3,10 @@
 
 #include "io_format.h"
 
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <utility>
 
@@ -167,10 +169,20 @@ struct Obj {
   void add(const char * k, const std::string & v) { items.emplace_back(k, v); }
 };
 
+// cJSON's print_number (util_cJSON.c:549-583): an integral value as %d,
+// else %1.15g unless that does not read back to within one ulp-scale
+// (compare_double), then %1.17g
 std::string num(double v) {
   char buf[64];
-  if (v == (double) (int) v) snprintf(buf, sizeof(buf), "%d", (int) v);
-  else snprintf(buf, sizeof(buf), "%1.15g", v);
+  const int vi = (v >= 2147483647.0) ? 2147483647 : ((v <= -2147483648.0) ? (-2147483647 - 1) : (int) v);
+  if (std::isnan(v) || std::isinf(v)) snprintf(buf, sizeof(buf), "null");
+  else if ((v - (double) vi) == 0.0) snprintf(buf, sizeof(buf), "%d", vi);
+  else {
+    snprintf(buf, sizeof(buf), "%1.15g", v);
+    double test = 0.0;
+    const double mx = (fabs(test = strtod(buf, nullptr)) > fabs(v)) ? fabs(test) : fabs(v);
+    if (!(fabs(test - v) <= mx*DBL_EPSILON)) snprintf(buf, sizeof(buf), "%1.17g", v);
+  }
   return buf;
 }
 
@@ -195,7 +207,8 @@ std::string render(const Obj & o, int depth) {
 
 }  // namespace
 
-std::string metadata_json(const Layout & l, const io_options_t & opts, const double lmin[3]) {
+std::string metadata_json(const Layout & l, const io_options_t & opts, const double lmin[3],
+			  const lees_edw_options_t & leo) {
   const int one[3] = {1, 1, 1};
   const int zero[3] = {0, 0, 0};
   const bool ascii = (opts.iorformat == IO_RECORD_ASCII);
@@ -204,8 +217,16 @@ std::string metadata_json(const Layout & l, const io_options_t & opts, const dou
   options.add("System size (total)", array3(l.ntotal));
   options.add("Periodic boundaries", array3(one));
   options.add("Left-end limit Lmin", array3(lmin));
-  Obj le;                             // lees_edw_opts_to_json, no planes
-  le.add("Number of planes", num(0));
+  Obj le;                             // lees_edw_opts_to_json (lees_edwards_options.c:65-100)
+  le.add("Number of planes", num(leo.nplanes));
+  if (leo.nplanes > 0) {
+    const char * type = (leo.type == LE_SHEAR_TYPE_STEADY) ? "STEADY"
+      : ((leo.type == LE_SHEAR_TYPE_OSCILLATORY) ? "OSCILLATORY" : "INVALID");
+    le.add("Shear type", str(type));
+    if (leo.type == LE_SHEAR_TYPE_OSCILLATORY) le.add("Period (timesteps)", num(leo.period));
+    le.add("Reference time", num(leo.nt0));
+    le.add("Plane speed", num(leo.uy));
+  }
   Obj coords;
   coords.add("options", render(options, 3));
   coords.add("lees_edwards", render(le, 3));

@@ -47,7 +47,8 @@ int read_data(const std::string & path, const Layout & l, io_record_format_enum_
 	      double * rec, std::string & err);
 
 // The metadata document for one quantity (options of the output side)
-std::string metadata_json(const Layout & l, const io_options_t & opts, const double lmin[3]);
+std::string metadata_json(const Layout & l, const io_options_t & opts, const double lmin[3],
+			  const lees_edw_options_t & leo);
 int write_text(const std::string & path, const std::string & text, std::string & err);
 
 // The I/O decomposition and format are ones this library writes

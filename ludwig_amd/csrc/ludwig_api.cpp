@@ -273,6 +273,7 @@ struct cs_s {
   int init;
   int str[3];
   int nsites;
+  lees_edw_options_t leopts;   // copy for the I/O metadata (leesedwards.c:99)
 };
 
 extern "C" int cs_create(pe_t * pe, cs_t ** pcs) {
@@ -716,6 +717,7 @@ extern "C" int lees_edw_create(pe_t *, cs_t * cs, const lees_edw_options_t * opt
   lees_edw_t * le = (lees_edw_t *) calloc(1, sizeof(lees_edw_t));
   *ple = le;
   le->cs = cs;
+  if (opts && cs) cs->leopts = *opts;
   if (opts == nullptr || opts->nplanes == 0) return 0;
   if (!cs || !cs->init) FAIL("lees_edw_create: the coordinate system must be initialised first");
   le->opts = *opts;
@@ -2269,7 +2271,7 @@ static int io_write(const char * who, const char * stub, int timestep, const io_
   if (!meta_written) {
     double lmin[3];
     cs_lmin(cs, lmin);
-    if (ldw::io::write_text(ldw::io::metadata_name(stub), ldw::io::metadata_json(l, o, lmin), err)) {
+    if (ldw::io::write_text(ldw::io::metadata_name(stub), ldw::io::metadata_json(l, o, lmin, cs->leopts), err)) {
       FAIL("%s: %s", who, err.c_str());
     }
     meta_written = 1;

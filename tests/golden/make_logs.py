@@ -35,6 +35,9 @@ CHAINS = {
     "iodrop": ["iodrop-mpi1-io1", "iodrop-mpi1-io2", "iodrop-mpi1-io3"],
     # the same restart with ascii records
     "iodrop-ascii": ["iodrop-ascii-io2", "iodrop-ascii-io3"],
+    # two Lees-Edwards planes (serial-le3d-st1 with advection order 3): the
+    # metadata records the planes, and the restarted time counter
+    "le-restart": ["le-io1", "le-io2", "le-io3"],
 }
 
 
