@@ -806,29 +806,28 @@ static double le_buffer_displacement(const lees_edw_t * le, int ib, double t) {
 /* the split of each buffer plane's displacement for field_leesedwards
  * (field.c:470-475: t0 = 0, fr = 1 - fraction) or hydro_lees_edwards
  * (hydro.c:398-403: t0 = 1, fr = fraction) */
-/* integer part of a displacement already reduced modulo ly: in [-ly, ly]
- * for any finite input, which the kernels' periodic indexing relies on */
-static int le_floor(double dy, double ly) {
-  if (!(fabs(dy) <= ly)) {
-    fprintf(stderr, "libludwig_hip: fatal: Lees-Edwards displacement %g outside [-%g, %g]\n", dy, ly, ly);
-    abort();
-  }
-  return (int) floor(dy);
+/* integer part of a displacement already reduced modulo ly; it lies in
+ * [-ly, ly] for any finite input, which the kernels' periodic indexing
+ * relies on, so anything else (a non-finite time or speed) is refused */
+static int le_floor(double dy, double ly, int * jdy) {
+  if (!(fabs(dy) <= ly)) FAIL("Lees-Edwards displacement %g outside [-%g, %g]", dy, ly, ly);
+  *jdy = (int) floor(dy);
+  return 0;
 }
 
-static ldw::LeShift le_buffer_shift(const lees_edw_t * le, double t0, int one_minus) {
-  ldw::LeShift sh;
-  memset(&sh, 0, sizeof(sh));
+static int le_buffer_shift(const lees_edw_t * le, double t0, int one_minus, ldw::LeShift * sh) {
+  memset(sh, 0, sizeof(*sh));
   const double ly = (double) le->n[1];
   const double t = phys_time();
   for (int ib = 0; ib < le->geo.nxbuf; ib++) {
     double dy = le_buffer_displacement(le, ib, t + t0);
     dy = fmod(dy, ly);
-    const int jdy = le_floor(dy, ly);
-    sh.jdy[ib] = jdy;
-    sh.fr[ib] = one_minus ? 1.0 - (dy - jdy) : dy - jdy;
+    int jdy = 0;
+    if (le_floor(dy, ly, &jdy)) return -1;
+    sh->jdy[ib] = jdy;
+    sh->fr[ib] = one_minus ? 1.0 - (dy - jdy) : dy - jdy;
   }
-  return sh;
+  return 0;
 }
 
 struct map_s { int ndata; };
@@ -1065,7 +1064,8 @@ extern "C" int field_grad_compute(field_grad_t * g) {
     // field_leesedwards() then the gradient (field_grad.c:319-325): the
     // buffer planes at the current time, then real and buffer sites
     if (flush_force() || materialise_derived()) return -1;
-    const ldw::LeShift sh = le_buffer_shift(f->le, 0.0, 1);
+    ldw::LeShift sh;
+    if (le_buffer_shift(f->le, 0.0, 1, &sh)) return -1;
     KTimer t("le_gradient");
     ldw::launch_le_phi_buffer(f->geo, f->le->geo, sh, (const double *) f->d_data[f->cur],
 			      f->d_phib, g_stream);
@@ -1708,7 +1708,8 @@ extern "C" int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi, hydro
       if (hydro_materialise(hydro)) return -1;
       u = (const double *) hydro->d_u;
       ub = hydro->d_ub;
-      const ldw::LeShift sh = le_buffer_shift(le, 1.0, 0);
+      ldw::LeShift sh;
+      if (le_buffer_shift(le, 1.0, 0, &sh)) return -1;
       ldw::launch_le_u_buffer(g, le->geo, sh, le->opts.uy, u, ub, g_stream);
       CHECK_LAUNCH();
     }
@@ -1716,7 +1717,8 @@ extern "C" int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi, hydro
     const double dy = phys_time()*le->opts.uy;
     const double ly = (double) g.ny;
     const double dyw = fmod(+dy, ly), dye = fmod(-dy, ly);
-    const int jw = le_floor(dyw, ly), je = le_floor(dye, ly);
+    int jw = 0, je = 0;
+    if (le_floor(dyw, ly, &jw) || le_floor(dye, ly, &je)) return -1;
     {
       KTimer t("le_cahn_hilliard");
       ldw::launch_le_cahn_hilliard(g, le->geo, p, (const double *) phi->d_data[phi->cur], phi->d_phib,
@@ -1926,9 +1928,10 @@ extern "C" int lb_data_apply_le_boundary_conditions(lb_t * lb, lees_edw_t * le) 
   double fr[2], duy[2];
   for (int side = 0; side < 2; side++) {
     const int cx = 1 - 2*side;
-    dj[side] = le_floor(fmod(dy*cx, ly), ly);
     const double d = fmod(dy*cx, ly);
-    const int jdy = le_floor(d, ly);
+    int jdy = 0;
+    if (le_floor(d, ly, &jdy)) return -1;
+    dj[side] = jdy;                // model_le.c:378 floor(fmod(dy cx, L_y))
     fr[side] = d - jdy;
     duy[side] = -1.0*cx*le->opts.uy;
   }
