@@ -389,6 +389,15 @@ int main(int argc, char ** argv) {
     fatal("colloids are not supported by this build", NULL);
   }
   get_double("temperature", &(double) {0.0});
+  {
+    /* physics_rt.c:144-146: an external chemical potential gradient adds
+     * -M grad_mu to every Cahn-Hilliard face flux (phi_cahn_hilliard.c:
+     * 1373-1398); only the zero gradient is built */
+    double gm[3] = {0.0, 0.0, 0.0};
+    if (get_dvec("grad_mu", gm) && (gm[0] != 0.0 || gm[1] != 0.0 || gm[2] != 0.0)) {
+      fatal("a non-zero grad_mu (external chemical potential gradient) is not supported by this build", NULL);
+    }
+  }
 
   pe_t * pe = NULL;
   cs_t * cs = NULL;
