@@ -46,7 +46,8 @@ def within(got, want, bound, what):
     (16, 16, 1, SHAPE), (32, 48, 2, SHAPE), (48, 64, 1, SHAPE), (16, 96, 2, SHAPE),
     (32, 160, 1, SHAPE), (160, 32, 1, SHAPE),
     (32, 96, 2, (8, 4, 16)), (48, 64, 1, (5, 6, 20)), (16, 16, 1, (4, 4, 32)),
-    (80, 112, 1, (6, 5, 17)), (160, 64, 1, (4, 8, 16))])
+    (80, 112, 1, (6, 5, 17)), (160, 64, 1, (4, 8, 16)), (48, 32, 2, (8, 8, 16)),
+    (32, 16, 1, (4, 12, 16))])
 def test_conv_fwd_bwd_against_oracle(cin, cout, batch, shape):
     SHAPE = shape
     lib = L.lib()
