@@ -323,6 +323,34 @@ void launch_le_cahn_hilliard(const Geom & g, const LeGeom & le, const FeParams &
 			     const double * db, const double * u, const double * ub,
 			     double * xface, int jdy_w, double fr_w, int jdy_e, double fr_e,
 			     double * phi_out, double * csum, hipStream_t s);
+// the gradient at the real planes either side of each plane and at buffer
+// planes nh - 1 and nh only (what the plane pre-pass of the step reads)
+void launch_le_grad_planes(const Geom & g, const LeGeom & le, int stencil, const double * phi,
+			   const double * phib, double * gr, double * dr, double * gb, double * db,
+			   hipStream_t s);
+// the raw x-faces at the planes: force fluxes [plane][side][3][nyz] and
+// Cahn-Hilliard fluxes [plane][side][nyz] (side 0 fluxe at x = L, side 1
+// fluxw at x = L + 1)
+void launch_le_force_faces(const Geom & g, const LeGeom & le, const FeParams & fe,
+			   const double * phi, const double * phib, const double * gr,
+			   const double * dr, const double * gb, const double * db,
+			   double * xface, hipStream_t s);
+void launch_le_ch_faces(const Geom & g, const LeGeom & le, const FeParams & fe,
+			const double * phi, const double * phib, const double * dr,
+			const double * db, const double * u, const double * ub,
+			double * xface, hipStream_t s);
+// the per-plane force correction sums from the force x-faces
+void launch_le_plane_sums(const Geom & g, const LeGeom & le, const double * xface, double * fcor,
+			  hipStream_t s);
+// the whole step in two launches of the tiled single-pass kernel
+// (le_step.hip): slab interiors, then (after sums_done) the slab edges with
+// the sums fcor; force stored; csum for conserve 1 (or null)
+void launch_le_step(const Geom & g, const FeParams & fe, const LbParams & prm, const LeGeom & le,
+		    const double * phib, const double * ub, const double * fcor,
+		    const double * chface, double ra, int jw, double frw, int je, double fre,
+		    const double * phi, double * phi_out, const double * u_prev, double * u_next,
+		    double * rho, const double * g_in, double * g_out, double * force, double * csum,
+		    hipStream_t st, hipEvent_t sums_done);
 // lb_data_apply_le_boundary_conditions (model_le.c:78-560) on the
 // post-collision distributions g: side 0 reprojects cx = +1 at plane
 // sites, side 1 cx = -1 above; dj/fr per side, duy the velocity jump

@@ -43,6 +43,7 @@
 #include "d3q19.h"
 #include "grad27.h"
 #include "kernels.h"
+#include "le_plane_sum.h"
 
 namespace ldw {
 
@@ -139,15 +140,13 @@ le_phi_buffer_kernel(Geom g, LeGeom le, LeShift sh, const double * __restrict__ 
     + r6*fr*(fr*fr - 1.0)*pl[(int64_t) j3*g.nz];
 }
 
-__global__ void __launch_bounds__(LB)
-le_grad_kernel(Geom g, LeGeom le, int stencil, const double * __restrict__ phi,
-	       const double * __restrict__ phib, double * __restrict__ gr,
-	       double * __restrict__ dr, double * __restrict__ gb, double * __restrict__ db) {
-  const int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x;
+__device__ __forceinline__ void grad_site(const Geom & g, const LeGeom & le, int stencil,
+					  const double * __restrict__ phi, const double * __restrict__ phib,
+					  double * __restrict__ gr, double * __restrict__ dr,
+					  double * __restrict__ gb, double * __restrict__ db,
+					  int v, int y, int z) {
   const int64_t nyz = (int64_t) g.ny*g.nz;
-  if (i >= (int64_t) (g.nx + le.nxbuf)*nyz) return;
-  const Coord c = coord_of(i, g.ny, g.nz);
-  const int v = c.v, y = c.y, z = c.z;
+  const int64_t i = (int64_t) v*nyz + (int64_t) y*g.nz + z;
   int vm, vp;
   if (v < g.nx) {
     vm = xnb(le, g.nx, v, -1);
@@ -199,6 +198,33 @@ le_grad_kernel(Geom g, LeGeom le, int stencil, const double * __restrict__ phi,
     for (int a = 0; a < 3; a++) gb[a*nb + j] = gv[a];
     db[j] = d;
   }
+}
+
+__global__ void __launch_bounds__(LB)
+le_grad_kernel(Geom g, LeGeom le, int stencil, const double * __restrict__ phi,
+	       const double * __restrict__ phib, double * __restrict__ gr,
+	       double * __restrict__ dr, double * __restrict__ gb, double * __restrict__ db) {
+  const int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x;
+  const int64_t nyz = (int64_t) g.ny*g.nz;
+  if (i >= (int64_t) (g.nx + le.nxbuf)*nyz) return;
+  const Coord c = coord_of(i, g.ny, g.nz);
+  grad_site(g, le, stencil, phi, phib, gr, dr, gb, db, c.v, c.y, c.z);
+}
+
+// the gradient only where the plane pre-pass reads it: for each plane the
+// real planes either side (x = L - 1, L, 0-based) and buffer planes nh - 1
+// and nh
+__global__ void __launch_bounds__(LB)
+le_grad_planes_kernel(Geom g, LeGeom le, int stencil, const double * __restrict__ phi,
+		      const double * __restrict__ phib, double * __restrict__ gr,
+		      double * __restrict__ dr, double * __restrict__ gb, double * __restrict__ db) {
+  const int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x;
+  const int64_t nyz = (int64_t) g.ny*g.nz;
+  if (i >= (int64_t) le.nplane*4*nyz) return;
+  const Coord c = coord_of(i, g.ny, g.nz);
+  const int p = c.v/4, k = c.v%4;
+  const int v = (k < 2) ? le.loc[p] - 1 + k : g.nx + 2*p*le.nh + le.nh - 1 + (k - 2);
+  grad_site(g, le, stencil, phi, phib, gr, dr, gb, db, v, c.y, c.z);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -257,47 +283,13 @@ le_force_faces_kernel(Geom g, LeGeom le, FeParams fe, GradFields F, double * __r
   for (int a = 0; a < 3; a++) xface[((int64_t) ps*3 + a)*nyz + rem] = 0.5*(s1[a] + s0[a]);
 }
 
-// fbar_a = sum over the plane, in site order, of -fluxe + fluxw
-// (phi_force.c:470-500), by one block per plane and component: each
-// addition needs the previous one, so thread 0 adds alone, reading 16
-// values from LDS ahead of the 16 dependent additions, while waves 1-3
-// stage the next tile (double-buffered).
-constexpr int PS_TILE = 1024;
-
-__device__ __forceinline__ void plane_sum_block(const double * __restrict__ xface, int64_t nyz,
-						double * __restrict__ fcor, int blk) {
-  __shared__ double buf[2][PS_TILE];
-  const int p = blk/3, a = blk%3;
-  const double * fe = xface + ((int64_t) (2*p + 0)*3 + a)*nyz;
-  const double * fw = xface + ((int64_t) (2*p + 1)*3 + a)*nyz;
-  const int tid = threadIdx.x;
-  const int64_t ntile = (nyz + PS_TILE - 1)/PS_TILE;
-  for (int k = tid; k < PS_TILE; k += blockDim.x) buf[0][k] = (k < nyz) ? - fe[k] + fw[k] : 0.0;
-  __syncthreads();
-  double s = 0.0;
-  for (int64_t t = 0; t < ntile; t++) {
-    const int cur = (int) (t & 1);
-    if (tid == 0) {
-      const int m = (int) ((nyz - t*PS_TILE < PS_TILE) ? nyz - t*PS_TILE : PS_TILE);
-      int k = 0;
-      for (; k + 16 <= m; k += 16) {
-	double v[16];
-#pragma unroll
-	for (int q = 0; q < 16; q++) v[q] = buf[cur][k + q];
-#pragma unroll
-	for (int q = 0; q < 16; q++) s += v[q];
-      }
-      for (; k < m; k++) s += buf[cur][k];
-    }
-    else if (tid >= 64 && t + 1 < ntile) {
-      const int64_t base = (t + 1)*PS_TILE;
-      for (int k = tid - 64; k < PS_TILE; k += blockDim.x - 64) {
-	buf[1 - cur][k] = (base + k < nyz) ? - fe[base + k] + fw[base + k] : 0.0;
-      }
-    }
-    __syncthreads();
-  }
-  if (tid == 0) fcor[3*p + a] = s;
+// the plane sums alone, one block per plane and component.  The bound of
+// 10 waves per SIMD holds it to 48 VGPRs, so that a block fits on a CU
+// beside a block of the single-pass step (2 waves per SIMD at 232 of the
+// 512 VGPRs, 112 KB of the 160 KB LDS)
+__global__ void __launch_bounds__(LB, 10)
+le_plane_sums_kernel(const double * __restrict__ xface, int64_t nyz, double * __restrict__ fcor) {
+  plane_sum_block(xface, nyz, fcor, blockIdx.x);
 }
 
 // force = -(fluxe - fluxw + fluxy - fluxy(y-1) + fluxz - fluxz(z-1)) at
@@ -677,6 +669,48 @@ void launch_le_grad(const Geom & g, const LeGeom & le, int stencil, const double
   const int64_t n = (int64_t) (g.nx + le.nxbuf)*g.ny*g.nz;
   hipLaunchKernelGGL(le_grad_kernel, dim3(nblk(n)), dim3(LB), 0, s, g, le, stencil, phi, phib,
 		     gr, dr, gb, db);
+}
+
+void launch_le_grad_planes(const Geom & g, const LeGeom & le, int stencil, const double * phi,
+			   const double * phib, double * gr, double * dr, double * gb, double * db,
+			   hipStream_t s) {
+  const int64_t n = (int64_t) le.nplane*4*g.ny*g.nz;
+  hipLaunchKernelGGL(le_grad_planes_kernel, dim3(nblk(n)), dim3(LB), 0, s, g, le, stencil, phi, phib,
+		     gr, dr, gb, db);
+}
+
+void launch_le_plane_sums(const Geom & g, const LeGeom & le, const double * xface, double * fcor,
+			  hipStream_t s) {
+  hipLaunchKernelGGL(le_plane_sums_kernel, dim3(3*le.nplane), dim3(LB), 0, s, xface,
+		     (int64_t) g.ny*g.nz, fcor);
+}
+
+void launch_le_force_faces(const Geom & g, const LeGeom & le, const FeParams & fe,
+			   const double * phi, const double * phib, const double * gr,
+			   const double * dr, const double * gb, const double * db,
+			   double * xface, hipStream_t s) {
+  const GradFields F = grad_fields(g, le, phi, phib, gr, dr, gb, db);
+  const int64_t nf = (int64_t) le.nplane*2*g.ny*g.nz;
+  hipLaunchKernelGGL(le_force_faces_kernel, dim3(nblk(nf)), dim3(LB), 0, s, g, le, fe, F, xface);
+}
+
+void launch_le_ch_faces(const Geom & g, const LeGeom & le, const FeParams & fe,
+			const double * phi, const double * phib, const double * dr,
+			const double * db, const double * u, const double * ub,
+			double * xface, hipStream_t s) {
+  const int64_t nyz = (int64_t) g.ny*g.nz;
+  const int64_t nb = (int64_t) le.nxbuf*nyz;
+  ChFields C;
+  C.phi = {phi, phib, g.nx, g.ny, g.nz};
+  C.d = {dr, db, g.nx, g.ny, g.nz};
+  C.adv = (u != nullptr);
+  const double * u0 = u ? u : phi;
+  const double * b0 = ub ? ub : phib;
+  C.ux = {u0, b0, g.nx, g.ny, g.nz};
+  C.uy = {u ? u + g.nsite : u0, ub ? ub + nb : b0, g.nx, g.ny, g.nz};
+  C.uz = {u ? u + 2*g.nsite : u0, ub ? ub + 2*nb : b0, g.nx, g.ny, g.nz};
+  const int64_t nf = (int64_t) le.nplane*2*nyz;
+  hipLaunchKernelGGL(le_ch_faces_kernel, dim3(nblk(nf)), dim3(LB), 0, s, g, le, fe, C, xface);
 }
 
 void launch_le_u_buffer(const Geom & g, const LeGeom & le, const LeShift & sh, double uy,

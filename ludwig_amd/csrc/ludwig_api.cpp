@@ -93,6 +93,11 @@ int ldw::api::fail(const char * file, int line, const char * fmt, ...) {
 /* ------------------------------------------------------------------------ */
 
 static hipStream_t g_stream = nullptr;
+// a second stream, default priority like the first, for work that runs
+// beside a launch on g_stream (the Lees-Edwards plane sums), and the
+// events that order it
+static hipStream_t g_stream2 = nullptr;
+static hipEvent_t g_ev_fork = nullptr, g_ev_join = nullptr;
 static int g_ready = 0;
 static int flush_force(void);
 static int materialise_derived(void);
@@ -106,6 +111,9 @@ static int device_init(void) {
     FAIL("no HIP device available: libludwig_hip runs on an MI355X (gfx950)");
   }
   HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  HIP_CHECK(hipStreamCreateWithFlags(&g_stream2, hipStreamNonBlocking));
+  HIP_CHECK(hipEventCreateWithFlags(&g_ev_fork, hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&g_ev_join, hipEventDisableTiming));
   g_ready = 1;
   return 0;
 }
@@ -707,6 +715,7 @@ struct lees_edw_s {
   double * d_xface;        // nplane*2*3*ny*nz
   double * d_fcor;         // nplane*3
   double * d_sbuf;         // nplane*2*5*ny*nz
+  double * d_chface;       // nplane*2*ny*nz (Cahn-Hilliard faces of the single-pass step)
 };
 
 static int le_active(const lees_edw_t * le) { return le != nullptr && le->nplanes > 0; }
@@ -755,6 +764,7 @@ extern "C" int lees_edw_create(pe_t *, cs_t * cs, const lees_edw_options_t * opt
   if (dmalloc((void **) &le->d_xface, 8*(size_t) opts->nplanes*2*3*nyz)) return -1;
   if (dmalloc((void **) &le->d_fcor, 8*(size_t) opts->nplanes*3)) return -1;
   if (dmalloc((void **) &le->d_sbuf, 8*(size_t) opts->nplanes*2*5*nyz)) return -1;
+  if (dmalloc((void **) &le->d_chface, 8*(size_t) opts->nplanes*2*nyz)) return -1;
   return 0;
 }
 
@@ -764,6 +774,7 @@ extern "C" int lees_edw_free(lees_edw_t * le) {
   if (le->d_xface) (void) hipFree(le->d_xface);
   if (le->d_fcor) (void) hipFree(le->d_fcor);
   if (le->d_sbuf) (void) hipFree(le->d_sbuf);
+  if (le->d_chface) (void) hipFree(le->d_chface);
   free(le);
   return 0;
 }
@@ -895,6 +906,7 @@ struct field_grad_s {
   // itself: real sites (d_gr, d_dr) and buffer sites (d_gb, d_db)
   double * d_gr, * d_dr, * d_gb, * d_db;
   long le_version;
+  int le_partial;          // after a single-pass step: gradient only next to the planes
 };
 
 extern "C" field_options_t field_options_ndata_nhalo(int ndata, int nhalo) {
@@ -1073,6 +1085,7 @@ extern "C" int field_grad_compute(field_grad_t * g) {
 			f->d_phib, g->d_gr, g->d_dr, g->d_gb, g->d_db, g_stream);
     CHECK_LAUNCH();
     g->le_version = f->version;
+    g->le_partial = 0;
   }
   return 0;
 }
@@ -1083,7 +1096,15 @@ extern "C" int field_grad_memcpy(field_grad_t * g, tdpMemcpyKind flag) {
   const Geom geo = f->geo;
   if (f->prec != 64) FAIL("field_grad_memcpy: gradient read-back is built for FP64 only");
   if (le_active(f->le)) {
-    // the gradient the last field_grad_compute() formed
+    // the gradient the last field_grad_compute() formed; after a
+    // single-pass step only the planes' neighbourhood was formed, so the
+    // rest is formed now from the phi and buffer planes of that step's start
+    if (g->le_partial) {
+      ldw::launch_le_grad(geo, f->le->geo, grad_stencil(g), (const double *) f->d_data[g->grad_buf],
+			  f->d_phib, g->d_gr, g->d_dr, g->d_gb, g->d_db, g_stream);
+      CHECK_LAUNCH();
+      g->le_partial = 0;
+    }
     std::vector<double> b1(3*geo.nsite), b2(geo.nsite);
     HIP_CHECK(hipMemcpyAsync(b1.data(), g->d_gr, 8*3*geo.nsite, hipMemcpyDeviceToHost, g_stream));
     HIP_CHECK(hipMemcpyAsync(b2.data(), g->d_dr, 8*geo.nsite, hipMemcpyDeviceToHost, g_stream));
@@ -1915,17 +1936,13 @@ extern "C" int lb_collide(lb_t * lb, hydro_t * hydro, map_t *, noise_t * noise, 
 
 /* model_le.c:78-170: on the post-collision distributions of lb_collide();
  * the displacement is that of time step t = physics_control_timestep() */
-extern "C" int lb_data_apply_le_boundary_conditions(lb_t * lb, lees_edw_t * le) {
-  if (!lb || !le) FAIL("lb_data_apply_le_boundary_conditions: null argument");
-  if (!le_active(le)) return 0;
-  if (lb->prec != 64) FAIL("lb_data_apply_le_boundary_conditions: Lees-Edwards planes are built for FP64");
-  if (lb->streamed) FAIL("lb_data_apply_le_boundary_conditions: needs the post-collision distributions (call it between lb_collide and lb_propagation)");
+/* the integer displacement, fraction and velocity jump of each plane side
+ * at time step t = physics_control_timestep() (model_le.c:100-105, 245-395,
+ * 585-625) */
+static int le_lb_shifts(const lees_edw_t * le, double ly, int dj[2], double fr[2], double duy[2]) {
   const double t = 1.0*phys_timestep();
-  const double ly = (double) lb->geo.ny;
   // lees_edw_buffer_displacement at ib = nhalo (sign +1); plane_uy_now
   const double dy = le_buffer_displacement(le, le->nh, t);
-  int dj[2];
-  double fr[2], duy[2];
   for (int side = 0; side < 2; side++) {
     const int cx = 1 - 2*side;
     const double d = fmod(dy*cx, ly);
@@ -1935,6 +1952,17 @@ extern "C" int lb_data_apply_le_boundary_conditions(lb_t * lb, lees_edw_t * le) 
     fr[side] = d - jdy;
     duy[side] = -1.0*cx*le->opts.uy;
   }
+  return 0;
+}
+
+extern "C" int lb_data_apply_le_boundary_conditions(lb_t * lb, lees_edw_t * le) {
+  if (!lb || !le) FAIL("lb_data_apply_le_boundary_conditions: null argument");
+  if (!le_active(le)) return 0;
+  if (lb->prec != 64) FAIL("lb_data_apply_le_boundary_conditions: Lees-Edwards planes are built for FP64");
+  if (lb->streamed) FAIL("lb_data_apply_le_boundary_conditions: needs the post-collision distributions (call it between lb_collide and lb_propagation)");
+  int dj[2];
+  double fr[2], duy[2];
+  if (le_lb_shifts(le, (double) lb->geo.ny, dj, fr, duy)) return -1;
   {
     KTimer tm("le_lb_bc");
     ldw::launch_le_lb_bc(lb->geo, le->geo, dj, fr, duy, (double *) lb->d_g[lb->cur], le->d_sbuf, g_stream);
@@ -2069,6 +2097,89 @@ static int step_fused(lb_t * lb, hydro_t * hydro, field_t * phi, fe_symm_t * fs,
   return 0;
 }
 
+/* The step with Lees-Edwards planes through the tiled single-pass kernel
+ * (le_step.hip): the plane kernels fill the buffer planes and the raw
+ * plane faces, two launches form the step (slab interiors with the force
+ * correction sums, then the slab edges), and the LB plane conditions act
+ * on the new post-collision distributions.  The same bits as
+ * step_staged().  The force is stored; the gradient only next to the
+ * planes (field_grad_memcpy forms the rest on demand). */
+static int step_le_fused(lb_t * lb, hydro_t * hydro, field_t * phi, fe_symm_t * fs, pth_t * pth,
+			 phi_ch_t * pch) {
+  lees_edw_t * le = phi->le;
+  field_grad_t * dphi = fs->dphi;
+  const Geom g = lb->geo;
+  const double ly = (double) g.ny;
+  if (flush_force() || materialise_derived()) return -1;
+  if (g_derived == hydro) { g_derived = nullptr; hydro->fd_derived = 0; }
+  if (hydro->d_u2 == nullptr && dmalloc(&hydro->d_u2, 8*3*(size_t) g.nsite)) return -1;
+  // the step starts from hydro_f_zero(0) and the velocity of the last collision
+  const double fz[3] = {0.0, 0.0, 0.0};
+  if (hydro_f_zero(hydro, fz) || hydro_materialise(hydro)) return -1;
+  ldw::LeShift shp, shu;
+  if (le_buffer_shift(le, 0.0, 1, &shp) || le_buffer_shift(le, 1.0, 0, &shu)) return -1;
+  const double dy = phys_time()*le->opts.uy;
+  const double dyw = fmod(+dy, ly), dye = fmod(-dy, ly);
+  int jw = 0, je = 0;
+  if (le_floor(dyw, ly, &jw) || le_floor(dye, ly, &je)) return -1;
+  int dj[2];
+  double fr[2], duy[2];
+  if (le_lb_shifts(le, ly, dj, fr, duy)) return -1;
+  const FeParams p = fe_params_of(fs, pth->method);
+  const LbParams q = lb_params_of(lb, nullptr);
+  const double ra = 0.5/((double) g.ny*(double) g.nz);
+  const double * phi0 = (const double *) phi->d_data[phi->cur];
+  dphi->grad_version = phi->version;
+  dphi->grad_buf = phi->cur;
+  {
+    KTimer t("le_planes");
+    ldw::launch_le_phi_buffer(g, le->geo, shp, phi0, phi->d_phib, g_stream);
+    ldw::launch_le_grad_planes(g, le->geo, grad_stencil(dphi), phi0, phi->d_phib, dphi->d_gr, dphi->d_dr,
+			       dphi->d_gb, dphi->d_db, g_stream);
+    ldw::launch_le_force_faces(g, le->geo, p, phi0, phi->d_phib, dphi->d_gr, dphi->d_dr, dphi->d_gb,
+			       dphi->d_db, le->d_xface, g_stream);
+    CHECK_LAUNCH();
+    // the serial plane sums, the step's critical path, start as soon as
+    // the x-faces are there and run on the second stream beside the rest
+    HIP_CHECK(hipEventRecord(g_ev_fork, g_stream));
+    HIP_CHECK(hipStreamWaitEvent(g_stream2, g_ev_fork, 0));
+    ldw::launch_le_plane_sums(g, le->geo, le->d_xface, le->d_fcor, g_stream2);
+    CHECK_LAUNCH();
+    HIP_CHECK(hipEventRecord(g_ev_join, g_stream2));
+    ldw::launch_le_u_buffer(g, le->geo, shu, le->opts.uy, (const double *) hydro->d_u, hydro->d_ub, g_stream);
+    ldw::launch_le_ch_faces(g, le->geo, p, phi0, phi->d_phib, dphi->d_dr, dphi->d_db,
+			    (const double *) hydro->d_u, hydro->d_ub, le->d_chface, g_stream);
+  }
+  CHECK_LAUNCH();
+  {
+    KTimer t("le_step");
+    ldw::launch_le_step(g, p, q, le->geo, phi->d_phib, hydro->d_ub, le->d_fcor, le->d_chface,
+			ra, jw, dyw - jw, je, dye - je, phi0, (double *) phi->d_data[1 - phi->cur],
+			(const double *) hydro->d_u, (double *) hydro->d_u2, (double *) hydro->d_rho,
+			(const double *) lb->d_g[lb->cur], (double *) lb->d_g[1 - lb->cur],
+			(double *) hydro->d_force, (double *) pch->d_csum, g_stream, g_ev_join);
+  }
+  CHECK_LAUNCH();
+  {
+    KTimer t("le_lb_bc");
+    ldw::launch_le_lb_bc(g, le->geo, dj, fr, duy, (double *) lb->d_g[1 - lb->cur], le->d_sbuf, g_stream);
+  }
+  CHECK_LAUNCH();
+  phi->cur = 1 - phi->cur;
+  phi->version++;
+  void * tu = hydro->d_u; hydro->d_u = hydro->d_u2; hydro->d_u2 = tu;
+  lb->cur = 1 - lb->cur;
+  lb->streamed = 1;
+  hydro->u_uniform = 0;
+  hydro->force_uniform = 0;
+  dphi->le_version = phi->version - 1;
+  dphi->le_partial = 1;
+  dphi->mu_version = 0;
+  return ch_subtract_drift(pch, phi);
+}
+
+static int g_le_fused = -1;       // LDW_LE_FUSED=0 keeps the staged plane path
+
 static int g_fused_step = -1;     // LDW_FUSED_STEP=0 selects the staged calls
 
 extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_grad_t * dphi,
@@ -2076,6 +2187,10 @@ extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_
   if (g_fused_step < 0) {
     const char * e = getenv("LDW_FUSED_STEP");
     g_fused_step = (e && atoi(e) == 0) ? 0 : 1;
+  }
+  if (g_le_fused < 0) {
+    const char * e = getenv("LDW_LE_FUSED");
+    g_le_fused = (e && atoi(e) == 0) ? 0 : 1;
   }
   fe_symm_t * fs = as_symm(fe);
   physics_t * phys = nullptr;
@@ -2087,8 +2202,18 @@ extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_
       && lb->streamed && !hydro->u_uniform && !hydro->fp_pending
       && lb->prec == hydro->prec && phi->prec == lb->prec && fs->phi == phi
       && pch && pch->info.conserve == 0 && !le_active(phi->le) && !le_active(hydro->le);
+    // the planes' single-pass step: FP64, a force to form, the velocity of
+    // a collision, and slabs at least 3 planes deep
+    const bool le_fusable = g_fused_step && g_le_fused && fs && pth && hydro && pch
+      && le_active(phi->le) && hydro->le == phi->le && fs->phi == phi
+      && pth->method != FE_FORCE_METHOD_NO_FORCE && lb->streamed
+      && lb->prec == 64 && phi->prec == 64 && hydro->prec == 64
+      && phi->geo.nx/phi->le->nplanes >= 3;
     if (fusable) {
       if (step_fused(lb, hydro, phi, fs, pth)) return -1;
+    }
+    else if (le_fusable) {
+      if (step_le_fused(lb, hydro, phi, fs, pth, pch)) return -1;
     }
     else {
       if (step_staged(lb, hydro, phi, dphi, fe, pth, pch)) return -1;

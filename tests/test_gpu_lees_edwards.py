@@ -150,6 +150,44 @@ def test_le_binary_vs_oracle(shape, planes, uy, order, stencil, conserve, profil
     b.close()
 
 
+@pytest.mark.parametrize("order,stencil,conserve", [(1, 7, 0), (3, 27, 1), (2, 7, 2)])
+def test_le_single_pass_step_equals_staged_calls(order, stencil, conserve):
+    """ldw_step_binary takes the tiled single-pass step with planes (from the
+    second step, once the distributions are streamed); the reference's call
+    sequence, stage by stage, gives the same bits on 64 x 48 x 40 with four
+    planes, 12 steps."""
+    shape, planes, uy = (64, 48, 40), 4, 0.23
+    print(f"case single-pass vs staged order={order} stencil={stencil} conserve={conserve}", flush=True)
+    kw = dict(tau=0.8, mobility=0.2, order=order, stencil=stencil, conserve=conserve,
+              le_planes=planes, le_uy=uy)
+    a = L.BinaryFluid(shape, **kw)
+    a.init_spinodal(99, 0.05, 0.1)
+    a.le_init_shear_profile()
+    a.step(12)
+    b = L.BinaryFluid(shape, **kw)
+    b.init_spinodal(99, 0.05, 0.1)
+    b.le_init_shear_profile()
+    for _ in range(12):
+        b.next_step()
+        b.f_zero()
+        b.gradient_stage()
+        b.force_stage()
+        b.cahn_hilliard_stage()
+        b.u_zero()
+        b.collide()
+        b.le_boundary_conditions()
+        b.propagate()
+    assert_exact(a.phi_field(), b.phi_field(), "phi")
+    ra, ua, fa = a.hydro_fields()
+    rb, ub, fb = b.hydro_fields()
+    assert_exact(ra, rb, "rho")
+    assert_exact(ua, ub, "u")
+    assert_exact(fa, fb, "force")
+    assert_exact(a.f(), b.f(), "f")
+    a.close()
+    b.close()
+
+
 def test_le_conservation_and_shear_profile():
     """64 x 32 x 32 with 4 planes: sums of rho and phi stay put to rounding
     over 60 steps; with phi = 0 (no thermodynamic force) the steady shear
