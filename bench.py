@@ -125,9 +125,14 @@ def run_ensemble(nrep, steps, warmup, precision=32, seed=2026):
             "mass_first_last": mass}
 
 
-# Lees-Edwards path (FP64, one kernel per stage): algorithmic bytes per site
-# of each kernel; the plane-only kernels (faces, sums, LB plane conditions)
-# touch O(N^2) sites and are left out
+# Lees-Edwards path (FP64).  The single-pass step (two launches of the
+# tiled kernel, timer "le_step") reads phi, u and f and writes phi, rho, u,
+# force and f: 400 bytes per site, the plane-free step's 376 plus the stored
+# force.  The staged path (LDW_LE_FUSED=0) has one kernel per stage, with the
+# algorithmic bytes per site of each below.  The plane-only kernels (buffer
+# planes, faces, sums, LB plane conditions) touch O(N^2) sites and are left
+# out of both.
+LE_STEP_BYTES_SITE = 400
 LE_BYTES_SITE = {"le_gradient": 40,        # phi in; grad (3) and delsq out
                  "le_force": 64,           # phi, grad, delsq in; force out
                  "le_cahn_hilliard": 48,   # phi, delsq, u in; phi out
@@ -155,20 +160,24 @@ def run_le(n=128, steps=50, warmup=5, nplanes=2, uy=0.05):
     b.synchronize()
     lib.ldw_timer_enable(0)
     kt, rate = {}, {}
-    for name in ("le_gradient", "le_force", "le_cahn_hilliard", "collide", "le_lb_bc"):
+    bytes_site = dict(LE_BYTES_SITE, le_step=LE_STEP_BYTES_SITE)
+    for name in ("le_planes", "le_step", "le_gradient", "le_force", "le_cahn_hilliard", "collide",
+                 "le_lb_bc"):
         ms, cnt = ctypes.c_double(), ctypes.c_int()
         lib.ldw_timer_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
         if cnt.value:
             kt[name] = ms.value / cnt.value
-            if name in LE_BYTES_SITE:
-                rate[name] = LE_BYTES_SITE[name] * n ** 3 / (kt[name] * 1e-3) / 1e9
+            if name in bytes_site:
+                rate[name] = bytes_site[name] * n ** 3 / (kt[name] * 1e-3) / 1e9
     r, _ = b.conserved_sums()
     b.close()
     mlups = n ** 3 * steps / wall / 1e6
+    bps = LE_STEP_BYTES_SITE if "le_step" in kt else sum(LE_BYTES_SITE.values())
     return {"n": n, "planes": nplanes, "plane_speed": uy, "steps": steps,
             "ms_per_step": 1e3 * wall / steps, "mlups": mlups,
-            "bytes_per_site_algorithmic": sum(LE_BYTES_SITE.values()),
-            "hbm_frac_step": mlups * 1e6 * sum(LE_BYTES_SITE.values()) / (HBM_PEAK_GBS * 1e9),
+            "path": "single-pass" if "le_step" in kt else "staged",
+            "bytes_per_site_algorithmic": bps,
+            "hbm_frac_step": mlups * 1e6 * bps / (HBM_PEAK_GBS * 1e9),
             "kernel_ms": kt, "kernel_GBps": rate,
             "mass_ok": bool(abs(r - n ** 3) / n ** 3 < 1e-6)}
 

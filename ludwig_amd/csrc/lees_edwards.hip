@@ -283,13 +283,13 @@ le_force_faces_kernel(Geom g, LeGeom le, FeParams fe, GradFields F, double * __r
   for (int a = 0; a < 3; a++) xface[((int64_t) ps*3 + a)*nyz + rem] = 0.5*(s1[a] + s0[a]);
 }
 
-// the plane sums alone, one block per plane and component.  The bound of
-// 10 waves per SIMD holds it to 48 VGPRs, so that a block fits on a CU
-// beside a block of the single-pass step (2 waves per SIMD at 232 of the
-// 512 VGPRs, 112 KB of the 160 KB LDS)
-__global__ void __launch_bounds__(LB, 10)
+// the plane sums alone, one block per plane and component.  The lean sum
+// takes 48 VGPRs and 16 KB of LDS, so that a block fits on a CU beside a
+// block of the single-pass step (2 waves per SIMD at 232 of the 512 VGPRs,
+// 112 KB of the 160 KB LDS)
+__global__ void __launch_bounds__(LB)
 le_plane_sums_kernel(const double * __restrict__ xface, int64_t nyz, double * __restrict__ fcor) {
-  plane_sum_block(xface, nyz, fcor, blockIdx.x);
+  plane_sum_block<true>(xface, nyz, fcor, blockIdx.x);
 }
 
 // force = -(fluxe - fluxw + fluxy - fluxy(y-1) + fluxz - fluxz(z-1)) at
@@ -357,7 +357,7 @@ le_force_kernel(Geom g, LeGeom le, FeParams fe, GradFields F, const double * __r
 		double fz0, double fz1, double fz2) {
   const int nsum = 3*le.nplane;
   if ((int) blockIdx.x < nsum) {
-    plane_sum_block(xface, (int64_t) g.ny*g.nz, fcor, blockIdx.x);
+    plane_sum_block<false>(xface, (int64_t) g.ny*g.nz, fcor, blockIdx.x);
     return;
   }
   const int64_t i = (int64_t) (blockIdx.x - nsum)*blockDim.x + threadIdx.x;

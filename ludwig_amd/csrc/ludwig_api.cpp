@@ -2113,9 +2113,13 @@ static int step_le_fused(lb_t * lb, hydro_t * hydro, field_t * phi, fe_symm_t * 
   if (flush_force() || materialise_derived()) return -1;
   if (g_derived == hydro) { g_derived = nullptr; hydro->fd_derived = 0; }
   if (hydro->d_u2 == nullptr && dmalloc(&hydro->d_u2, 8*3*(size_t) g.nsite)) return -1;
-  // the step starts from hydro_f_zero(0) and the velocity of the last collision
+  // the step starts from hydro_f_zero(0) and the velocity of the last
+  // collision; it writes the force (zero plus the thermodynamic force) at
+  // every site, so the zero is not written first
   const double fz[3] = {0.0, 0.0, 0.0};
-  if (hydro_f_zero(hydro, fz) || hydro_materialise(hydro)) return -1;
+  if (hydro_f_zero(hydro, fz)) return -1;
+  hydro->force_uniform = 0;
+  if (hydro_materialise(hydro)) return -1;
   ldw::LeShift shp, shu;
   if (le_buffer_shift(le, 0.0, 1, &shp) || le_buffer_shift(le, 1.0, 0, &shu)) return -1;
   const double dy = phys_time()*le->opts.uy;
