@@ -119,11 +119,14 @@ def test_le_binary_step_golden(golden, case):
     ((48, 20, 12), 4, 0.3, 3, 27, 0, True, 15),
     ((40, 24, 16), 2, 0.71, 2, 7, 1, False, 12),
     ((36, 18, 10), 3, 0.05, 1, 7, 2, True, 10),
+    ((20, 12, 10), 1, 0.17, 2, 27, 0, True, 8),       # one plane: one slab that wraps in x
+    ((24, 8, 6), 4, 0.17, 1, 7, 1, False, 8),         # 6-deep slabs, the shallowest allowed
 ])
 def test_le_binary_vs_oracle(shape, planes, uy, order, stencil, conserve, profile, nsteps):
-    """Non-cubic lattices, up to four planes, displacements that wrap in y,
-    conserve 1 and 2: the GPU step loop against the CPU oracle's
-    (oracle/le_oracle.c), bit for bit."""
+    """Non-cubic lattices, one to four planes, displacements that wrap in y,
+    conserve 1 and 2, and slabs 6 sites deep (planes must sit more than
+    nhalo sites from the x boundaries, so no slab is shallower): the GPU
+    step loop against the CPU oracle's (oracle/le_oracle.c), bit for bit."""
     import oracle as O
     print(f"case {shape} planes={planes} uy={uy} order={order} stencil={stencil} "
           f"conserve={conserve}", flush=True)
