@@ -27,13 +27,13 @@ namespace {
 
 using namespace tile;
 
-template <typename T, class S, int ORDER, int METHOD, bool GZ>
+template <typename T, class S, int ORDER, int METHOD, bool GZ, bool CONS>
 __global__ void __launch_bounds__(S::NT)
 binary_step_kernel(Geom g, FeParams fe, LbParams prm, int xchunk, int nbz, int nby, int nbx, int remap,
 		   const T * __restrict__ phi, T * __restrict__ phi_out,
 		   const T * __restrict__ u_prev, T * __restrict__ u_next,
 		   T * __restrict__ rho, const T * __restrict__ g_in,
-		   T * __restrict__ g_out, T fz0, T fz1, T fz2) {
+		   T * __restrict__ g_out, T fz0, T fz1, T fz2, T * __restrict__ csum) {
 
   __shared__ Smem<T, S> s;
 
@@ -41,8 +41,8 @@ binary_step_kernel(Geom g, FeParams fe, LbParams prm, int xchunk, int nbz, int n
   if (!tile_of_block(nbz, nby, nbx, remap, bzi, byi, bxi)) return;
   const int xb = bxi*xchunk;
   const int xe = min(xb + xchunk, g.nx);
-  step_tile<T, S, ORDER, METHOD, GZ>(s, g, fe, prm, byi*S::TY, bzi*S::TZ, xb, xe, phi, phi_out,
-				     u_prev, u_next, rho, g_in, g_out, fz0, fz1, fz2);
+  step_tile<T, S, ORDER, METHOD, GZ, CONS>(s, g, fe, prm, byi*S::TY, bzi*S::TZ, xb, xe, phi, phi_out,
+					   u_prev, u_next, rho, g_in, g_out, fz0, fz1, fz2, csum);
 }
 
 // Blocks of this tile that the whole GPU holds at once (resident blocks per
@@ -54,7 +54,7 @@ int resident_slots() {
     int per_cu = 0, dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess
-	|| hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, binary_step_kernel<T, S, 1, 0, true>,
+	|| hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, binary_step_kernel<T, S, 1, 0, true, false>,
 							 S::NT, 0) != hipSuccess || per_cu < 1) {
       (void) hipGetLastError();
       return 256;
@@ -82,10 +82,10 @@ inline int step_xchunk(int nx, int64_t ncol, int slots) {
   return best;
 }
 
-template <typename T, class S>
+template <typename T, class S, bool CONS>
 void launch_shape(const Geom & g, const FeParams & fe, const LbParams & prm, const T * phi,
 		  T * phi_out, const T * u_prev, T * u_next, T * rho, const T * g_in,
-		  T * g_out, const double fzero[3], hipStream_t st) {
+		  T * g_out, const double fzero[3], T * csum, hipStream_t st) {
   const int bz = (g.nz + S::TZ - 1)/S::TZ;
   const int by = (g.ny + S::TY - 1)/S::TY;
   int xchunk = g.nx;
@@ -102,9 +102,9 @@ void launch_shape(const Geom & g, const FeParams & fe, const LbParams & prm, con
   dim3 grid(((ntile + NXCD - 1)/NXCD)*NXCD), block(S::TZ, S::TY, 1);
   const T f0 = T(fzero[0]), f1 = T(fzero[1]), f2 = T(fzero[2]);
   const bool gz = (prm.relax == 0);
-#define L_(O, M, G) hipLaunchKernelGGL((binary_step_kernel<T, S, O, M, G>), grid, block, 0, st, \
+#define L_(O, M, G) hipLaunchKernelGGL((binary_step_kernel<T, S, O, M, G, CONS>), grid, block, 0, st, \
 				       g, fe, prm, xchunk, bz, by, bx, tune_remap(), phi, phi_out, u_prev, u_next, rho, \
-				       g_in, g_out, f0, f1, f2)
+				       g_in, g_out, f0, f1, f2, csum)
 #define L_G(O, M) do { if (gz) L_(O, M, true); else L_(O, M, false); } while (0)
 #define L_M(O) do { if (fe.force_method == 0) L_G(O, 0); else L_G(O, 1); } while (0)
   if (fe.order == 1) L_M(1);
@@ -119,15 +119,21 @@ void launch_shape(const Geom & g, const FeParams & fe, const LbParams & prm, con
 // 64 x 8 / 128 x 4 tiles, which halve the share of pulled rows that cross a
 // tile edge in y or z; chosen by precision and lattice size
 // (tune_step_tile).  The tile changes only which block computes a site,
-// not the arithmetic.
+// not the arithmetic.  With a compensation array (conserve 1) the 64 x 8
+// tile is used, so the compensated kernels are built for one tile only.
 template <typename T>
 void launch(const Geom & g, const FeParams & fe, const LbParams & prm, const T * phi,
 	    T * phi_out, const T * u_prev, T * u_next, T * rho, const T * g_in,
-	    T * g_out, const double fzero[3], hipStream_t st) {
+	    T * g_out, const double fzero[3], T * csum, hipStream_t st) {
+  if (csum) {
+    launch_shape<T, Shape<64, 8>, true>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero,
+					csum, st);
+    return;
+  }
   switch (tune_step_tile(sizeof(T) == 8 ? 64 : 32, g.nsite)) {
-  case 1: launch_shape<T, Shape<64, 8>>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero, st); break;
-  case 2: launch_shape<T, Shape<128, 4>>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero, st); break;
-  default: launch_shape<T, Shape64x4>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero, st); break;
+  case 1: launch_shape<T, Shape<64, 8>, false>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero, nullptr, st); break;
+  case 2: launch_shape<T, Shape<128, 4>, false>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero, nullptr, st); break;
+  default: launch_shape<T, Shape64x4, false>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero, nullptr, st); break;
   }
 }
 
@@ -136,15 +142,15 @@ void launch(const Geom & g, const FeParams & fe, const LbParams & prm, const T *
 void launch_binary_step_f64(const Geom & g, const FeParams & fe, const LbParams & prm,
 			    const double * phi, double * phi_out, const double * u_prev,
 			    double * u_next, double * rho, const double * g_in,
-			    double * g_out, const double fzero[3], hipStream_t s) {
-  launch<double>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero, s);
+			    double * g_out, const double fzero[3], double * csum, hipStream_t s) {
+  launch<double>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero, csum, s);
 }
 
 void launch_binary_step_f32(const Geom & g, const FeParams & fe, const LbParams & prm,
 			    const float * phi, float * phi_out, const float * u_prev,
 			    float * u_next, float * rho, const float * g_in,
-			    float * g_out, const double fzero[3], hipStream_t s) {
-  launch<float>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero, s);
+			    float * g_out, const double fzero[3], float * csum, hipStream_t s) {
+  launch<float>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero, csum, s);
 }
 
 }  // namespace ldw

@@ -115,11 +115,11 @@ void launch_force_ch_f32(const Geom & g, const FeParams & fe, const float * phi,
 void launch_binary_step_f64(const Geom & g, const FeParams & fe, const LbParams & prm,
 			    const double * phi, double * phi_out, const double * u_prev,
 			    double * u_next, double * rho, const double * g_in,
-			    double * g_out, const double fzero[3], hipStream_t s);
+			    double * g_out, const double fzero[3], double * csum, hipStream_t s);
 void launch_binary_step_f32(const Geom & g, const FeParams & fe, const LbParams & prm,
 			    const float * phi, float * phi_out, const float * u_prev,
 			    float * u_next, float * rho, const float * g_in,
-			    float * g_out, const double fzero[3], hipStream_t s);
+			    float * g_out, const double fzero[3], float * csum, hipStream_t s);
 
 // Replica ensemble (ensemble.hip).  Each replica owns a slice of shared
 // pools: its 19 distribution planes start at element off_f of the f pools,

@@ -2048,7 +2048,11 @@ static int step_staged(lb_t * lb, hydro_t * hydro, field_t * phi, field_grad_t *
 
 /* The same step as step_staged() in one kernel (binary_step.hip).  Needs
  * streamed distributions, a stored velocity field and one precision. */
-static int step_fused(lb_t * lb, hydro_t * hydro, field_t * phi, fe_symm_t * fs, pth_t * pth) {
+// the single-pass step; conserve 1 takes the compensated update (pch's
+// per-site compensation), conserve 2 the global subtraction after it, as
+// phi_cahn_hilliard does
+static int step_fused(lb_t * lb, hydro_t * hydro, field_t * phi, fe_symm_t * fs, pth_t * pth,
+		      phi_ch_t * pch) {
   const Geom g = lb->geo;
   if (hydro->d_u2 == nullptr) {
     size_t es = (hydro->prec == 64) ? 8 : 4;
@@ -2069,14 +2073,14 @@ static int step_fused(lb_t * lb, hydro_t * hydro, field_t * phi, fe_symm_t * fs,
 				  (double *) phi->d_data[1 - phi->cur], (const double *) hydro->d_u,
 				  (double *) hydro->d_u2, (double *) hydro->d_rho,
 				  (const double *) lb->d_g[lb->cur], (double *) lb->d_g[1 - lb->cur],
-				  fz, g_stream);
+				  fz, (double *) pch->d_csum, g_stream);
     }
     else {
       ldw::launch_binary_step_f32(g, p, q, (const float *) phi->d_data[phi->cur],
 				  (float *) phi->d_data[1 - phi->cur], (const float *) hydro->d_u,
 				  (float *) hydro->d_u2, (float *) hydro->d_rho,
 				  (const float *) lb->d_g[lb->cur], (float *) lb->d_g[1 - lb->cur],
-				  fz, g_stream);
+				  fz, (float *) pch->d_csum, g_stream);
     }
   }
   CHECK_LAUNCH();
@@ -2094,7 +2098,7 @@ static int step_fused(lb_t * lb, hydro_t * hydro, field_t * phi, fe_symm_t * fs,
   hydro->fd_dphi = fs->dphi;
   g_derived = hydro;
   fs->dphi->mu_version = 0;
-  return 0;
+  return ch_subtract_drift(pch, phi);
 }
 
 /* The step with Lees-Edwards planes through the tiled single-pass kernel
@@ -2205,7 +2209,8 @@ extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_
       && (pth->method == FE_FORCE_METHOD_STRESS_DIVERGENCE || pth->method == FE_FORCE_METHOD_PHI_GRADMU)
       && lb->streamed && !hydro->u_uniform && !hydro->fp_pending
       && lb->prec == hydro->prec && phi->prec == lb->prec && fs->phi == phi
-      && pch && pch->info.conserve == 0 && !le_active(phi->le) && !le_active(hydro->le);
+      && pch && (pch->info.conserve == 0 || pch->prec == phi->prec)
+      && !le_active(phi->le) && !le_active(hydro->le);
     // the planes' single-pass step: FP64, a force to form, the velocity of
     // a collision, and slabs at least 3 planes deep
     const bool le_fusable = g_fused_step && g_le_fused && fs && pth && hydro && pch
@@ -2214,7 +2219,7 @@ extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_
       && lb->prec == 64 && phi->prec == 64 && hydro->prec == 64
       && phi->geo.nx/phi->le->nplanes >= 3;
     if (fusable) {
-      if (step_fused(lb, hydro, phi, fs, pth)) return -1;
+      if (step_fused(lb, hydro, phi, fs, pth, pch)) return -1;
     }
     else if (le_fusable) {
       if (step_le_fused(lb, hydro, phi, fs, pth, pch)) return -1;

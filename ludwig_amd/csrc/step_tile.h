@@ -41,15 +41,18 @@ __device__ __forceinline__ void load_f(T (&f)[NVEL], const T * __restrict__ g_in
 
 // phi -> phi_out, u_prev -> u_next (the Cahn-Hilliard part reads u_prev),
 // g_in -> g_out with pull indexing, rho written.  fz: the uniform force
-// recorded by hydro_f_zero, added to the phi force.  Ends with a block
-// barrier, so the caller may reuse the LDS tiles at once.
-template <typename T, class S, int ORDER, int METHOD, bool GZ>
+// recorded by hydro_f_zero, added to the phi force.  CONS: the compensated
+// update of cahn_hilliard_options_conserve 1, with the per-site
+// compensation csum read and written.  Ends with a block barrier, so the
+// caller may reuse the LDS tiles at once.
+template <typename T, class S, int ORDER, int METHOD, bool GZ, bool CONS = false>
 __device__ __forceinline__ void step_tile(Smem<T, S> & s, const Geom & g, const FeParams & fe,
 					  const LbParams & prm, int y0, int z0, int xb, int xe,
 					  const T * __restrict__ phi, T * __restrict__ phi_out,
 					  const T * __restrict__ u_prev, T * __restrict__ u_next,
 					  T * __restrict__ rho, const T * __restrict__ g_in,
-					  T * __restrict__ g_out, T fz0, T fz1, T fz2) {
+					  T * __restrict__ g_out, T fz0, T fz1, T fz2,
+					  T * __restrict__ csum = nullptr) {
   const int tz = threadIdx.x, ty = threadIdx.y;
   const int y = y0 + ty, z = z0 + tz;
   const bool active = (y < g.ny) && (z < g.nz);
@@ -77,7 +80,8 @@ __device__ __forceinline__ void step_tile(Smem<T, S> & s, const Geom & g, const 
     if (active) {
       const int64_t site = ((int64_t) x*g.ny + y)*g.nz + z;
       T F[3], pnew;
-      force_ch_site<T, S, ORDER, METHOD, true>(s, fe, x, ty, tz, uc, F, pnew);
+      force_ch_site<T, S, ORDER, METHOD, true>(s, fe, x, ty, tz, uc, F, pnew,
+					       CONS ? csum + site : nullptr);
       phi_out[site] = pnew;
       // site force: force_global + (hydro_f_zero value + phi force)
       // (collision.c:323-327, phi_force_colloid.c:457-462)

@@ -232,15 +232,25 @@ def test_binary_nonuniform_box_vs_oracle(shape, order, nsteps):
     b.close()
 
 
-def test_staged_calls_equal_fused_step():
-    """The reference's call sequence through the C ABI equals ldw_step_binary."""
-    shape = (16, 16, 16)
-    b1 = L.BinaryFluid(shape, tau=1.0)
+@pytest.mark.parametrize("shape,conserve,precision,order", [
+    ((16, 16, 16), 0, 64, 1),
+    ((20, 16, 24), 1, 64, 2),      # compensated update in the single-pass step
+    ((20, 16, 24), 2, 64, 3),      # global subtraction after the single-pass step
+    ((20, 16, 24), 1, 32, 1),
+    ((20, 16, 24), 2, 32, 2),
+])
+def test_staged_calls_equal_fused_step(shape, conserve, precision, order):
+    """The reference's call sequence through the C ABI equals ldw_step_binary,
+    with cahn_hilliard_options_conserve 0, 1 and 2, FP64 and FP32."""
+    print(f"case staged vs fused {shape} conserve={conserve} fp{precision} order={order}", flush=True)
+    kw = dict(tau=1.0, conserve=conserve, precision=precision, order=order)
+    nsteps = 3 if conserve == 0 else 6
+    b1 = L.BinaryFluid(shape, **kw)
     b1.init_spinodal(5)
-    b1.step(3)
-    b2 = L.BinaryFluid(shape, tau=1.0)
+    b1.step(nsteps)
+    b2 = L.BinaryFluid(shape, **kw)
     b2.init_spinodal(5)
-    for _ in range(3):
+    for _ in range(nsteps):
         b2.f_zero()
         b2.force_stage()
         b2.cahn_hilliard_stage()
