@@ -191,6 +191,24 @@ def test_le_single_pass_step_equals_staged_calls(order, stencil, conserve):
     b.close()
 
 
+def test_le_force_method_is_the_flux_form():
+    """With planes the reference takes the flux form of the force whatever
+    fe_force_method says (phi_force.c:89-97): phi_gradmu and stress
+    divergence give the same bits, over staged and single-pass steps."""
+    shape = (32, 12, 10)
+    print("case le force method 32x12x10", flush=True)
+    out = []
+    for method in (L.FORCE_STRESS_DIVERGENCE, L.FORCE_PHI_GRADMU):
+        b = L.BinaryFluid(shape, tau=0.9, mobility=0.2, force_method=method, le_planes=2, le_uy=0.11)
+        b.init_spinodal(31, 0.0, 0.1)
+        b.le_init_shear_profile()
+        b.step(6)
+        out.append((b.phi_field(), b.f()))
+        b.close()
+    assert_exact(out[1][0], out[0][0], "phi")
+    assert_exact(out[1][1], out[0][1], "f")
+
+
 def test_le_conservation_and_shear_profile():
     """64 x 32 x 32 with 4 planes: sums of rho and phi stay put to rounding
     over 60 steps; with phi = 0 (no thermodynamic force) the steady shear
