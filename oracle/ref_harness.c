@@ -28,8 +28,9 @@
  *    lebc     N seed                     lb_data_apply_le_boundary_conditions
  *                                        on random post-collision f
  *
- *  Lees-Edwards planes: LDW_HARNESS_LE="nplanes uy" creates the planes
- *  (steady shear, lees_edw_create leesedwards.c:81); LDW_HARNESS_T0=k
+ *  Lees-Edwards planes: LDW_HARNESS_LE="nplanes uy [nt0]" creates the
+ *  planes (steady shear, lees_edw_create leesedwards.c:81; nt0 the
+ *  LE_time_offset, default 0); LDW_HARNESS_T0=k
  *  advances the physics time by k steps before a single-call case runs
  *  (the plane displacement depends on it); LDW_HARNESS_LE_PROFILE=1 starts
  *  the binary case from lb_le_init_shear_profile (model_le.c:649).  The
@@ -167,7 +168,8 @@ static void sys_create(sys_t * s, int n, int nhalo, lb_relaxation_enum_t relax,
     const char * le = getenv("LDW_HARNESS_LE");
     if (le) {
       opts.type = LE_SHEAR_TYPE_STEADY;
-      if (sscanf(le, "%d %lf", &opts.nplanes, &opts.uy) != 2) usage();
+      const int nread = sscanf(le, "%d %lf %d", &opts.nplanes, &opts.uy, &opts.nt0);
+      if (nread < 2) usage();
     }
     lees_edw_create(s->pe, s->cs, &opts, &s->le);
   }

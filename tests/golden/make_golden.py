@@ -123,6 +123,10 @@ CASES = {
                            ["phi", "rho", "u", "csum"], ["f"],
                            {"LDW_HARNESS_LE": "2 0.05", "LDW_HARNESS_CONSERVE": "1",
                             "OMP_NUM_THREADS": "1"}),
+    # LE_time_offset 7: the buffers move with t - 7, the Cahn-Hilliard faces with t
+    "le_binary16_t0_s12": (["binary", 16, 12, 99, 2, A, B, K, MS, 0.8, 0.0, 0.1], 16,
+                           ["phi", "rho", "u"], ["f"],
+                           {"LDW_HARNESS_LE": "2 0.15 7", "LDW_HARNESS_LE_PROFILE": "1"}),
 }
 
 NSAMPLE = 256
@@ -155,9 +159,10 @@ def main():
                 out["stencil"] = np.array(int(spec[4].get("LDW_HARNESS_STENCIL", 7)))
                 out["conserve"] = np.array(int(spec[4].get("LDW_HARNESS_CONSERVE", 0)))
                 if "LDW_HARNESS_LE" in spec[4]:
-                    nplane, uy = spec[4]["LDW_HARNESS_LE"].split()
-                    out["le_planes"] = np.array(int(nplane))
-                    out["le_uy"] = np.array(float(uy))
+                    le = spec[4]["LDW_HARNESS_LE"].split()
+                    out["le_planes"] = np.array(int(le[0]))
+                    out["le_uy"] = np.array(float(le[1]))
+                    out["le_nt0"] = np.array(int(le[2]) if len(le) > 2 else 0)
                     out["le_t0"] = np.array(int(spec[4].get("LDW_HARNESS_T0", 0)))
                     out["le_profile"] = np.array(int(spec[4].get("LDW_HARNESS_LE_PROFILE", 0)))
             out["sample_sites"] = samples

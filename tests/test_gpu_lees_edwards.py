@@ -6,7 +6,7 @@ N_LE_plane planes of steady shear: the gradient and the flux-form force at
 a time step where the planes have slid 2.96 sites, one Cahn-Hilliard update
 for advection orders 1-3, the LB plane conditions on random distributions,
 and whole step loops (shear-profile start, 27-point gradient, three planes,
-conserve 1).  The bar is bit-exact equality.
+conserve 1, an LE_time_offset of 7).  The bar is bit-exact equality.
 
 On non-cubic lattices with up to four planes the GPU step loop matches
 the CPU oracle's restatement of the plane code (oracle/le_oracle.c, itself
@@ -42,7 +42,8 @@ def assert_exact(got, want, what):
 
 
 def _le(g):
-    return dict(le_planes=int(g["le_planes"]), le_uy=float(g["le_uy"]))
+    nt0 = int(g["le_nt0"]) if "le_nt0" in g else 0
+    return dict(le_planes=int(g["le_planes"]), le_uy=float(g["le_uy"]), le_nt0=nt0)
 
 
 @pytest.mark.parametrize("case,stencil", [("le_force16", 7), ("le_force16_g27", 27)])
@@ -94,7 +95,8 @@ def test_le_lb_boundary_conditions_golden(golden):
 
 
 @pytest.mark.parametrize("case", ["le_binary16_o1_s20", "le_binary16_o3_g27_s10",
-                                  "le_binary24_p3_o2_s12", "le_binary16_c1_s10"])
+                                  "le_binary24_p3_o2_s12", "le_binary16_c1_s10",
+                                  "le_binary16_t0_s12"])
 def test_le_binary_step_golden(golden, case):
     g = golden(case)
     args = [str(a) for a in g["args"]]

@@ -38,7 +38,8 @@ def assert_exact(got, want, what):
 
 
 def _le(g, nhalo=2):
-    return O.LeParam(int(g["le_planes"]), nhalo, float(g["le_uy"]), 0)
+    nt0 = int(g["le_nt0"]) if "le_nt0" in g else 0
+    return O.LeParam(int(g["le_planes"]), nhalo, float(g["le_uy"]), nt0)
 
 
 @pytest.mark.parametrize("case,stencil", [("le_force16", 7), ("le_force16_g27", 27)])
@@ -70,7 +71,8 @@ def test_le_lb_bc_golden(golden):
 
 
 @pytest.mark.parametrize("case", ["le_binary16_o1_s20", "le_binary16_o3_g27_s10",
-                                  "le_binary24_p3_o2_s12", "le_binary16_c1_s10"])
+                                  "le_binary24_p3_o2_s12", "le_binary16_c1_s10",
+                                  "le_binary16_t0_s12"])
 def test_le_binary_steps_golden(golden, case):
     g = golden(case)
     args = [str(a) for a in g["args"]]
