@@ -76,7 +76,8 @@ def run_driver(name, tmp_path):
 @pytest.mark.parametrize("name", ["serial-dist-3du", "serial-spin-fd1", "spin7-o1", "spin7-o3",
                                   "c2-m025", "gradmu-trt", "single-bgk", "spin7-c1", "spin7-c2",
                                   "serial-le3d-st1", "serial-le3d-st2", "serial-le3d-st3",
-                                  "serial-le3d-st5", "serial-le3d-st6", "serial-le3d-st7"])
+                                  "serial-le3d-st5", "serial-le3d-st6", "serial-le3d-st7",
+                                  "le-t0"])
 def test_driver_matches_reference_log(name, tmp_path):
     got = run_driver(name, tmp_path)
     want = open(os.path.join(LOGS, name + ".ref")).read()
