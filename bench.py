@@ -10,17 +10,22 @@ noise, rho = 1, u = 0.  Inputs are resident in HBM before the timed region.
 
 Prints one JSON line:
   value        MLUPS = sites * steps / wall time of the timed region
-  roofline     for the dominant kernel (binary_step_kernel): algorithmic bytes
-               per launch (376 B/site in FP64) / average kernel duration from
-               HIP events on the library's stream; peak 8000 GB/s; traffic =
-               HBM bytes per launch measured with rocprofv3 counters
+  roofline     for the dominant kernel (binary_step_kernel, one launch per
+               step): algorithmic bytes per launch (376 B/site in FP64) /
+               its average duration, from HIP events on the library's stream
+               around the timed region divided by the launches in it; peak
+               8000 GB/s; traffic = HBM bytes per launch measured with
+               rocprofv3 FETCH_SIZE/WRITE_SIZE counters
   cpu_baseline the reference's own serial/OpenMP build (oracle/_ref/Ludwig.exe,
-               compiled from the reference sources) on the host cores, or the
-               CPU oracle when that binary is absent
-  extra        256^3 FP64 and FP32-shifted throughput, per-kernel times; the
-               binary fluid at 128^3 with two Lees-Edwards planes; the
-               256-replica uneven-size ensemble (FP32) throughput; the
-               surrogate U-Net's BF16 MFMA inference rate (TFLOP/s)
+               compiled from the reference sources) on the host cores: 100
+               steps at 128^3 and 20 at 256^3; the CPU oracle when that binary
+               is absent
+  extra        each over at least 100 steps: 256^3 FP64 (with its own
+               roofline block) and FP32-shifted, 128^3 FP32; the binary fluid
+               at 128^3 with two Lees-Edwards planes; the 256-replica
+               uneven-size ensemble (FP32); the surrogate: time-to-T speedup
+               and L(T) error of surrogate + LB against plain LB on the
+               held-out replicas, and the U-Net's BF16 MFMA rate
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
 """
@@ -57,7 +62,19 @@ C2 = dict(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25, tau=1.0, order=1)
 SEED = 8361235
 
 
-def run_gpu(n, steps, warmup, precision, timing_steps):
+def _elapsed(lib, a, b):
+    """Milliseconds between stream marks a and b (HIP events)."""
+    import ctypes
+    import ludwig_amd as L
+    ms = ctypes.c_double()
+    L._ck(lib.ldw_stream_elapsed(a, b, ctypes.byref(ms)), "ldw_stream_elapsed")
+    return ms.value
+
+
+def run_gpu(n, steps, warmup, precision, breakdown_steps=20):
+    """`steps` timed steps after `warmup`: wall time, and HIP events on the
+    library's stream around the same region; then a per-kernel breakdown
+    (events around every launch) over `breakdown_steps` more steps."""
     import ludwig_amd as L
     b = L.BinaryFluid((n, n, n), precision=precision, fatal=True, **C2)
     b.init_spinodal(SEED, 0.0, 0.1)
@@ -65,17 +82,20 @@ def run_gpu(n, steps, warmup, precision, timing_steps):
     b.step(warmup)
     b.synchronize()
     t0 = time.perf_counter()
+    lib.ldw_stream_mark(0)
     b.step(steps)
+    lib.ldw_stream_mark(1)
     b.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0
+    event_ms = _elapsed(lib, 0, 1)
     # kernel durations: HIP events around each launch, on the library stream
     lib.ldw_timer_reset()
     lib.ldw_timer_enable(1)
-    b.step(timing_steps)
+    b.step(breakdown_steps)
     b.synchronize()
     lib.ldw_timer_enable(0)
-    kt = {}
+    kt, kc = {}, {}
     import ctypes
     for name in ("binary_step", "collide", "force_ch", "phi_force_mu", "cahn_hilliard", "mu", "propagate"):
         ms = ctypes.c_double()
@@ -83,15 +103,43 @@ def run_gpu(n, steps, warmup, precision, timing_steps):
         lib.ldw_timer_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
         if cnt.value:
             kt[name] = ms.value / cnt.value
+            kc[name] = cnt.value
     r, p = b.conserved_sums()
     ok = bool(abs(r - n ** 3) / n ** 3 < 1e-6)
     b.close()
     nsite = n ** 3
     mlups = nsite * steps / wall / 1e6
+    # one launch per step of a single kernel family: the region's event time
+    # per step is that kernel's average launch duration
+    single = len(kc) == 1 and list(kc.values())[0] == breakdown_steps
     return {"n": n, "precision": precision, "steps": steps, "wall_s": wall,
-            "ms_per_step": 1e3 * wall / steps, "mlups": mlups,
+            "ms_per_step": 1e3 * wall / steps, "event_ms_per_step": event_ms / steps, "mlups": mlups,
             "hbm_frac_step": mlups * 1e6 * BYTES_SITE[precision]["step"] / (HBM_PEAK_GBS * 1e9),
-            "kernel_ms": kt, "mass_ok": ok}
+            "kernel_ms": kt, "single_kernel": list(kc)[0] if single else None, "mass_ok": ok}
+
+
+def roofline(r, traffic_b_site=None):
+    """The roofline block of a run_gpu() result: the dominant kernel, its
+    algorithmic bytes per launch over its average launch duration."""
+    kms = r["kernel_ms"]
+    if not kms:
+        return None
+    name = r["single_kernel"] or max(kms, key=kms.get)
+    # a single kernel per step: its launches fill the timed region, whose
+    # event time per step is the average launch duration (it cannot exceed
+    # the wall time per step); otherwise the per-launch events
+    ms = r["event_ms_per_step"] if r["single_kernel"] else kms[name]
+    nsite = r["n"] ** 3
+    bps = BYTES_SITE[r["precision"]][name]
+    achieved = bps * nsite / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_b_site * nsite if traffic_b_site else None,
+            "kernel": KERNEL_DESC.get(name, name),
+            "algorithmic_bytes_per_site": bps, "avg_kernel_ms": ms,
+            "timing": ("HIP events on the library stream around the timed region / launches"
+                       if r["single_kernel"] else "HIP events around each launch"),
+            "launches": r["steps"] if r["single_kernel"] else None}
 
 
 def run_ensemble(nrep, steps, warmup, precision=32, seed=2026):
@@ -224,12 +272,21 @@ def run_surrogate_conv(n=64, batch=8, reps=5):
             "d0_layer_frac_bf16_peak": fl / dl / 1e12 / BF16_PEAK_TFLOPS}
 
 
-def cpu_baseline(n=128, steps=10):
-    """Time the reference's CPU build on this host, bounded to ~10-30 s."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    exe = os.path.join(HERE, "oracle", "_ref", "Ludwig.exe")
-    if os.path.exists(exe):
-        inp = f"""N_cycles {steps}
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fp:
+            for line in fp:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _ludwig_exe(exe, n, steps, threads):
+    """One run of the reference's Ludwig.exe on the C2 input at n^3:
+    (seconds in its 'Time step loop' timer, steps timed)."""
+    inp = f"""N_cycles {steps}
 size {n}_{n}_{n}
 grid 1_1_1
 viscosity 0.16666666666666667
@@ -249,23 +306,40 @@ periodicity 1_1_1
 freq_statistics {steps}
 config_at_end no
 """
-        with tempfile.TemporaryDirectory() as tmp:
-            with open(os.path.join(tmp, "input"), "w") as fp:
-                fp.write(inp)
-            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
-                       OMP_PLACES="cores")
-            out = subprocess.run([exe], cwd=tmp, env=env, capture_output=True, text=True,
-                                 timeout=600).stdout
-        m = re.search(r"Time step loop:\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)\s+\((\d+) calls\)", out)
-        if m:
-            total, calls = float(m.group(3)), int(m.group(5))
-            return {"value": n ** 3 * calls / total / 1e6, "unit": "MLUPS", "cores": threads,
-                    "kind": "reference",
-                    "sample": f"reference Ludwig.exe (gcc -O2 -fopenmp -DNDEBUG), {n}^3 binary "
-                              f"fluid, {calls} steps, 'Time step loop' timer, {threads} OpenMP threads"}
+    with tempfile.TemporaryDirectory() as tmp:
+        with open(os.path.join(tmp, "input"), "w") as fp:
+            fp.write(inp)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+                   OMP_PLACES="cores")
+        out = subprocess.run([exe], cwd=tmp, env=env, capture_output=True, text=True,
+                             timeout=900).stdout
+    m = re.search(r"Time step loop:\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)\s+\((\d+) calls\)", out)
+    if not m:
+        raise RuntimeError(f"Ludwig.exe {n}^3: no 'Time step loop' timer in its output")
+    return float(m.group(3)), int(m.group(5))
+
+
+def cpu_baseline(runs=((128, 100), (256, 20))):
+    """Time the reference's CPU build on this host's cores: the C2 input at
+    each (size, steps) of `runs`; the value is the first run's MLUPS."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    exe = os.path.join(HERE, "oracle", "_ref", "Ludwig.exe")
+    cpu = _cpu_model()
+    if os.path.exists(exe):
+        rows = []
+        for n, steps in runs:
+            t, calls = _ludwig_exe(exe, n, steps, threads)
+            rows.append({"n": n, "steps": calls, "loop_s": t, "mlups": n ** 3 * calls / t / 1e6})
+            print(f"# cpu baseline {n}^3: {rows[-1]}", file=sys.stderr, flush=True)
+        return {"value": rows[0]["mlups"], "unit": "MLUPS", "cores": threads, "kind": "reference",
+                "cpu_model": cpu, "runs": rows,
+                "sample": "reference Ludwig.exe (gcc -O2 -fopenmp -DNDEBUG), binary fluid "
+                          + ", ".join(f"{r['n']}^3 x {r['steps']} steps" for r in rows)
+                          + f", 'Time step loop' timer, {threads} OpenMP threads"
+                            " (OMP_PROC_BIND=close, OMP_PLACES=cores)"}
     # fall back to the C restatement
-    import numpy as np
     import oracle as O
+    n, steps = runs[0][0], 10
     shape = (n, n, n)
     f0 = O.equilibrium(shape, rho_uniform=1.0)
     phi0 = O.spinodal(shape, SEED, 0.0, 0.1)
@@ -273,7 +347,22 @@ config_at_end no
     O.binary_steps(shape, steps, O.lb_param(tau=1.0), O.fe_param(), f0, phi0)
     dt = time.perf_counter() - t0
     return {"value": n ** 3 * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "cpu_model": cpu,
             "sample": f"C oracle (gcc -O2 -fopenmp), {n}^3 binary fluid, {steps} steps"}
+
+
+def run_surrogate_gain():
+    """Surrogate + LB against plain LB to the same time T on the held-out
+    replicas with the shipped weights: 64^3 replicas (the stated
+    configuration) and every held-out size; 20 and 30 correction steps."""
+    from ludwig_amd import surrogate_study as S
+    if not os.path.exists(S.WEIGHTS):
+        return {"status": f"{os.path.basename(S.WEIGHTS)} absent"}
+    out = {"held_out_64^3": S.gain(size=64, corrs=(20, 30))}
+    print(f"# surrogate 64^3: {out['held_out_64^3']}", file=sys.stderr, flush=True)
+    out["held_out_all_sizes"] = S.gain(size=None, corrs=(20,))
+    print(f"# surrogate all sizes: {out['held_out_all_sizes']}", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -337,40 +426,36 @@ def main():
               "(the multi-GPU decomposition is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
-    main_run = run_gpu(args.size, args.steps, args.warmup, 64, min(args.steps, 50))
+    xsteps = max(100, args.steps)          # every extra over >= 100 steps
+    main_run = run_gpu(args.size, args.steps, args.warmup, 64)
     print(f"# main: {main_run}", file=sys.stderr, flush=True)
     extra = {}
     if not args.no_extra:
         for n, prec in ((256, 64), (256, 32), (128, 32)):
-            r = run_gpu(n, max(20, args.steps // 4), 5, prec, 20)
-            extra[f"{n}^3_fp{prec}"] = {k: r[k] for k in ("mlups", "ms_per_step", "hbm_frac_step",
+            r = run_gpu(n, xsteps, 10, prec)
+            extra[f"{n}^3_fp{prec}"] = {k: r[k] for k in ("steps", "mlups", "ms_per_step",
+                                                          "event_ms_per_step", "hbm_frac_step",
                                                           "kernel_ms", "mass_ok")}
+            extra[f"{n}^3_fp{prec}"]["roofline"] = roofline(
+                r, TRAFFIC_B_SITE.get(("binary_step", prec, n)))
             print(f"# extra {n} fp{prec}: {r}", file=sys.stderr, flush=True)
+        e64, e32 = extra["256^3_fp64"], extra["256^3_fp32"]
+        extra["256^3_fp32_speedup_over_fp64"] = {
+            "wall": e64["ms_per_step"] / e32["ms_per_step"],
+            "events": e64["event_ms_per_step"] / e32["event_ms_per_step"]}
 
-        r = run_le()
+        r = run_le(steps=xsteps)
         extra["lees_edwards_128^3_2planes_fp64"] = r
         print(f"# lees-edwards: {r}", file=sys.stderr, flush=True)
-        r = run_ensemble(256, max(20, args.steps // 4), 5)
+        r = run_ensemble(256, xsteps, 5)
         extra["ensemble_256_replicas_fp32"] = r
         print(f"# ensemble: {r}", file=sys.stderr, flush=True)
+        extra["surrogate_gain"] = run_surrogate_gain()
         r = run_surrogate_conv()
         extra["surrogate_unet_inference_bf16_mfma"] = r
         print(f"# surrogate conv: {r}", file=sys.stderr, flush=True)
 
-    nsite = args.size ** 3
-    roof = None
-    kms = main_run["kernel_ms"]
-    if kms:
-        name = max(kms, key=kms.get)          # dominant kernel
-        ms = kms[name]
-        achieved = BYTES_SITE[64][name] * nsite / (ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS,
-                "traffic": (TRAFFIC_B_SITE[(name, 64, args.size)] * nsite
-                            if (name, 64, args.size) in TRAFFIC_B_SITE else None),
-                "kernel": KERNEL_DESC.get(name, name),
-                "algorithmic_bytes_per_site": BYTES_SITE[64][name],
-                "avg_kernel_ms": ms}
+    roof = roofline(main_run, TRAFFIC_B_SITE.get(("binary_step", 64, args.size)))
     cpu = None if args.no_cpu else cpu_baseline()
 
     line = {
@@ -391,6 +476,7 @@ def main():
                                f"kappa=0.04, M=0.25 (stated 0.45 is unstable), advection order 1",
                    "lattice": [args.size] * 3, "precision": "fp64"},
         "hbm_frac_step": main_run["hbm_frac_step"],
+        "event_ms_per_step": main_run["event_ms_per_step"],
         "kernel_ms": main_run["kernel_ms"],
         "roofline": roof,
         "cpu_baseline": cpu,

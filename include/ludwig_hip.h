@@ -497,6 +497,12 @@ int ldw_timer_enable(int on);
 int ldw_timer_reset(void);
 int ldw_timer_read(const char * name, double * ms, int * count);
 
+/* Region timing on the library's stream: ldw_stream_mark(slot) records an
+ * event (slots 0..7); ldw_stream_elapsed(a, b, &ms) waits for slot b and
+ * returns the time from slot a to slot b. */
+int ldw_stream_mark(int slot);
+int ldw_stream_elapsed(int a, int b, double * ms);
+
 /* Bandwidth calibration: mean time per launch of a plain device copy of
  * nbytes (read nbytes + write nbytes) over reps launches */
 int ldw_copy_calibration(long long nbytes, int reps, double * ms);

@@ -144,6 +144,11 @@ def lib():
         "field_grad_create": [vp, vp, ctypes.c_int, pvp], "field_grad_free": [vp],
         "field_grad_compute": [vp], "field_grad_memcpy": [vp, ctypes.c_int],
         "field_grad_set": [vp, vp, vp],
+        "field_grad_scalar_grad": [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+        "field_grad_scalar_delsq": [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+        "fe_symm_mu": [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+        "fe_symm_fed": [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+        "fe_symm_str": [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
         "hydro_create": [vp, vp, vp, ctypes.POINTER(HydroOptions), pvp], "hydro_free": [vp],
         "hydro_f_zero": [vp, ctypes.POINTER(ctypes.c_double)],
         "hydro_u_zero": [vp, ctypes.POINTER(ctypes.c_double)],
@@ -163,6 +168,8 @@ def lib():
         "ldw_conserved_sums": [vp, vp, ctypes.POINTER(ctypes.c_double)],
         "ldw_timer_enable": [ctypes.c_int], "ldw_timer_reset": [],
         "ldw_timer_read": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)],
+        "ldw_stream_mark": [ctypes.c_int],
+        "ldw_stream_elapsed": [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
         "ldw_lb_f_read": [vp, dp], "ldw_lb_f_write": [vp, dp],
         "ldw_field_read": [vp, dp], "ldw_field_write": [vp, dp],
         "ldw_hydro_read": [vp, dp, dp, dp], "ldw_hydro_write": [vp, dp, dp, dp],
@@ -543,6 +550,38 @@ class BinaryFluid(_Base):
         _ck(lib().phi_cahn_hilliard(self.pch, self.fe, self.phi,
                                     self.hydro if with_hydro else None, self.map, None),
             "phi_cahn_hilliard")
+
+    def gradient_readback(self):
+        """The reference's host read-back after field_grad_compute():
+        field_grad_memcpy(D2H), then field_grad_scalar_grad/_delsq and
+        fe_symm_mu/_fed/_str at every interior site (cs_index addressing).
+        Returns grad (3, nsite), delsq, mu, fed (nsite) and str (9, nsite),
+        sites z fastest.  One call per site: for small lattices."""
+        L = lib()
+        _ck(L.field_memcpy(self.phi, D2H), "field_memcpy")
+        _ck(L.field_grad_memcpy(self.dphi, D2H), "field_grad_memcpy")
+        nx, ny, nz = self.shape
+        grad = np.empty((3, self.nsite))
+        out = {k: np.empty(self.nsite) for k in ("delsq", "mu", "fed")}
+        st = np.empty((9, self.nsite))
+        g3 = (ctypes.c_double * 3)()
+        d = ctypes.c_double()
+        s9 = (ctypes.c_double * 9)()
+        s = 0
+        for i in range(nx):
+            for j in range(ny):
+                for k in range(nz):
+                    idx = L.cs_index(self.cs, i + 1, j + 1, k + 1)
+                    _ck(L.field_grad_scalar_grad(self.dphi, idx, g3), "field_grad_scalar_grad")
+                    grad[:, s] = g3[:]
+                    for name, fn in (("delsq", L.field_grad_scalar_delsq), ("mu", L.fe_symm_mu),
+                                     ("fed", L.fe_symm_fed)):
+                        _ck(fn(self.dphi if name == "delsq" else self.fe, idx, ctypes.byref(d)), name)
+                        out[name][s] = d.value
+                    _ck(L.fe_symm_str(self.fe, idx, s9), "fe_symm_str")
+                    st[:, s] = s9[:]
+                    s += 1
+        return grad, out["delsq"], out["mu"], out["fed"], st
 
     def phi_field(self):
         out = np.empty(self.nsite)

@@ -252,6 +252,31 @@ extern "C" int ldw_timer_read(const char * name, double * ms, int * count) {
   return 0;
 }
 
+/* Events on the library's stream around a whole region of launches: mark
+ * slot a before, slot b after, then the elapsed time between them.  Every
+ * launch the region makes is on that stream, so this is the device time of
+ * the region, gaps between launches included. */
+static hipEvent_t g_marks[8] = {nullptr};
+
+extern "C" int ldw_stream_mark(int slot) {
+  if (slot < 0 || slot >= 8) FAIL("ldw_stream_mark: slot %d outside 0..7", slot);
+  if (device_init()) return -1;
+  if (!g_marks[slot]) HIP_CHECK(hipEventCreate(&g_marks[slot]));
+  HIP_CHECK(hipEventRecord(g_marks[slot], g_stream));
+  return 0;
+}
+
+extern "C" int ldw_stream_elapsed(int a, int b, double * ms) {
+  if (a < 0 || a >= 8 || b < 0 || b >= 8 || !g_marks[a] || !g_marks[b]) {
+    FAIL("ldw_stream_elapsed: slots %d, %d not both marked", a, b);
+  }
+  float t = 0.0f;
+  HIP_CHECK(hipEventSynchronize(g_marks[b]));
+  HIP_CHECK(hipEventElapsedTime(&t, g_marks[a], g_marks[b]));
+  *ms = t;
+  return 0;
+}
+
 static int dmalloc(void ** p, size_t bytes) {
   if (device_init()) return -1;
   HIP_CHECK(hipMalloc(p, bytes));
@@ -2201,6 +2226,12 @@ extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_
     g_le_fused = (e && atoi(e) == 0) ? 0 : 1;
   }
   fe_symm_t * fs = as_symm(fe);
+  // refused on the host before any launch: the conserve-1 compensation
+  // buffer is sized for the precision phi_ch_t was created with, and every
+  // path reads it with the precision of phi
+  if (pch && pch->info.conserve && pch->prec != phi->prec) {
+    FAIL("ldw_step_binary: phi_ch_t (%d-bit) and phi (%d-bit) differ in precision", pch->prec, phi->prec);
+  }
   physics_t * phys = nullptr;
   physics_ref(&phys);
   for (int step = 0; step < nsteps; step++) {
@@ -2217,6 +2248,7 @@ extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_
       && le_active(phi->le) && hydro->le == phi->le && fs->phi == phi
       && pth->method != FE_FORCE_METHOD_NO_FORCE && lb->streamed
       && lb->prec == 64 && phi->prec == 64 && hydro->prec == 64
+      && (pch->info.conserve == 0 || pch->prec == 64)
       && phi->geo.nx/phi->le->nplanes >= 3;
     if (fusable) {
       if (step_fused(lb, hydro, phi, fs, pth, pch)) return -1;

@@ -114,18 +114,55 @@ class UNet:
     def grads(self):
         return self.g.download()
 
-    def save(self, path):
-        """Weights and optimiser state (numpy .npz)."""
+    def save(self, path, norm=None):
+        """Weights and optimiser state (numpy .npz), with the normaliser the
+        weights were trained under when it is given."""
+        extra = {} if norm is None else {"norm_" + k: np.array(v) for k, v in norm.to_dict().items()}
         np.savez(path, w=self.w.download(), m=self.m.download(), v=self.v.download(),
-                 step=np.array(self.step_count))
+                 step=np.array(self.step_count), **extra)
+
+    def save_inference(self, path, norm):
+        """What inference needs, compactly: the convolution weights as BF16
+        (round to nearest even, the rounding the MFMA copies get anyway),
+        biases in FP32, and the normaliser.  Loading it reproduces the
+        network's outputs exactly."""
+        w = self.w.download()
+        keep = np.ones(self.nparam, dtype=bool)
+        for name, ci, co in LAYERS:
+            o = self.layout[name][0]
+            keep[o + 27 * co * ci:o + 27 * co * ci + co] = False
+        u = w[keep].view(np.uint32).astype(np.uint64)
+        bits = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+        np.savez_compressed(path, w_bf16=bits, bias=w[~keep], step=np.array(self.step_count),
+                            **{"norm_" + k: np.array(v) for k, v in norm.to_dict().items()})
 
     def load(self, path):
+        """Weights from save() or save_inference(); returns the normaliser
+        stored with them, or None.  An inference file resets the optimiser
+        state."""
         d = np.load(path)
-        if d["w"].size != self.nparam:
-            raise ValueError(f"{path}: {d['w'].size} parameters, the network has {self.nparam}")
-        self.w.upload(d["w"]); self.m.upload(d["m"]); self.v.upload(d["v"])
+        if "w_bf16" in d:
+            keep = np.ones(self.nparam, dtype=bool)
+            for name, ci, co in LAYERS:
+                o = self.layout[name][0]
+                keep[o + 27 * co * ci:o + 27 * co * ci + co] = False
+            if d["w_bf16"].size != int(keep.sum()) or d["bias"].size != int((~keep).sum()):
+                raise ValueError(f"{path}: parameter counts do not match the network")
+            w = np.zeros(self.nparam, dtype=np.float32)
+            w[keep] = (d["w_bf16"].astype(np.uint32) << 16).view(np.float32)
+            w[~keep] = d["bias"]
+            self.w.upload(w)
+            z = np.zeros(self.nparam, dtype=np.float32)
+            self.m.upload(z); self.v.upload(z)
+        else:
+            if d["w"].size != self.nparam:
+                raise ValueError(f"{path}: {d['w'].size} parameters, the network has {self.nparam}")
+            self.w.upload(d["w"]); self.m.upload(d["m"]); self.v.upload(d["v"])
         self.step_count = int(d["step"])
         self.prepare()
+        if "norm_u_scale" in d:
+            return Normaliser(**{k[5:]: float(d[k]) for k in d.files if k.startswith("norm_")})
+        return None
 
     def grad_norm(self):
         out = ctypes.c_double()

@@ -28,6 +28,7 @@ Run as a script for the full protocol; prints one JSON object.
 
 import argparse
 import json
+import os
 import time
 
 import numpy as np
@@ -49,9 +50,11 @@ def _sync():
     lib().ldw_device_synchronize()
 
 
-def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print, variants=None):
+def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print, variants=None,
+             control=True, identity=True):
     """Plain LB, surrogate + LB and identity + LB on `specs` from the LB
-    state at t_start."""
+    state at t_start.  control=False skips the FP64 control run and
+    identity=False the identity jumps."""
     T = jumps * (every + corr)
     out = {"T": T, "t_start": t_start, "replicas": len(specs)}
     plain = Ensemble(specs, precision=32, fatal=True)
@@ -66,10 +69,11 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print, variant
     plain.close()
     # control: the same plain run in FP64 storage -- how far two faithful
     # lattice Boltzmann runs drift apart on this chaotic problem
-    ctl = Ensemble(specs, precision=64, fatal=True)
-    ctl.step(t_start + T)
-    ref64 = [ctl.snapshot(r)[0] for r in range(len(specs))]
-    ctl.close()
+    if control:
+        ctl = Ensemble(specs, precision=64, fatal=True)
+        ctl.step(t_start + T)
+        ref64 = [ctl.snapshot(r)[0] for r in range(len(specs))]
+        ctl.close()
 
     def jumped(with_net, u_mode="pred", conserve=False):
         e = Ensemble(specs, precision=32, fatal=True)
@@ -98,8 +102,10 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print, variant
     runs = {}
     for name, (u_mode, conserve) in variants.items():
         runs[name], out[name + "_s"] = jumped(True, u_mode, conserve)
-    runs["identity"], out["identity_s"] = jumped(False)
-    runs["fp64_control"] = ref64
+    if identity:
+        runs["identity"], out["identity_s"] = jumped(False)
+    if control:
+        runs["fp64_control"] = ref64
     names = tuple(runs)
     rows = []
     for r, s in enumerate(specs):
@@ -128,6 +134,46 @@ def evaluate(net, norm, specs, every, jumps, corr, t_start=0, log=print, variant
     return out
 
 
+# The trained network shipped with the package: BF16 weights, FP32 biases
+# and the normaliser, from run(..., save_to=...) (save_inference).
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "surrogate_unet.npz")
+
+
+def heldout_specs(nrep=256, seed=2026, size=None):
+    """The held-out 20 % of the training ensemble's replicas (split by seed,
+    as run() splits them), optionally only those of one lattice size."""
+    specs = draw_replicas(nrep, seed=seed)
+    _, val_idx = split_by_seed(specs, 0.2, seed)
+    return [specs[i] for i in val_idx if size is None or specs[i].nx == size]
+
+
+def gain(weights=WEIGHTS, size=64, corrs=(20, 30), every=250, jumps=8, t_start=500,
+         conserve=True, log=None):
+    """Surrogate + LB against plain LB on the held-out replicas (all sizes
+    with size=None): for each number of correction steps, the time-to-T
+    speedup and the L(T) error per replica (mean, max) and of the ensemble
+    mean, with T = jumps * (every + corr)."""
+    net = UNet(seed=0)
+    norm = net.load(weights)
+    if norm is None:
+        raise ValueError(f"{weights}: no normaliser stored with the weights")
+    specs = heldout_specs(size=size)
+    out = {"weights": os.path.basename(weights), "replicas": len(specs),
+           "sizes": sorted({s.nx for s in specs}), "jumps": jumps, "every": every, "t_start": t_start,
+           "sums_restored": conserve}
+    for c in corrs:
+        ev = evaluate(net, norm, specs, every, jumps, c, t_start=t_start,
+                      log=log or (lambda m: None), variants={"surrogate": ("pred", conserve)},
+                      control=False, identity=False)
+        out[f"corr{c}"] = {"T": ev["T"], "speedup": ev["speedup"], "plain_s": ev["plain_s"],
+                           "surrogate_s": ev["surrogate_s"],
+                           "L_err_per_replica_mean": ev["L_err_surrogate_mean"],
+                           "L_err_per_replica_max": ev["L_err_surrogate_max"],
+                           "L_err_ensemble_mean": ev["meanL_err_surrogate"],
+                           "phi_rms_err_mean": ev["phi_rms_err_surrogate_mean"]}
+    return out
+
+
 def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=0, iters=2000,
         batch=4, lr=5e-4, jumps=8, corr=30, seed=2026, max_eval=None, train_seconds=None,
         t_start=500, replica_kw=None, init_from=None, save_to=None, log=print):
@@ -145,7 +191,9 @@ def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=0, it
     norm = data.normaliser(sample=min(32, len(data.pairs(kmin=kmin))), kmin=kmin)
     net = UNet(seed=seed)
     if init_from:
-        net.load(init_from)
+        stored = net.load(init_from)
+        if stored is not None:
+            norm = stored
         log(f"weights from {init_from} (after {net.step_count} steps)")
     tr = Trainer(net, norm, batch=batch)
     tpairs = [(r, k) for (r, k) in data.pairs(kmin=kmin) if r in set(train_idx)]
@@ -162,7 +210,9 @@ def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=0, it
         f"held-out loss {val_loss:.4e} (zero-change prediction {zero_loss:.4e})")
     data.free()
     if save_to:
-        net.save(save_to)
+        net.save(save_to, norm)
+        net.save_inference(save_to[:-4] + "_inference.npz" if save_to.endswith(".npz")
+                           else save_to + "_inference", norm)
     # eval_size 0: every held-out replica
     ev_specs = [specs[i] for i in val_idx if specs[i].nx == eval_size or not eval_size] \
         or [specs[i] for i in val_idx]

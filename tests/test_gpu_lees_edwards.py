@@ -62,6 +62,57 @@ def test_le_force_golden(golden, case, stencil):
     b.close()
 
 
+@pytest.mark.parametrize("case,stencil", [("le_force16", 7), ("le_force16_g27", 27)])
+def test_le_gradient_readback_golden(golden, case, stencil):
+    """With planes, field_grad_memcpy + field_grad_scalar_grad/_delsq and
+    fe_symm_mu read back the reference's grad, delsq and mu bit for bit."""
+    g = golden(case)
+    n = int(g["n"])
+    a, b_, k = map(float, [str(x) for x in g["args"]][4:7])
+    print(f"case {case} read-back", flush=True)
+    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, tau=1.0, stencil=stencil, **_le(g))
+    b.set_phi(g["phi"])
+    b.next_step(int(g["le_t0"]))
+    b.gradient_stage()
+    grad, delsq, mu, _, _ = b.gradient_readback()
+    assert_exact(grad.ravel(), g["grad"], "grad")
+    assert_exact(delsq, g["delsq"], "delsq")
+    assert_exact(mu, g["mu"], "mu")
+    b.close()
+
+
+@pytest.mark.parametrize("stencil", [7, 27])
+def test_le_gradient_readback_after_single_pass_step(stencil):
+    """After the single-pass step with planes, the gradient read-back is
+    formed on demand from the step's starting phi and buffer planes; it
+    equals field_grad_compute() on that phi at the same time step."""
+    shape, planes, uy, nsteps = (24, 16, 12), 2, 0.19, 6
+    print(f"case read-back after single-pass step stencil={stencil}", flush=True)
+    kw = dict(tau=0.8, mobility=0.2, order=2, stencil=stencil, le_planes=planes, le_uy=uy)
+    a = L.BinaryFluid(shape, **kw)
+    a.init_spinodal(77, 0.05, 0.1)
+    a.le_init_shear_profile()
+    a.step(nsteps)
+    got = a.gradient_readback()
+    b = L.BinaryFluid(shape, **kw)
+    b.init_spinodal(77, 0.05, 0.1)
+    b.le_init_shear_profile()
+    b.step(nsteps - 1)
+    phi_start = b.phi_field()
+    b.next_step()
+    b.gradient_stage()
+    want = b.gradient_readback()
+    for name, x, y in zip(("grad", "delsq"), got[:2], want[:2]):
+        assert_exact(x.ravel(), y.ravel(), name)
+    # fe_symm_mu combines the host phi (now the step's result) with that delsq
+    phi_now = a.phi_field()
+    assert not np.array_equal(phi_start, phi_now)
+    assert_exact(got[2], -0.0625 * phi_now + 0.0625 * phi_now * phi_now * phi_now - 0.04 * got[1],
+                 "mu")
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_le_cahn_hilliard_golden(golden, order):
     g = golden(f"le_ch16_o{order}")
@@ -123,6 +174,7 @@ def test_le_binary_step_golden(golden, case):
     ((36, 18, 10), 3, 0.05, 1, 7, 2, True, 10),
     ((20, 12, 10), 1, 0.17, 2, 27, 0, True, 8),       # one plane: one slab that wraps in x
     ((24, 8, 6), 4, 0.17, 1, 7, 1, False, 8),         # 6-deep slabs, the shallowest allowed
+    ((24, 48, 48), 2, 0.13, 2, 7, 0, True, 4),        # ny*nz = 2304: the plane sums stage 3 tiles
 ])
 def test_le_binary_vs_oracle(shape, planes, uy, order, stencil, conserve, profile, nsteps):
     """Non-cubic lattices, one to four planes, displacements that wrap in y,
@@ -252,3 +304,32 @@ def test_le_errors():
         L.BinaryFluid((16, 16, 16), tau=1.0, le_planes=3, le_uy=0.1)
     with pytest.raises(RuntimeError, match="FP64"):
         L.BinaryFluid((32, 32, 32), tau=1.0, le_planes=2, le_uy=0.1, precision=32)
+
+
+@pytest.mark.parametrize("planes", [0, 2])
+def test_conserve_precision_mismatch_refused(planes):
+    """A conserve-1 phi_ch_t created under FP32 storage beside FP64 fields
+    holds a 4-byte compensation per site; the step reads it as doubles.
+    The step loop refuses the pair on the host: no launch, the time
+    counter and phi unchanged."""
+    import ctypes
+    n = 12
+    b = L.BinaryFluid((n, n, n), tau=1.0, conserve=1, le_planes=planes, le_uy=0.01)
+    b.init_spinodal(8361235)
+    lib = L.lib()
+    L.set_precision(32)
+    try:
+        pch = ctypes.c_void_p()
+        L._ck(lib.phi_ch_create(b.pe, b.cs, b.le, ctypes.byref(L.PhiChInfo(1, 0)), ctypes.byref(pch)),
+              "phi_ch_create")
+    finally:
+        L.set_precision(64)
+    lib.phi_ch_free(b.pch)
+    b.pch = pch
+    phi0 = b.phi_field()
+    t0 = b.time()
+    with pytest.raises(RuntimeError, match="differ in precision"):
+        b.step(1)
+    assert b.time() == t0
+    assert np.array_equal(b.phi_field(), phi0)
+    b.close()

@@ -95,6 +95,51 @@ def test_force27_golden(golden, case, method):
     b.close()
 
 
+def _host_fed_str(phi, grad, delsq, a, b, kappa):
+    """fe_symm_fed and fe_symm_str (symmetric.c:285-299, 334-362) in the
+    reference's operation order, elementwise in IEEE double."""
+    gg = grad[0] * grad[0] + grad[1] * grad[1] + grad[2] * grad[2]
+    fed = (0.5 * a + 0.25 * b * phi * phi) * phi * phi + 0.5 * kappa * gg
+    p0 = 0.5 * a * phi * phi + 0.75 * b * phi * phi * phi * phi - kappa * phi * delsq - 0.5 * kappa * gg
+    st = np.empty((9,) + phi.shape)
+    for i in range(3):
+        for j in range(3):
+            st[3 * i + j] = p0 * float(i == j) + kappa * grad[i] * grad[j]
+    return fed, st
+
+
+@pytest.mark.parametrize("case,stencil", [("force_stress", 7), ("force27_stress", 27),
+                                          ("force_gradmu", 7)])
+def test_gradient_and_mu_readback_golden(golden, case, stencil):
+    """After field_grad_compute(), the host read-back the reference's
+    statistics use (field_grad_memcpy, field_grad_scalar_grad/_delsq,
+    fe_symm_mu; field_grad.c:319-339, symmetric.c:308-318) equals the
+    reference's grad, delsq and mu bit for bit; fe_symm_fed/_str equal the
+    reference's formulas evaluated on those."""
+    g = golden(case)
+    n = int(g["n"])
+    args = [str(a) for a in g["args"]]
+    a, b_, k = map(float, args[4:7])
+    print(f"case {case}", flush=True)
+    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, tau=1.0, stencil=stencil)
+    b.set_phi(g["phi"])
+    b.gradient_stage()
+    grad, delsq, mu, fed, st = b.gradient_readback()
+    assert_exact(grad.ravel(), g["grad"], "grad")
+    assert_exact(delsq, g["delsq"], "delsq")
+    assert_exact(mu, g["mu"], "mu")
+    fed_ref, st_ref = _host_fed_str(g["phi"], g["grad"].reshape(3, -1), g["delsq"], a, b_, k)
+    assert_exact(fed, fed_ref, "fed")
+    assert_exact(st.ravel(), st_ref.ravel(), "str")
+    # the read-back after a force calculation on the same phi is the same
+    b.f_zero()
+    b.force_stage()
+    grad2, delsq2, mu2, _, _ = b.gradient_readback()
+    assert_exact(grad2.ravel(), g["grad"], "grad after force")
+    assert_exact(mu2, g["mu"], "mu after force")
+    b.close()
+
+
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_cahn_hilliard_golden(golden, order):
     g = golden(f"ch_order{order}")
