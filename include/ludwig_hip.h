@@ -106,6 +106,11 @@ int physics_eta_shear(physics_t * phys, double * eta);
 int physics_eta_bulk(physics_t * phys, double * eta);
 int physics_mobility(physics_t * phys, double * mobility);
 int physics_fbody(physics_t * phys, double f[3]);
+/* External chemical potential gradient ("grad_mu"): every Cahn-Hilliard
+ * face flux loses M grad_mu (phi_cahn_hilliard.c:1373-1398); with the
+ * phi_gradmu force the fluid also feels -phi grad_mu (phi_grad_mu.c:123-160) */
+int physics_grad_mu_set(physics_t * phys, double gm[3]);      /* physics.h:37 */
+int physics_grad_mu(physics_t * phys, double gm[3]);          /* physics.h:56 */
 /* the time step counter (ludwig.c:528-531); the Lees-Edwards planes move
  * with physics_control_time() = t_start + t_current - 1 */
 int physics_control_init_time(physics_t * phys, int nstart, int nsteps); /* physics.h:34 */

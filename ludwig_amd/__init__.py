@@ -126,6 +126,7 @@ def lib():
         "physics_rho0_set": [vp, ctypes.c_double], "physics_eta_shear_set": [vp, ctypes.c_double],
         "physics_eta_bulk_set": [vp, ctypes.c_double], "physics_mobility_set": [vp, ctypes.c_double],
         "physics_fbody_set": [vp, ctypes.POINTER(ctypes.c_double)],
+        "physics_grad_mu_set": [vp, ctypes.POINTER(ctypes.c_double)],
         "lb_data_create": [vp, vp, ctypes.POINTER(LbDataOptions), pvp], "lb_free": [vp],
         "lb_memcpy": [vp, ctypes.c_int], "lb_halo": [vp], "lb_init_rest_f": [vp, ctypes.c_double],
         "lb_collision_relaxation_set": [vp, ctypes.c_int],
@@ -465,14 +466,18 @@ class BinaryFluid(_Base):
     solver and the 7-point gradient (nhalo 2)."""
 
     def __init__(self, shape, a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25,
-                 order=1, force_method=FORCE_STRESS_DIVERGENCE, stencil=7, conserve=0, **kw):
+                 order=1, force_method=FORCE_STRESS_DIVERGENCE, stencil=7, conserve=0,
+                 grad_mu=(0.0, 0.0, 0.0), **kw):
         """conserve: cahn_hilliard_options_conserve 0, 1 (compensated
         update) or 2 (global subtraction; the reference sum is taken by
-        the init_* and read_config methods)."""
+        the init_* and read_config methods).  order: advection order 1-5.
+        grad_mu: the external chemical potential gradient (input key
+        grad_mu)."""
         super().__init__(shape, 2, **kw)
         L = lib()
         vp = ctypes.c_void_p
         _ck(L.physics_mobility_set(self.phys, mobility), "physics_mobility_set")
+        _ck(L.physics_grad_mu_set(self.phys, (ctypes.c_double * 3)(*grad_mu)), "physics_grad_mu_set")
         _ck(L.advection_order_set(order), "advection_order_set")
         self.order = order
         self.phi = vp()

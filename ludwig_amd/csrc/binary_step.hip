@@ -107,7 +107,11 @@ void launch_shape(const Geom & g, const FeParams & fe, const LbParams & prm, con
 				       g_in, g_out, f0, f1, f2, csum)
 #define L_G(O, M) do { if (gz) L_(O, M, true); else L_(O, M, false); } while (0)
 #define L_M(O) do { if (fe.force_method == 0) L_G(O, 0); else L_G(O, 1); } while (0)
-  if (fe.order == 1) L_M(1);
+  if (fe.order == 4) {
+    // advection order 4 is built for the 64 x 8 tile only (launch() picks it)
+    if constexpr (S::TZ == 64 && S::TY == 8) L_M(4);
+  }
+  else if (fe.order == 1) L_M(1);
   else if (fe.order == 2) L_M(2);
   else L_M(3);
 #undef L_M
@@ -119,8 +123,9 @@ void launch_shape(const Geom & g, const FeParams & fe, const LbParams & prm, con
 // 64 x 8 / 128 x 4 tiles, which halve the share of pulled rows that cross a
 // tile edge in y or z; chosen by precision and lattice size
 // (tune_step_tile).  The tile changes only which block computes a site,
-// not the arithmetic.  With a compensation array (conserve 1) the 64 x 8
-// tile is used, so the compensated kernels are built for one tile only.
+// not the arithmetic.  With a compensation array (conserve 1) or advection
+// order 4 the 64 x 8 tile is used, so those kernels are built for one tile
+// only.
 template <typename T>
 void launch(const Geom & g, const FeParams & fe, const LbParams & prm, const T * phi,
 	    T * phi_out, const T * u_prev, T * u_next, T * rho, const T * g_in,
@@ -128,6 +133,11 @@ void launch(const Geom & g, const FeParams & fe, const LbParams & prm, const T *
   if (csum) {
     launch_shape<T, Shape<64, 8>, true>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero,
 					csum, st);
+    return;
+  }
+  if (fe.order == 4) {
+    launch_shape<T, Shape<64, 8>, false>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero,
+					 nullptr, st);
     return;
   }
   switch (tune_step_tile(sizeof(T) == 8 ? 64 : 32, g.nsite)) {

@@ -24,6 +24,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "advect.h"
 #include "ch_conserve.h"
 #include "grad27.h"
 #include "kernels.h"
@@ -156,17 +157,27 @@ phi_force_mu_kernel(Geom geo, FeParams fe, const T * __restrict__ phi,
     }
 
     // hydro->force += F (phi_force_colloid.c:457-462) on a force that
-    // hydro_f_zero() set to fzero, or read-modify-write otherwise
+    // hydro_f_zero() set to fzero, or read-modify-write otherwise; then,
+    // with phi grad mu, the external gradient's -phi grad mu_ext as a
+    // second addition (phi_grad_mu.c:123-160, 509-534)
+    T fx, fy, fz;
     if (force_mode == 0) {
-      force[site] = fz0 + F[0];
-      force[geo.nsite + site] = fz1 + F[1];
-      force[2*geo.nsite + site] = fz2 + F[2];
+      fx = fz0 + F[0]; fy = fz1 + F[1]; fz = fz2 + F[2];
     }
     else {
-      force[site] += F[0];
-      force[geo.nsite + site] += F[1];
-      force[2*geo.nsite + site] += F[2];
+      fx = force[site] + F[0];
+      fy = force[geo.nsite + site] + F[1];
+      fz = force[2*geo.nsite + site] + F[2];
     }
+    if (fe.force_method == 1 && fe.gmu_on) {
+      const T phi0 = c0.phi;
+      fx += -phi0*T(fe.gmu[0]);
+      fy += -phi0*T(fe.gmu[1]);
+      fz += -phi0*T(fe.gmu[2]);
+    }
+    force[site] = fx;
+    force[geo.nsite + site] = fy;
+    force[2*geo.nsite + site] = fz;
   }
 
   mu[site] = mu_of(fe, c0.phi, c0.delsq);
@@ -276,6 +287,28 @@ cahn_hilliard_kernel(Geom geo, FeParams fe, const T * __restrict__ phi,
       fzp = uzf*T(0.5)*(p0 + P.at(x, y, z + 1));
       fzm = uzb*T(0.5)*(P.at(x, y, z - 1) + p0);
     }
+    else if (fe.order == 4) {
+      fw = adv::order4(uw, P.at(x - 2, y, z), P.at(x - 1, y, z), p0, P.at(x + 1, y, z));
+      fe_ = adv::order4(ue, P.at(x - 1, y, z), p0, P.at(x + 1, y, z), P.at(x + 2, y, z));
+      fyp = adv::order4(uyf, P.at(x, y - 1, z), p0, P.at(x, y + 1, z), P.at(x, y + 2, z));
+      fym = adv::order4(uyb, P.at(x, y - 2, z), P.at(x, y - 1, z), p0, P.at(x, y + 1, z));
+      fzp = adv::order4(uzf, P.at(x, y, z - 1), p0, P.at(x, y, z + 1), P.at(x, y, z + 2));
+      fzm = adv::order4(uzb, P.at(x, y, z - 2), P.at(x, y, z - 1), p0, P.at(x, y, z + 1));
+    }
+    else if (fe.order == 5) {
+      fw = adv::order5(uw, uw > T(0.0), P.at(x - 3, y, z), P.at(x - 2, y, z), P.at(x - 1, y, z), p0,
+		       P.at(x + 1, y, z), P.at(x + 2, y, z));
+      fe_ = adv::order5(ue, !(ue < T(0.0)), P.at(x - 2, y, z), P.at(x - 1, y, z), p0, P.at(x + 1, y, z),
+			P.at(x + 2, y, z), P.at(x + 3, y, z));
+      fyp = adv::order5(uyf, !(uyf < T(0.0)), P.at(x, y - 2, z), P.at(x, y - 1, z), p0, P.at(x, y + 1, z),
+			P.at(x, y + 2, z), P.at(x, y + 3, z));
+      fym = adv::order5(uyb, !(uyb < T(0.0)), P.at(x, y - 3, z), P.at(x, y - 2, z), P.at(x, y - 1, z), p0,
+			P.at(x, y + 1, z), P.at(x, y + 2, z));
+      fzp = adv::order5(uzf, !(uzf < T(0.0)), P.at(x, y, z - 2), P.at(x, y, z - 1), p0, P.at(x, y, z + 1),
+			P.at(x, y, z + 2), P.at(x, y, z + 3));
+      fzm = adv::order5(uzb, !(uzb < T(0.0)), P.at(x, y, z - 3), P.at(x, y, z - 2), P.at(x, y, z - 1), p0,
+			P.at(x, y, z + 1), P.at(x, y, z + 2));
+    }
     else {
       T d1, d2, d3;
       // west face of x
@@ -314,6 +347,7 @@ cahn_hilliard_kernel(Geom geo, FeParams fe, const T * __restrict__ phi,
     fzp -= mob*(M.at(x, y, z + 1) - m0);
     fzm -= mob*(m0 - M.at(x, y, z - 1));
   }
+  if (fe.gmu_on) adv::mu_ext(mob, fe.gmu, fw, fe_, fyp, fym, fzp, fzm);
 
   // phi_ch_ufs_kernel (phi_cahn_hilliard.c:1039-1044), wz = 1, or the
   // compensated update of conserve 1 (phi_cahn_hilliard.c:1181-1218)

@@ -78,18 +78,17 @@ force_ch_kernel(Geom g, FeParams fe, int xchunk, int nbz, int nby, int nbx, int 
     if (active) {
       const int64_t site = ((int64_t) x*g.ny + y)*g.nz + z;
       T F[3], pnew;
-      force_ch_site<T, S, ORDER, METHOD, HAVE_U>(s, fe, x, ty, tz, uc, F, pnew,
+      // the force before this calculation: the hydro_f_zero value, or the
+      // stored force (read-modify-write)
+      T fzv[3] = {fz0, fz1, fz2};
+      if (force_mode != 0) {
+	fzv[0] = force[site]; fzv[1] = force[g.nsite + site]; fzv[2] = force[2*g.nsite + site];
+      }
+      force_ch_site<T, S, ORDER, METHOD, HAVE_U>(s, fe, x, ty, tz, uc, fzv, F, pnew,
 						 csum ? csum + site : nullptr);
-      if (force_mode == 0) {
-	force[site] = fz0 + F[0];
-	force[g.nsite + site] = fz1 + F[1];
-	force[2*g.nsite + site] = fz2 + F[2];
-      }
-      else {
-	force[site] += F[0];
-	force[g.nsite + site] += F[1];
-	force[2*g.nsite + site] += F[2];
-      }
+      force[site] = F[0];
+      force[g.nsite + site] = F[1];
+      force[2*g.nsite + site] = F[2];
       phi_out[site] = pnew;
     }
     __syncthreads();
@@ -120,7 +119,8 @@ void launch(const Geom & g, const FeParams & fe, const T * phi, const T * u, T *
 #define L_M(O) do { if (fe.force_method == 0) L_U(O, 0); else L_U(O, 1); } while (0)
   if (fe.order == 1) L_M(1);
   else if (fe.order == 2) L_M(2);
-  else L_M(3);
+  else if (fe.order == 3) L_M(3);
+  else L_M(4);
 #undef L_M
 #undef L_U
 #undef L_

@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "advect.h"
 #include "ch_conserve.h"
 #include "grad27.h"
 #include "kernels.h"
@@ -194,11 +195,15 @@ __device__ __forceinline__ T adv1(T uf, T phi_self, T phi_other, bool west) {
 
 // Force and Cahn-Hilliard update at site (x, y, z) of the output plane x,
 // from the LDS rings (Q planes x-1, x, x+1; phi planes x-2 .. x+2) and the
-// velocities uc of load_u().  Returns F and the updated phi; with csum_site
-// (the site's compensation) the update is the compensated one.
+// velocities uc of load_u().  Returns the site's hydrodynamic force
+// F = fz + F_phi (fz: the force before this calculation, the
+// hydro_f_zero value or the stored force), plus -phi grad mu_ext as a
+// second addition under phi grad mu with an external gradient, and the
+// updated phi; with csum_site (the site's compensation) the update is the
+// compensated one.  Advection orders 1-4 (order 4 reads phi to x +- 2).
 template <typename T, class S, int ORDER, int METHOD, bool HAVE_U>
 __device__ __forceinline__ void force_ch_site(const Smem<T, S> & s, const FeParams & fe,
-					      int x, int ty, int tz, const T (&uc)[9],
+					      int x, int ty, int tz, const T (&uc)[9], const T (&fz)[3],
 					      T (&F)[3], T & pnew, T * csum_site = nullptr) {
   constexpr int PZ = S::PZ, QZ = S::QZ, PPLANE = S::PPLANE, QPLANE = S::QPLANE;
   const T kappa = T(fe.kappa);
@@ -221,14 +226,19 @@ __device__ __forceinline__ void force_ch_site(const Smem<T, S> & s, const FePara
 	  s0 = pab_q(q0, c, kappa, a, 2);
 	  fa -= T(0.5)*(pab_q(q0, c + 1, kappa, a, 2) + s0);
 	  fa += T(0.5)*(pab_q(q0, c - 1, kappa, a, 2) + s0);
-	  F[a] = fa;
+	  F[a] = fz[a] + fa;
 	}
   }
   else {
 	const T phi0 = s.phi[modn(x, 5)][(ty + 2)*PZ + (tz + 2)];
-	F[0] = T(0.0) + -phi0*T(0.5)*(qp[4][c] - qm[4][c]);
-	F[1] = T(0.0) + -phi0*T(0.5)*(q0[4][c + QZ] - q0[4][c - QZ]);
-	F[2] = T(0.0) + -phi0*T(0.5)*(q0[4][c + 1] - q0[4][c - 1]);
+	F[0] = fz[0] + (T(0.0) + -phi0*T(0.5)*(qp[4][c] - qm[4][c]));
+	F[1] = fz[1] + (T(0.0) + -phi0*T(0.5)*(q0[4][c + QZ] - q0[4][c - QZ]));
+	F[2] = fz[2] + (T(0.0) + -phi0*T(0.5)*(q0[4][c + 1] - q0[4][c - 1]));
+	if (fe.gmu_on) {
+	  F[0] += -phi0*T(fe.gmu[0]);
+	  F[1] += -phi0*T(fe.gmu[1]);
+	  F[2] += -phi0*T(fe.gmu[2]);
+	}
   }
   // Cahn-Hilliard fluxes: phi from the ring, mu from Q
   const T (&P0)[PPLANE] = s.phi[modn(x, 5)];
@@ -263,7 +273,18 @@ __device__ __forceinline__ void force_ch_site(const Smem<T, S> & s, const FePara
 	  fzp = uzf*T(0.5)*(p0 + P0[cp + 1]);
 	  fzm = uzb*T(0.5)*(P0[cp - 1] + p0);
 	}
+	else if constexpr (ORDER == 4) {
+	  const T (&PM2)[PPLANE] = s.phi[modn(x - 2, 5)];
+	  const T (&PP2)[PPLANE] = s.phi[modn(x + 2, 5)];
+	  fw = adv::order4(uw, PM2[cp], PM[cp], p0, PP[cp]);
+	  fe_ = adv::order4(ue, PM[cp], p0, PP[cp], PP2[cp]);
+	  fyp = adv::order4(uyf, P0[cp - PZ], p0, P0[cp + PZ], P0[cp + 2*PZ]);
+	  fym = adv::order4(uyb, P0[cp - 2*PZ], P0[cp - PZ], p0, P0[cp + PZ]);
+	  fzp = adv::order4(uzf, P0[cp - 1], p0, P0[cp + 1], P0[cp + 2]);
+	  fzm = adv::order4(uzb, P0[cp - 2], P0[cp - 1], p0, P0[cp + 1]);
+	}
 	else {
+	  static_assert(ORDER == 3, "the tiles hold phi to +-2: advection orders 1-4");
 	  const T (&PM2)[PPLANE] = s.phi[modn(x - 2, 5)];
 	  const T (&PP2)[PPLANE] = s.phi[modn(x + 2, 5)];
 	  T d1, d2, d3;
@@ -296,6 +317,7 @@ __device__ __forceinline__ void force_ch_site(const Smem<T, S> & s, const FePara
 	fzp -= mob*(q0[4][c + 1] - m0);
 	fzm -= mob*(m0 - q0[4][c - 1]);
   }
+  if (fe.gmu_on) adv::mu_ext(mob, fe.gmu, fw, fe_, fyp, fym, fzp, fzm);
   pnew = p0;
   if (csum_site != nullptr) {
     T cs = *csum_site;

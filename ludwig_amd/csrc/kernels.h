@@ -30,9 +30,11 @@ struct LbParams {
 struct FeParams {
   double a, b, kappa;
   double mobility;
-  int order;                // advection order 1, 2, 3
+  int order;                // advection order 1..5
   int force_method;         // 0 stress divergence, 1 phi grad mu
   int stencil;              // gradient: 7 (3d_7pt_fluid) or 27 (3d_27pt_fluid)
+  int gmu_on;               // non-zero: the external gradient gmu acts
+  double gmu[3];            // external chemical potential gradient (physics grad_mu)
 };
 
 // Launch tuning of the tiled kernels, read once from the environment:

@@ -164,7 +164,8 @@ void launch_le_step(const Geom & g, const FeParams & fe, const LbParams & prm, c
 #define LB_(O, G) launch_both<S, O, G>(g, fe, prm, le, a, phi, phi_out, u_prev, u_next, rho, g_in, g_out, force, csum, st, sums_done)
   if (fe.order == 1) { if (gz) LB_(1, true); else LB_(1, false); }
   else if (fe.order == 2) { if (gz) LB_(2, true); else LB_(2, false); }
-  else { if (gz) LB_(3, true); else LB_(3, false); }
+  else if (fe.order == 3) { if (gz) LB_(3, true); else LB_(3, false); }
+  else { if (gz) LB_(4, true); else LB_(4, false); }
 #undef LB_
 }
 

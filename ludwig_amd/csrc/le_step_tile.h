@@ -171,7 +171,18 @@ __device__ __forceinline__ void le_force_ch_site(const Smem<double, S> & s, cons
       fzp = uzf*0.5*(p0 + P0[cp + 1]);
       fzm = uzb*0.5*(P0[cp - 1] + p0);
     }
+    else if constexpr (ORDER == 4) {
+      const double (&PM2)[PPLANE] = s.phi[modn(x - 2, 5)];
+      const double (&PP2)[PPLANE] = s.phi[modn(x + 2, 5)];
+      fw = adv::order4(uw, PM2[cp], PM[cp], p0, PP[cp]);
+      fe_ = adv::order4(ue, PM[cp], p0, PP[cp], PP2[cp]);
+      fyp = adv::order4(uyf, P0[cp - PZ], p0, P0[cp + PZ], P0[cp + 2*PZ]);
+      fym = adv::order4(uyb, P0[cp - 2*PZ], P0[cp - PZ], p0, P0[cp + PZ]);
+      fzp = adv::order4(uzf, P0[cp - 1], p0, P0[cp + 1], P0[cp + 2]);
+      fzm = adv::order4(uzb, P0[cp - 2], P0[cp - 1], p0, P0[cp + 1]);
+    }
     else {
+      static_assert(ORDER == 3, "the tiles hold phi to +-2: advection orders 1-4");
       const double (&PM2)[PPLANE] = s.phi[modn(x - 2, 5)];
       const double (&PP2)[PPLANE] = s.phi[modn(x + 2, 5)];
       double d1, d2, d3;
@@ -204,6 +215,7 @@ __device__ __forceinline__ void le_force_ch_site(const Smem<double, S> & s, cons
     fzp -= mob*(q0[4][c + 1] - m0);
     fzm -= mob*(m0 - q0[4][c - 1]);
   }
+  if (fe.gmu_on) adv::mu_ext(mob, fe.gmu, fw, fe_, fyp, fym, fzp, fzm);
   // phi_ch_le_fix_fluxes (phi_cahn_hilliard.c:655-720) at the slab's edges
   if (L.chface != nullptr) {
     const int N = g.ny;

@@ -39,6 +39,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "advect.h"
 #include "ch_conserve.h"
 #include "d3q19.h"
 #include "grad27.h"
@@ -463,6 +464,16 @@ __device__ __forceinline__ ChFlux ch_fluxes(const Geom & g, const LeGeom & le, c
       f.fzp = uzf*0.5*(p0 + P.at(x, y, z + 1));
       f.fzm = uzb*0.5*(P.at(x, y, z - 1) + p0);
     }
+    else if (fe.order == 4) {
+      // advection_le_4th (advection.c:1153-1262), x neighbours mapped
+      const int m2 = xnb(le, g.nx, x, -2), p2 = xnb(le, g.nx, x, +2);
+      f.fw = adv::order4(uw, P.at(m2, y, z), P.at(m1, y, z), p0, P.at(p1, y, z));
+      f.fe = adv::order4(ue, P.at(m1, y, z), p0, P.at(p1, y, z), P.at(p2, y, z));
+      f.fyp = adv::order4(uyf, P.at(x, y - 1, z), p0, P.at(x, y + 1, z), P.at(x, y + 2, z));
+      f.fym = adv::order4(uyb, P.at(x, y - 2, z), P.at(x, y - 1, z), p0, P.at(x, y + 1, z));
+      f.fzp = adv::order4(uzf, P.at(x, y, z - 1), p0, P.at(x, y, z + 1), P.at(x, y, z + 2));
+      f.fzm = adv::order4(uzb, P.at(x, y, z - 2), P.at(x, y, z - 1), p0, P.at(x, y, z + 1));
+    }
     else {
       const int m2 = xnb(le, g.nx, x, -2), p2 = xnb(le, g.nx, x, +2);
       double d1, d2, d3;
@@ -494,6 +505,9 @@ __device__ __forceinline__ ChFlux ch_fluxes(const Geom & g, const LeGeom & le, c
   f.fym -= mob*(m0 - mu_at(fe, P, D, x, y - 1, z));
   f.fzp -= mob*(mu_at(fe, P, D, x, y, z + 1) - m0);
   f.fzm -= mob*(m0 - mu_at(fe, P, D, x, y, z - 1));
+  // phi_ch_flux_mu_ext (phi_cahn_hilliard.c:1373-1398), before the plane
+  // faces are averaged
+  if (fe.gmu_on) adv::mu_ext(mob, fe.gmu, f.fw, f.fe, f.fyp, f.fym, f.fzp, f.fzm);
   return f;
 }
 

@@ -380,6 +380,7 @@ static Geom geom_of(cs_t * cs) {
 
 struct physics_s {
   double rho0, phi0, eta_shear, eta_bulk, mobility, fbody[3];
+  double grad_mu[3];                // external chemical potential gradient
   int t_start, t_current, nsteps;   // time step control (physics.c:600-670)
 };
 
@@ -420,6 +421,8 @@ extern "C" int physics_eta_shear(physics_t * p, double * v) { *v = p->eta_shear;
 extern "C" int physics_eta_bulk(physics_t * p, double * v) { *v = p->eta_bulk; return 0; }
 extern "C" int physics_mobility(physics_t * p, double * v) { *v = p->mobility; return 0; }
 extern "C" int physics_fbody(physics_t * p, double f[3]) { for (int a = 0; a < 3; a++) f[a] = p->fbody[a]; return 0; }
+extern "C" int physics_grad_mu_set(physics_t * p, double gm[3]) { for (int a = 0; a < 3; a++) p->grad_mu[a] = gm[a]; return 0; }
+extern "C" int physics_grad_mu(physics_t * p, double gm[3]) { for (int a = 0; a < 3; a++) gm[a] = p->grad_mu[a]; return 0; }
 
 /* physics.c:600-670: the step counter runs nstart + 1 .. nstart + nsteps;
  * the "time" of step t_current is t_start + t_current - 1 */
@@ -1487,6 +1490,11 @@ static FeParams fe_params_of(fe_symm_t * fe, int method) {
   p.order = order;
   p.force_method = (method == FE_FORCE_METHOD_PHI_GRADMU) ? 1 : 0;
   p.stencil = grad_stencil(fe->dphi);
+  // the external gradient acts on every Cahn-Hilliard face flux
+  // (phi_cahn_hilliard.c:1344) and, under phi grad mu, on the force when
+  // it is non-zero (phi_grad_mu.c:138-145)
+  for (int a = 0; a < 3; a++) p.gmu[a] = phys->grad_mu[a];
+  p.gmu_on = (p.gmu[0] != 0.0 || p.gmu[1] != 0.0 || p.gmu[2] != 0.0);
   return p;
 }
 
@@ -1620,7 +1628,10 @@ static int materialise_derived(void) {
 static int g_adv_order = 1;    /* advection.c:74 default */
 
 extern "C" int advection_order_set(const int order) {
-  if (order < 1 || order > 3) FAIL("advection_order_set: order %d not built (1, 2, 3 are)", order);
+  // advection.c:463-472 dispatches orders 1-5; order 5 reads phi three
+  // sites away, which the periodic kernels here do by index wrap (the
+  // reference needs a coordinate halo of 3 for it)
+  if (order < 1 || order > 5) FAIL("advection_order_set: order %d (1 to 5)", order);
   g_adv_order = order;
   return 0;
 }
@@ -1746,6 +1757,10 @@ extern "C" int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi, hydro
     if (dphi->le_version != phi->version) {
       FAIL("phi_cahn_hilliard: with Lees-Edwards planes field_grad_compute() of the current phi must come first");
     }
+    if (p.order > 4) {
+      FAIL("phi_cahn_hilliard: advection order %d with Lees-Edwards planes: the buffer planes hold the "
+	   "two sites next to each plane (orders 1-4)", p.order);
+    }
     if (flush_force()) return -1;
     const double * u = nullptr;
     double * ub = nullptr;
@@ -1778,7 +1793,7 @@ extern "C" int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi, hydro
   }
 
   if (hydro && hydro->fp_pending && g_pending == hydro && hydro->fp_phi == phi
-      && hydro->fp_version == phi->version) {
+      && hydro->fp_version == phi->version && p.order <= 4) {
     // force and Cahn-Hilliard update in one pass (K23); a uniform zero
     // velocity is passed as no velocity (advective flux exactly zero)
     const void * uf = hydro->d_u;
@@ -2232,6 +2247,11 @@ extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_
   if (pch && pch->info.conserve && pch->prec != phi->prec) {
     FAIL("ldw_step_binary: phi_ch_t (%d-bit) and phi (%d-bit) differ in precision", pch->prec, phi->prec);
   }
+  int order = 1;
+  advection_order(&order);
+  if (order > 4 && le_active(phi->le)) {
+    FAIL("ldw_step_binary: advection order %d with Lees-Edwards planes (orders 1-4 are built)", order);
+  }
   physics_t * phys = nullptr;
   physics_ref(&phys);
   for (int step = 0; step < nsteps; step++) {
@@ -2240,7 +2260,7 @@ extern "C" int ldw_step_binary(lb_t * lb, hydro_t * hydro, field_t * phi, field_
       && (pth->method == FE_FORCE_METHOD_STRESS_DIVERGENCE || pth->method == FE_FORCE_METHOD_PHI_GRADMU)
       && lb->streamed && !hydro->u_uniform && !hydro->fp_pending
       && lb->prec == hydro->prec && phi->prec == lb->prec && fs->phi == phi
-      && pch && (pch->info.conserve == 0 || pch->prec == phi->prec)
+      && pch && (pch->info.conserve == 0 || pch->prec == phi->prec) && order <= 4
       && !le_active(phi->le) && !le_active(hydro->le);
     // the planes' single-pass step: FP64, a force to form, the velocity of
     // a collision, and slabs at least 3 planes deep

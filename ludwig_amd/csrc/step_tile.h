@@ -80,15 +80,16 @@ __device__ __forceinline__ void step_tile(Smem<T, S> & s, const Geom & g, const 
     if (active) {
       const int64_t site = ((int64_t) x*g.ny + y)*g.nz + z;
       T F[3], pnew;
-      force_ch_site<T, S, ORDER, METHOD, true>(s, fe, x, ty, tz, uc, F, pnew,
+      const T fzv[3] = {fz0, fz1, fz2};
+      force_ch_site<T, S, ORDER, METHOD, true>(s, fe, x, ty, tz, uc, fzv, F, pnew,
 					       CONS ? csum + site : nullptr);
       phi_out[site] = pnew;
       // site force: force_global + (hydro_f_zero value + phi force)
       // (collision.c:323-327, phi_force_colloid.c:457-462)
       T fs[3];
-      fs[0] = T(prm.fglobal[0]) + (fz0 + F[0]);
-      fs[1] = T(prm.fglobal[1]) + (fz1 + F[1]);
-      fs[2] = T(prm.fglobal[2]) + (fz2 + F[2]);
+      fs[0] = T(prm.fglobal[0]) + F[0];
+      fs[1] = T(prm.fglobal[1]) + F[1];
+      fs[2] = T(prm.fglobal[2]) + F[2];
       T r, uu[3];
       site::collide_site<T, GZ>(prm, fs, fc, r, uu);
       site::store_planes<T>(g_out, g.nsite, site, fc);

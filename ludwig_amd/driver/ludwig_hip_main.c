@@ -389,15 +389,12 @@ int main(int argc, char ** argv) {
     fatal("colloids are not supported by this build", NULL);
   }
   get_double("temperature", &(double) {0.0});
-  {
-    /* physics_rt.c:144-146: an external chemical potential gradient adds
-     * -M grad_mu to every Cahn-Hilliard face flux (phi_cahn_hilliard.c:
-     * 1373-1398); only the zero gradient is built */
-    double gm[3] = {0.0, 0.0, 0.0};
-    if (get_dvec("grad_mu", gm) && (gm[0] != 0.0 || gm[1] != 0.0 || gm[2] != 0.0)) {
-      fatal("a non-zero grad_mu (external chemical potential gradient) is not supported by this build", NULL);
-    }
-  }
+  /* physics_rt.c:144-146: an external chemical potential gradient adds
+   * -M grad_mu to every Cahn-Hilliard face flux (phi_cahn_hilliard.c:
+   * 1373-1398) and, with fe_force_method phi_gradmu, -phi grad_mu to the
+   * force (phi_grad_mu.c:123-160) */
+  double grad_mu[3] = {0.0, 0.0, 0.0};
+  get_dvec("grad_mu", grad_mu);
 
   pe_t * pe = NULL;
   cs_t * cs = NULL;
@@ -418,6 +415,7 @@ int main(int argc, char ** argv) {
   physics_eta_shear_set(phys, eta);
   physics_eta_bulk_set(phys, eta_bulk);
   physics_rho0_set(phys, rho0);
+  physics_grad_mu_set(phys, grad_mu);
 
   cs_create(pe, &cs);
   cs_nhalo_set(cs, symmetric ? 2 : 1);     /* ludwig.c:1183,1198 */

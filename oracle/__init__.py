@@ -35,7 +35,8 @@ class FeParam(ctypes.Structure):
     _fields_ = [("a", ctypes.c_double), ("b", ctypes.c_double),
                 ("kappa", ctypes.c_double), ("mobility", ctypes.c_double),
                 ("order", ctypes.c_int), ("force_method", ctypes.c_int),
-                ("stencil", ctypes.c_int), ("conserve", ctypes.c_int)]
+                ("stencil", ctypes.c_int), ("conserve", ctypes.c_int),
+                ("grad_mu", ctypes.c_double * 3)]
 
 
 class LeParam(ctypes.Structure):
@@ -91,8 +92,9 @@ def lb_param(tau=None, eta=None, eta_bulk=None, relax=M10, rho0=1.0,
 
 
 def fe_param(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.45, order=1,
-             force_method=0, stencil=7, conserve=0):
-    return FeParam(a, b, kappa, mobility, order, force_method, stencil, conserve)
+             force_method=0, stencil=7, conserve=0, grad_mu=(0.0, 0.0, 0.0)):
+    return FeParam(a, b, kappa, mobility, order, force_method, stencil, conserve,
+                   (ctypes.c_double * 3)(*grad_mu))
 
 
 def tables():

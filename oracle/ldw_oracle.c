@@ -509,6 +509,10 @@ void ora_force(int nx, int ny, int nz, const ora_fe_param_t * fe,
 	    force += -phi0*0.5*(mup1 - mum1);
 	    F[ia*nsite + is] = 0.0 + force;
 	  }
+	  /* phi_grad_mu_external_kernel: a second hydro_f_local_add */
+	  if (fe->grad_mu[0] != 0.0 || fe->grad_mu[1] != 0.0 || fe->grad_mu[2] != 0.0) {
+	    for (int ia = 0; ia < 3; ia++) F[ia*nsite + is] += -phi0*fe->grad_mu[ia];
+	  }
 	}
       }
     }
@@ -592,6 +596,46 @@ void ora_cahn_hilliard_ext(int nx, int ny, int nz, const ora_fe_param_t * fe,
 	  fyv = 0.5*(uy[i0] + uy[yp1])*0.5*(phi[i0] + phi[yp1]);
 	  fzv = 0.5*(uz[i0] + uz[zp1])*0.5*(phi[i0] + phi[zp1]);
 	}
+	else if (fe->order == 4) {
+	  /* advection_le_4th (advection.c:1153-1262) */
+	  const double b1 = 1.0/16.0, b2 = 9.0/16.0;
+	  uf = 0.5*(ux[i0] + ux[xm1]);
+	  w = uf*(- b1*phi[xm2] + b2*phi[xm1] + b2*phi[i0] - b1*phi[xp1]);
+	  uf = 0.5*(ux[i0] + ux[xp1]);
+	  e = uf*(- b1*phi[xm1] + b2*phi[i0] + b2*phi[xp1] - b1*phi[xp2]);
+	  uf = 0.5*(uy[i0] + uy[yp1]);
+	  fyv = uf*(- b1*phi[ym1] + b2*phi[i0] + b2*phi[yp1] - b1*phi[yp2]);
+	  uf = 0.5*(uz[i0] + uz[zp1]);
+	  fzv = uf*(- b1*phi[zm1] + b2*phi[i0] + b2*phi[zp1] - b1*phi[zp2]);
+	}
+	else if (fe->order == 5) {
+	  /* advection_le_5th (advection.c:1264-1430): five points, upwind
+	   * biased; the points of each face in order from the upwind side */
+	  const double c[5] = {0.055453, -0.305147, 0.916054, 0.361520, -0.027880};
+	  long xm3 = site3(wrap(x - 3, nx), y, z, ny, nz);
+	  long xp3 = site3(wrap(x + 3, nx), y, z, ny, nz);
+	  long ym2 = site3(x, wrap(y - 2, ny), z, ny, nz);
+	  long yp3 = site3(x, wrap(y + 3, ny), z, ny, nz);
+	  long zm2 = site3(x, y, wrap(z - 2, nz), ny, nz);
+	  long zp3 = site3(x, y, wrap(z + 3, nz), ny, nz);
+	  long s5[5];
+	  uf = 0.5*(ux[i0] + ux[xm1]);
+	  if (uf > 0.0) { long t[5] = {xm3, xm2, xm1, i0, xp1}; memcpy(s5, t, sizeof(t)); }
+	  else          { long t[5] = {xp2, xp1, i0, xm1, xm2}; memcpy(s5, t, sizeof(t)); }
+	  w = uf*(c[0]*phi[s5[0]] + c[1]*phi[s5[1]] + c[2]*phi[s5[2]] + c[3]*phi[s5[3]] + c[4]*phi[s5[4]]);
+	  uf = 0.5*(ux[i0] + ux[xp1]);
+	  if (uf < 0.0) { long t[5] = {xp3, xp2, xp1, i0, xm1}; memcpy(s5, t, sizeof(t)); }
+	  else          { long t[5] = {xm2, xm1, i0, xp1, xp2}; memcpy(s5, t, sizeof(t)); }
+	  e = uf*(c[0]*phi[s5[0]] + c[1]*phi[s5[1]] + c[2]*phi[s5[2]] + c[3]*phi[s5[3]] + c[4]*phi[s5[4]]);
+	  uf = 0.5*(uy[i0] + uy[yp1]);
+	  if (uf < 0.0) { long t[5] = {yp3, yp2, yp1, i0, ym1}; memcpy(s5, t, sizeof(t)); }
+	  else          { long t[5] = {ym2, ym1, i0, yp1, yp2}; memcpy(s5, t, sizeof(t)); }
+	  fyv = uf*(c[0]*phi[s5[0]] + c[1]*phi[s5[1]] + c[2]*phi[s5[2]] + c[3]*phi[s5[3]] + c[4]*phi[s5[4]]);
+	  uf = 0.5*(uz[i0] + uz[zp1]);
+	  if (uf < 0.0) { long t[5] = {zp3, zp2, zp1, i0, zm1}; memcpy(s5, t, sizeof(t)); }
+	  else          { long t[5] = {zm2, zm1, i0, zp1, zp2}; memcpy(s5, t, sizeof(t)); }
+	  fzv = uf*(c[0]*phi[s5[0]] + c[1]*phi[s5[1]] + c[2]*phi[s5[2]] + c[3]*phi[s5[3]] + c[4]*phi[s5[4]]);
+	}
 	else {
 	  double d1, d2, d3;
 	  uf = 0.5*(ux[i0] + ux[xm1]);
@@ -624,6 +668,11 @@ void ora_cahn_hilliard_ext(int nx, int ny, int nz, const ora_fe_param_t * fe,
 	  mu1 = mu_site(fe, phi[zp1], delsq[zp1]);
 	  fzv -= mob*(mu1 - mu0);
 	}
+	/* phi_ch_flux_mu_ext_kernel: every flux, unconditionally */
+	w -= mob*fe->grad_mu[0];
+	e -= mob*fe->grad_mu[0];
+	fyv -= mob*fe->grad_mu[1];
+	fzv -= mob*fe->grad_mu[2];
 	fw[i0] = w; fe_[i0] = e; fy[i0] = fyv; fz[i0] = fzv;
       }
     }

@@ -42,10 +42,11 @@ typedef struct ora_lb_param_s {
 typedef struct ora_fe_param_s {
   double a, b, kappa;      /* symmetric free energy A, B, kappa */
   double mobility;         /* Cahn-Hilliard mobility M */
-  int    order;            /* advection order 1, 2 or 3 */
+  int    order;            /* advection order 1 to 5 */
   int    force_method;     /* 0 stress divergence, 1 phi grad mu */
   int    stencil;          /* gradient: 7 (3d_7pt_fluid, also 0) or 27 (3d_27pt_fluid) */
   int    conserve;         /* cahn_hilliard_options_conserve 0, 1 or 2 */
+  double grad_mu[3];       /* external chemical potential gradient (physics grad_mu) */
 } ora_fe_param_t;
 
 /* D3Q19 tables: velocities, weights, ma (modes x velocities),
@@ -84,14 +85,18 @@ void ora_stress(int nsite, const ora_fe_param_t * fe, const double * phi,
 		const double * grad, const double * delsq, double * pth);
 
 /* Thermodynamic force: method 0 stress divergence
- * (phi_force_colloid.c:315-466), 1 -phi grad mu (phi_grad_mu.c:284-350).
- * F is overwritten (the reference adds to a zeroed force). */
+ * (phi_force_colloid.c:315-466), 1 -phi grad mu (phi_grad_mu.c:284-350),
+ * then under method 1 -phi grad_mu added when grad_mu is non-zero
+ * (phi_grad_mu.c:123-160, 509-534).  F is overwritten (the reference adds
+ * to a zeroed force). */
 void ora_force(int nx, int ny, int nz, const ora_fe_param_t * fe,
 	       const double * phi, const double * grad, const double * delsq,
 	       double * F);
 
 /* One Cahn-Hilliard update of phi in place (phi_cahn_hilliard.c:213-290,
- * advection.c, advection_bcs.c), using delsq of the same phi and u. */
+ * advection.c, advection_bcs.c), using delsq of the same phi and u:
+ * advective fluxes of order 1-5, then -M grad mu, then -M grad_mu
+ * (phi_ch_flux_mu_ext, phi_cahn_hilliard.c:1373-1398), then the update. */
 void ora_cahn_hilliard(int nx, int ny, int nz, const ora_fe_param_t * fe,
 		       double * phi, const double * delsq, const double * u);
 /* The same with the conservation option fe->conserve:

@@ -407,6 +407,19 @@ static void cahn_hilliard(const le_t * g, const ora_fe_param_t * fe, double t, d
 	    fyv = 0.5*(uy[s] + uy[syp1])*1*0.5*(pe[s] + pe[syp1]);
 	    fzv = 0.5*(uz[s] + uz[szp1])*1*0.5*(pe[s] + pe[szp1]);
 	  }
+	  else if (fe->order == 4) {
+	    /* advection_le_4th (advection.c:1153-1262) */
+	    const double b1 = 1.0/16.0, b2 = 9.0/16.0;
+	    long sym1_ = idx(g, x, y - 1, z), szm1_ = idx(g, x, y, z - 1);
+	    uf = 0.5*(ux[s] + ux[sxm1]);
+	    w = uf*(- b1*pe[sxm2] + b2*pe[sxm1] + b2*pe[s] - b1*pe[sxp1]);
+	    uf = 0.5*(ux[s] + ux[sxp1]);
+	    e = uf*(- b1*pe[sxm1] + b2*pe[s] + b2*pe[sxp1] - b1*pe[sxp2]);
+	    uf = 0.5*(uy[s] + uy[syp1]);
+	    fyv = uf*(- b1*pe[sym1_] + b2*pe[s] + b2*pe[syp1] - b1*pe[syp2]);
+	    uf = 0.5*(uz[s] + uz[szp1]);
+	    fzv = uf*(- b1*pe[szm1_] + b2*pe[s] + b2*pe[szp1] - b1*pe[szp2]);
+	  }
 	  else {
 	    double d1, d2, d3;
 	    uf = 0.5*1*(ux[s] + ux[sxm1]);
@@ -432,6 +445,11 @@ static void cahn_hilliard(const le_t * g, const ora_fe_param_t * fe, double t, d
 	e -= mob*(mu_at(fe, pe, de, sxp1) - mu0);
 	fyv -= mob*(mu_at(fe, pe, de, syp1) - mu0);
 	fzv -= mob*(mu_at(fe, pe, de, szp1) - mu0);
+	/* phi_ch_flux_mu_ext, before the plane faces are fixed */
+	w -= mob*fe->grad_mu[0];
+	e -= mob*fe->grad_mu[0];
+	fyv -= mob*fe->grad_mu[1];
+	fzv -= mob*fe->grad_mu[2];
 	long r = ((long) x*g->ny + y)*g->nz + z;
 	fw[r] = w; fe_[r] = e; fy[r] = fyv; fz[r] = fzv;
       }
@@ -601,6 +619,7 @@ int ora_le_cahn_hilliard(int nx, int ny, int nz, const ora_fe_param_t * fe,
 			 const ora_le_param_t * lp, double t, double * phi, const double * u,
 			 double * csum, double phi_init_sum) {
   le_t g;
+  if (fe->order > 4) return -1;          /* the buffer planes are nhalo = 2 deep */
   if (le_init(&g, nx, ny, nz, lp)) return -1;
   const long ns = (long) nx*ny*nz, ne = (long) (nx + g.nxb)*ny*nz;
   double * pe, * ge, * de, * ue = NULL;
@@ -662,6 +681,7 @@ int ora_le_binary_steps(int nx, int ny, int nz, int nsteps, int t_start, int t_c
 			const ora_le_param_t * lp, double * f, double * phi, double * rho,
 			double * u, double * csum, double phi_init_sum) {
   le_t g;
+  if (fe->order > 4) return -1;          /* the buffer planes are nhalo = 2 deep */
   if (le_init(&g, nx, ny, nz, lp)) return -1;
   const long ns = (long) nx*ny*nz, ne = (long) (nx + g.nxb)*ny*nz;
   double * pe, * ge, * de;

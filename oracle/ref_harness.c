@@ -37,6 +37,12 @@
  *  binary case applies lb_data_apply_le_boundary_conditions (model_le.c:78)
  *  after each collision, as ludwig.c:810-812 does.
  *
+ *  LDW_HARNESS_GRADMU="gx gy gz" sets the external chemical potential
+ *  gradient (physics_grad_mu_set, physics.c:587); LDW_HARNESS_METHOD=1
+ *  runs the binary case with fe_force_method phi_gradmu;
+ *  LDW_HARNESS_NHALO=n gives the order-parameter cases a coordinate halo
+ *  of n (3 for advection order 5, advection.c:1264-1430).
+ *
  *  Reference functions used (file:line in the reference tree):
  *    lb_collide          collision.c:143
  *    lb_halo             lb_data.c:754
@@ -148,12 +154,28 @@ static void sys_create(sys_t * s, int n, int nhalo, lb_relaxation_enum_t relax,
   memset(s, 0, sizeof(*s));
   s->n = n;
   s->nsites = n*n*n;
+  {
+    /* LDW_HARNESS_NHALO=3: a deeper coordinate halo for the order
+     * parameter cases (advection order 5 reads phi three sites away) */
+    const char * nh = getenv("LDW_HARNESS_NHALO");
+    if (nh && with_phi) nhalo = atoi(nh);
+  }
 
   pe_create(MPI_COMM_WORLD, PE_QUIET, &s->pe);
   physics_create(s->pe, &s->phys);
   physics_eta_shear_set(s->phys, eta);
   physics_eta_bulk_set(s->phys, eta);
   physics_rho0_set(s->phys, 1.0);
+  {
+    /* LDW_HARNESS_GRADMU="gx gy gz": the external chemical potential
+     * gradient (input key grad_mu) */
+    const char * gm = getenv("LDW_HARNESS_GRADMU");
+    if (gm) {
+      double v[3] = {0.0, 0.0, 0.0};
+      if (sscanf(gm, "%lf %lf %lf", v, v + 1, v + 2) != 3) usage();
+      physics_grad_mu_set(s->phys, v);
+    }
+  }
 
   cs_create(s->pe, &s->cs);
   {
@@ -471,7 +493,10 @@ int main(int argc, char ** argv) {
     double phi0 = atof(argv[12]), amp = atof(argv[13]);
     int dumpevery = (argc > 14) ? atoi(argv[14]) : 0;
     double eta = (tau - 0.5)/3.0;
-    sys_create(&s, n, 2, LB_RELAXATION_M10, eta, 1, FE_FORCE_METHOD_STRESS_DIVERGENCE);
+    /* LDW_HARNESS_METHOD=1 selects fe_force_method phi_gradmu */
+    const char * fmv = getenv("LDW_HARNESS_METHOD");
+    int fm = (fmv && atoi(fmv) == 1) ? FE_FORCE_METHOD_PHI_GRADMU : FE_FORCE_METHOD_STRESS_DIVERGENCE;
+    sys_create(&s, n, 2, LB_RELAXATION_M10, eta, 1, fm);
     physics_mobility_set(s.phys, mob);
     advection_order_set(order);
     fe_params(&s, a, b, k);

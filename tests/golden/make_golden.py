@@ -127,6 +127,41 @@ CASES = {
     "le_binary16_t0_s12": (["binary", 16, 12, 99, 2, A, B, K, MS, 0.8, 0.0, 0.1], 16,
                            ["phi", "rho", "u"], ["f"],
                            {"LDW_HARNESS_LE": "2 0.15 7", "LDW_HARNESS_LE_PROFILE": "1"}),
+    # external chemical potential gradient (grad_mu): the Cahn-Hilliard face
+    # fluxes and, with phi_gradmu, the force
+    "force_gradmu_ext": (["force", 8, 31, 1, A, B, K], 8, ["phi", "grad", "delsq", "mu", "force"], [],
+                         {"LDW_HARNESS_GRADMU": "0.001 -0.002 0.003"}),
+    "ch_gradmu_o2": (["ch", 8, 32, 2, A, B, K, M], 8, ["phi_in", "u", "phi_out"], [],
+                     {"LDW_HARNESS_GRADMU": "0.001 -0.002 0.003"}),
+    "binary16_gmu_s10": (["binary", 16, 10, 8361235, 1, -0.00625, 0.00625, 0.004, 1.25, 0.51875, 0.0, 0.1],
+                         16, ["phi", "rho", "u"], ["f"],
+                         {"LDW_HARNESS_STENCIL": "27", "LDW_HARNESS_METHOD": "1",
+                          "LDW_HARNESS_GRADMU": "0.00001 0.00002 0.00003"}),
+    "binary16_gmu_stress_s10": (["binary", 16, 10, 5151, 2, A, B, K, MS, 1.0, 0.0, 0.1],
+                                16, ["phi", "rho", "u"], ["f"],
+                                {"LDW_HARNESS_GRADMU": "0.0005 0.0 -0.001"}),
+    # advection orders 4 and 5 (order 5 with a coordinate halo of 3: its
+    # stencil reaches three sites)
+    "ch_order4": (["ch", 8, 33, 4, A, B, K, M], 8, ["phi_in", "u", "phi_out"], []),
+    "ch_order5": (["ch", 8, 34, 5, A, B, K, M], 8, ["phi_in", "u", "phi_out"], [],
+                  {"LDW_HARNESS_NHALO": "3"}),
+    "binary16_o4_s10": (["binary", 16, 10, 4343, 4, A, B, K, MS, 1.0, 0.0, 0.1], 16,
+                        ["phi", "rho", "u"], ["f"]),
+    "binary16_o5_s10": (["binary", 16, 10, 5454, 5, A, B, K, MS, 1.0, 0.0, 0.1], 16,
+                        ["phi", "rho", "u"], ["f"], {"LDW_HARNESS_NHALO": "3"}),
+    "binary16_o4_g27_c1_s10": (["binary", 16, 10, 77, 4, A, B, K, MS, 0.8, 0.05, 0.2], 16,
+                               ["phi", "rho", "u", "csum"], ["f"],
+                               {"LDW_HARNESS_STENCIL": "27", "LDW_HARNESS_CONSERVE": "1",
+                                "OMP_NUM_THREADS": "1"}),
+    "le_ch16_o4": (["ch", 16, 35, 4, A, B, K, M], 16, ["phi_in", "u", "phi_out"], [],
+                   {"LDW_HARNESS_LE": "2 0.37", "LDW_HARNESS_T0": "9"}),
+    "le_binary16_o4_s10": (["binary", 16, 10, 13, 4, A, B, K, MS, 0.8, 0.0, 0.1], 16,
+                           ["phi", "rho", "u"], ["f"],
+                           {"LDW_HARNESS_LE": "2 0.05", "LDW_HARNESS_LE_PROFILE": "1"}),
+    "le_binary16_gmu_s10": (["binary", 16, 10, 21, 2, A, B, K, MS, 0.8, 0.0, 0.1], 16,
+                            ["phi", "rho", "u"], ["f"],
+                            {"LDW_HARNESS_LE": "2 0.05", "LDW_HARNESS_LE_PROFILE": "1",
+                             "LDW_HARNESS_GRADMU": "0.0005 0.0 -0.001"}),
 }
 
 NSAMPLE = 256
@@ -158,6 +193,12 @@ def main():
             if len(spec) > 4:
                 out["stencil"] = np.array(int(spec[4].get("LDW_HARNESS_STENCIL", 7)))
                 out["conserve"] = np.array(int(spec[4].get("LDW_HARNESS_CONSERVE", 0)))
+                if "LDW_HARNESS_GRADMU" in spec[4]:
+                    out["grad_mu"] = np.array([float(v) for v in spec[4]["LDW_HARNESS_GRADMU"].split()])
+                if "LDW_HARNESS_METHOD" in spec[4]:
+                    out["force_method"] = np.array(int(spec[4]["LDW_HARNESS_METHOD"]))
+                if "LDW_HARNESS_NHALO" in spec[4]:
+                    out["nhalo"] = np.array(int(spec[4]["LDW_HARNESS_NHALO"]))
                 if "LDW_HARNESS_LE" in spec[4]:
                     le = spec[4]["LDW_HARNESS_LE"].split()
                     out["le_planes"] = np.array(int(le[0]))

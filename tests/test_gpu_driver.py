@@ -7,9 +7,13 @@ uses the 27-point gradient), token by token with the
 absolute tolerance 1e-12 of the reference's tests/awk-fp-diff.sh.
 
 Lees-Edwards planes: the reference's regression inputs
-tests/regression/d3q19-short/serial-le3d-st{1,2,3,5,6,7} (two planes of
-steady shear on 32^3 from the shear-profile start, advection orders 1-3,
+tests/regression/d3q19-short/serial-le3d-st{1,...,8} (two planes of
+steady shear on 32^3 from the shear-profile start, advection orders 1-4,
 the 7- and 27-point gradients) and their committed logs.
+
+External chemical potential gradient: serial-muex-st1 (grad_mu with the
+phi_gradmu force, a droplet on 32 x 32 x 16, the 27-point gradient) and its
+committed log.
 
 Restart: the reference's tests/regression/d3q19-io/iodrop-mpi1-io{1,2,3}
 run a droplet 0 -> 20 in one go, 0 -> 10 writing a configuration, and
@@ -77,7 +81,7 @@ def run_driver(name, tmp_path):
                                   "c2-m025", "gradmu-trt", "single-bgk", "spin7-c1", "spin7-c2",
                                   "serial-le3d-st1", "serial-le3d-st2", "serial-le3d-st3",
                                   "serial-le3d-st5", "serial-le3d-st6", "serial-le3d-st7",
-                                  "le-t0"])
+                                  "le-t0", "serial-muex-st1", "serial-le3d-st4", "serial-le3d-st8"])
 def test_driver_matches_reference_log(name, tmp_path):
     got = run_driver(name, tmp_path)
     want = open(os.path.join(LOGS, name + ".ref")).read()

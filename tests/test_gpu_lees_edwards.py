@@ -113,7 +113,7 @@ def test_le_gradient_readback_after_single_pass_step(stencil):
     b.close()
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
 def test_le_cahn_hilliard_golden(golden, order):
     g = golden(f"le_ch16_o{order}")
     n = int(g["n"])
@@ -147,15 +147,17 @@ def test_le_lb_boundary_conditions_golden(golden):
 
 @pytest.mark.parametrize("case", ["le_binary16_o1_s20", "le_binary16_o3_g27_s10",
                                   "le_binary24_p3_o2_s12", "le_binary16_c1_s10",
-                                  "le_binary16_t0_s12"])
+                                  "le_binary16_t0_s12", "le_binary16_o4_s10",
+                                  "le_binary16_gmu_s10"])
 def test_le_binary_step_golden(golden, case):
     g = golden(case)
     args = [str(a) for a in g["args"]]
     n, nsteps, seed, order = int(args[1]), int(args[2]), int(args[3]), int(args[4])
     a, b_, k, m, tau, phi0, amp = map(float, args[5:12])
     print(f"case {case}", flush=True)
+    gm = tuple(float(v) for v in g["grad_mu"]) if "grad_mu" in g else (0.0, 0.0, 0.0)
     b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, order=order, tau=tau,
-                      stencil=int(g["stencil"]), conserve=int(g["conserve"]), **_le(g))
+                      stencil=int(g["stencil"]), conserve=int(g["conserve"]), grad_mu=gm, **_le(g))
     b.init_spinodal(seed, phi0, amp)
     if int(g["le_profile"]):
         b.le_init_shear_profile()
@@ -175,6 +177,7 @@ def test_le_binary_step_golden(golden, case):
     ((20, 12, 10), 1, 0.17, 2, 27, 0, True, 8),       # one plane: one slab that wraps in x
     ((24, 8, 6), 4, 0.17, 1, 7, 1, False, 8),         # 6-deep slabs, the shallowest allowed
     ((24, 48, 48), 2, 0.13, 2, 7, 0, True, 4),        # ny*nz = 2304: the plane sums stage 3 tiles
+    ((30, 14, 18), 3, 0.21, 4, 27, 1, True, 8),       # advection order 4
 ])
 def test_le_binary_vs_oracle(shape, planes, uy, order, stencil, conserve, profile, nsteps):
     """Non-cubic lattices, one to four planes, displacements that wrap in y,
@@ -331,5 +334,17 @@ def test_conserve_precision_mismatch_refused(planes):
     with pytest.raises(RuntimeError, match="differ in precision"):
         b.step(1)
     assert b.time() == t0
+    assert np.array_equal(b.phi_field(), phi0)
+    b.close()
+
+
+def test_le_order5_refused():
+    """Order 5 reads phi three sites across a plane; the buffer planes hold
+    two (nhalo 2), so the step refuses it on the host before any launch."""
+    b = L.BinaryFluid((24, 12, 12), tau=1.0, order=5, le_planes=2, le_uy=0.05)
+    b.init_spinodal(5)
+    phi0 = b.phi_field()
+    with pytest.raises(RuntimeError, match="advection order 5"):
+        b.step(1)
     assert np.array_equal(b.phi_field(), phi0)
     b.close()

@@ -62,14 +62,22 @@ def test_propagation_golden(golden):
     s.close()
 
 
+def _gm(g):
+    return tuple(float(v) for v in g["grad_mu"]) if "grad_mu" in g else (0.0, 0.0, 0.0)
+
+
 @pytest.mark.parametrize("case,method", [("force_stress", L.FORCE_STRESS_DIVERGENCE),
-                                         ("force_gradmu", L.FORCE_PHI_GRADMU)])
+                                         ("force_gradmu", L.FORCE_PHI_GRADMU),
+                                         ("force_gradmu_ext", L.FORCE_PHI_GRADMU)])
 def test_force_golden(golden, case, method):
+    """force_gradmu_ext: an external chemical potential gradient adds
+    -phi grad_mu to the phi grad mu force (phi_grad_mu.c:123-160)."""
     g = golden(case)
     n = int(g["n"])
     args = [str(a) for a in g["args"]]
     a, b_, k = map(float, args[4:7])
-    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, tau=1.0, force_method=method)
+    print(f"case {case}", flush=True)
+    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, tau=1.0, force_method=method, grad_mu=_gm(g))
     b.set_phi(g["phi"])
     b.f_zero()
     b.force_stage()
@@ -140,13 +148,16 @@ def test_gradient_and_mu_readback_golden(golden, case, stencil):
     b.close()
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
-def test_cahn_hilliard_golden(golden, order):
-    g = golden(f"ch_order{order}")
+@pytest.mark.parametrize("case,order", [("ch_order1", 1), ("ch_order2", 2), ("ch_order3", 3),
+                                        ("ch_order4", 4), ("ch_order5", 5), ("ch_gradmu_o2", 2)])
+def test_cahn_hilliard_golden(golden, case, order):
+    """Advection orders 1-5 and the external gradient on the face fluxes."""
+    g = golden(case)
     n = int(g["n"])
     args = [str(a) for a in g["args"]]
     a, b_, k, m = map(float, args[4:8])
-    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, tau=1.0, order=order)
+    print(f"case {case}", flush=True)
+    b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, tau=1.0, order=order, grad_mu=_gm(g))
     b.set_phi(g["phi_in"])
     b.set_hydro(u=g["u"])
     b.cahn_hilliard_stage()
@@ -162,14 +173,20 @@ def _binary_args(g):
 
 
 @pytest.mark.parametrize("case", ["binary16_o1_s20", "binary16_o3_s10", "binary16_o2_s5",
-                                  "binary16_g27_s10", "binary16_g27_o3_s10"])
+                                  "binary16_g27_s10", "binary16_g27_o3_s10",
+                                  "binary16_o4_s10", "binary16_o5_s10",
+                                  "binary16_gmu_s10", "binary16_gmu_stress_s10"])
 def test_binary_step_golden(golden, case):
+    """Whole steps: order 4 through the single-pass step kernel, order 5
+    through the staged kernels; grad_mu with both force methods."""
     g = golden(case)
     n, nsteps, seed, order, a, b_, k, m, tau, phi0, amp = _binary_args(g)
     stencil = int(g["stencil"]) if "stencil" in g else 7
+    gradmu = "force_method" in g and int(g["force_method"]) == 1
+    method = L.FORCE_PHI_GRADMU if gradmu else L.FORCE_STRESS_DIVERGENCE
     print(f"case {case}", flush=True)
     b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, order=order, tau=tau,
-                      stencil=stencil)
+                      stencil=stencil, force_method=method, grad_mu=_gm(g))
     b.init_spinodal(seed, phi0, amp)
     if "phi_init" in g:
         assert_exact(b.phi_field(), g["phi_init"], "phi_init")
@@ -188,7 +205,7 @@ def test_binary_step_golden(golden, case):
 
 
 @pytest.mark.parametrize("case", ["binary16_c1_s20", "binary16_c1_o3_s10", "binary16_c2_s20",
-                                  "binary16_c2_o2_s10"])
+                                  "binary16_c2_o2_s10", "binary16_o4_g27_c1_s10"])
 def test_binary_conserve_golden(golden, case):
     """cahn_hilliard_options_conserve 1 (compensated per-site update) and 2
     (global subtraction with the site-order sum), bit-exact against the
@@ -198,8 +215,9 @@ def test_binary_conserve_golden(golden, case):
     n, nsteps, seed, order, a, b_, k, m, tau, phi0, amp = _binary_args(g)
     conserve = int(g["conserve"])
     print(f"case {case} conserve {conserve}", flush=True)
+    stencil = int(g["stencil"]) if "stencil" in g else 7
     b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, order=order, tau=tau,
-                      conserve=conserve)
+                      conserve=conserve, stencil=stencil)
     b.init_spinodal(seed, phi0, amp)
     b.step(nsteps)
     assert_exact(b.phi_field(), g["phi"], "phi")
@@ -257,16 +275,24 @@ def test_single_fluid_c1_golden(golden):
     s.close()
 
 
-@pytest.mark.parametrize("shape,order,nsteps", [((24, 20, 36), 1, 6), ((16, 40, 12), 3, 4),
-                                                ((33, 17, 9), 2, 3)])
-def test_binary_nonuniform_box_vs_oracle(shape, order, nsteps):
-    """Non-cubic, odd sizes: the periodic wrap in every kernel."""
-    fe = O.fe_param(order=order, mobility=0.25)
+@pytest.mark.parametrize("shape,order,nsteps,method,gm", [
+    ((24, 20, 36), 1, 6, 0, (0.0, 0.0, 0.0)), ((16, 40, 12), 3, 4, 0, (0.0, 0.0, 0.0)),
+    ((33, 17, 9), 2, 3, 0, (0.0, 0.0, 0.0)),
+    ((21, 34, 70), 4, 5, 0, (0.0, 0.0, 0.0)),           # order 4, single-pass step, tile edges
+    ((19, 23, 11), 5, 5, 1, (0.0, 0.0, 0.0)),           # order 5 wraps three sites on 11
+    ((25, 12, 66), 4, 4, 1, (2e-4, -1e-4, 3e-4)),      # grad_mu with phi grad mu
+    ((14, 30, 20), 2, 4, 0, (-3e-4, 0.0, 1e-4))])      # grad_mu, stress divergence
+def test_binary_nonuniform_box_vs_oracle(shape, order, nsteps, method, gm):
+    """Non-cubic, odd sizes: the periodic wrap in every kernel, for every
+    advection order and with an external chemical potential gradient."""
+    print(f"case {shape} order={order} method={method} grad_mu={gm}", flush=True)
+    fe = O.fe_param(order=order, mobility=0.25, force_method=method, grad_mu=gm)
     lb = O.lb_param(tau=0.9)
     phi0 = O.spinodal(shape, 99, 0.05, 0.1)
     f0 = O.equilibrium(shape, rho_uniform=1.0)
     f1, phi1, rho1, u1 = O.binary_steps(shape, nsteps, lb, fe, f0, phi0)
-    b = L.BinaryFluid(shape, order=order, tau=0.9, mobility=0.25)
+    b = L.BinaryFluid(shape, order=order, tau=0.9, mobility=0.25, grad_mu=gm,
+                      force_method=L.FORCE_PHI_GRADMU if method else L.FORCE_STRESS_DIVERGENCE)
     b.init_spinodal(99, 0.05, 0.1)
     b.step(nsteps)
     assert_exact(b.phi_field(), phi1, "phi")
@@ -277,18 +303,29 @@ def test_binary_nonuniform_box_vs_oracle(shape, order, nsteps):
     b.close()
 
 
-@pytest.mark.parametrize("shape,conserve,precision,order", [
-    ((16, 16, 16), 0, 64, 1),
-    ((20, 16, 24), 1, 64, 2),      # compensated update in the single-pass step
-    ((20, 16, 24), 2, 64, 3),      # global subtraction after the single-pass step
-    ((20, 16, 24), 1, 32, 1),
-    ((20, 16, 24), 2, 32, 2),
+NO_GM = (0.0, 0.0, 0.0)
+
+
+@pytest.mark.parametrize("shape,conserve,precision,order,method,gm", [
+    ((16, 16, 16), 0, 64, 1, 0, NO_GM),
+    ((20, 16, 24), 1, 64, 2, 0, NO_GM),      # compensated update in the single-pass step
+    ((20, 16, 24), 2, 64, 3, 0, NO_GM),      # global subtraction after the single-pass step
+    ((20, 16, 24), 1, 32, 1, 0, NO_GM),
+    ((20, 16, 24), 2, 32, 2, 0, NO_GM),
+    ((20, 16, 24), 0, 64, 4, 0, NO_GM),      # order 4 in the single-pass step
+    ((20, 16, 24), 1, 32, 4, 1, NO_GM),
+    ((20, 16, 24), 0, 64, 5, 0, NO_GM),      # order 5: staged kernels either way
+    ((20, 16, 24), 0, 64, 3, 1, (1e-3, 2e-3, -1e-3)),   # grad_mu with phi grad mu
+    ((20, 16, 24), 1, 32, 2, 1, (1e-3, 2e-3, -1e-3)),
 ])
-def test_staged_calls_equal_fused_step(shape, conserve, precision, order):
+def test_staged_calls_equal_fused_step(shape, conserve, precision, order, method, gm):
     """The reference's call sequence through the C ABI equals ldw_step_binary,
-    with cahn_hilliard_options_conserve 0, 1 and 2, FP64 and FP32."""
-    print(f"case staged vs fused {shape} conserve={conserve} fp{precision} order={order}", flush=True)
-    kw = dict(tau=1.0, conserve=conserve, precision=precision, order=order)
+    with cahn_hilliard_options_conserve 0, 1 and 2, FP64 and FP32, advection
+    orders 1-5 and an external chemical potential gradient."""
+    print(f"case staged vs fused {shape} conserve={conserve} fp{precision} order={order} "
+          f"method={method} grad_mu={gm}", flush=True)
+    kw = dict(tau=1.0, conserve=conserve, precision=precision, order=order, grad_mu=gm,
+              force_method=L.FORCE_PHI_GRADMU if method else L.FORCE_STRESS_DIVERGENCE)
     nsteps = 3 if conserve == 0 else 6
     b1 = L.BinaryFluid(shape, **kw)
     b1.init_spinodal(5)

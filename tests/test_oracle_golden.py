@@ -50,12 +50,19 @@ def test_propagate(golden):
     assert_exact(O.propagate((n, n, n), g["f_in"]), g["f_out"], "f")
 
 
-@pytest.mark.parametrize("case,method", [("force_stress", 0), ("force_gradmu", 1)])
+def _grad_mu(g):
+    return tuple(float(v) for v in g["grad_mu"]) if "grad_mu" in g else (0.0, 0.0, 0.0)
+
+
+@pytest.mark.parametrize("case,method", [("force_stress", 0), ("force_gradmu", 1),
+                                         ("force_gradmu_ext", 1)])
 def test_gradient_mu_force(golden, case, method):
+    """force_gradmu_ext: -phi grad_mu added to the phi grad mu force
+    (phi_grad_mu.c:123-160, 509-534)."""
     g = golden(case)
     n = int(g["n"])
     s = (n, n, n)
-    fe = O.fe_param(force_method=method)
+    fe = O.fe_param(force_method=method, grad_mu=_grad_mu(g))
     grad, delsq = O.grad7(s, g["phi"])
     assert_exact(grad, g["grad"], "grad")
     assert_exact(delsq, g["delsq"], "delsq")
@@ -63,14 +70,42 @@ def test_gradient_mu_force(golden, case, method):
     assert_exact(O.force(s, fe, g["phi"], grad, delsq), g["force"], "force")
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
-def test_cahn_hilliard(golden, order):
-    g = golden(f"ch_order{order}")
+@pytest.mark.parametrize("case,order", [("ch_order1", 1), ("ch_order2", 2), ("ch_order3", 3),
+                                        ("ch_order4", 4), ("ch_order5", 5), ("ch_gradmu_o2", 2)])
+def test_cahn_hilliard(golden, case, order):
+    """Orders 1-5 (advection.c:538-1430; order 5 from a reference built with a
+    coordinate halo of 3, which its stencil needs) and the external
+    chemical-potential gradient on the face fluxes
+    (phi_cahn_hilliard.c:1373-1398)."""
+    g = golden(case)
     n = int(g["n"])
     s = (n, n, n)
-    fe = O.fe_param(order=order)
+    fe = O.fe_param(order=order, grad_mu=_grad_mu(g))
     _, delsq = O.grad7(s, g["phi_in"])
     assert_exact(O.cahn_hilliard(s, fe, g["phi_in"], delsq, g["u"]), g["phi_out"], "phi")
+
+
+def test_orders_differ(golden):
+    """Each scheme gives its own update of the same input (a case routed to
+    the wrong order would fail here as well as above)."""
+    outs = [golden(f"ch_order{k}")["phi_out"] for k in range(1, 6)]
+    for i in range(5):
+        for j in range(i):
+            assert np.any(outs[i] != outs[j])
+
+
+def test_uniform_grad_mu_only_rounds_the_update(golden):
+    """A uniform external gradient shifts both faces of a cell alike, so in
+    exact arithmetic the divergence is unchanged: against the update without
+    it the difference is rounding only."""
+    g = golden("ch_gradmu_o2")
+    n = int(g["n"])
+    s = (n, n, n)
+    _, delsq = O.grad7(s, g["phi_in"])
+    plain = O.cahn_hilliard(s, O.fe_param(order=2), g["phi_in"], delsq, g["u"])
+    d = np.abs(plain - g["phi_out"])
+    assert np.any(d != 0.0)
+    assert np.max(d) < 64 * np.finfo(float).eps
 
 
 def _binary(g):
@@ -81,8 +116,9 @@ def _binary(g):
     lb = O.lb_param(tau=tau)
     stencil = int(g["stencil"]) if "stencil" in g else 7
     conserve = int(g["conserve"]) if "conserve" in g else 0
+    method = int(g["force_method"]) if "force_method" in g else 0
     fe = O.fe_param(a=a, b=b, kappa=k, mobility=m, order=order, stencil=stencil,
-                    conserve=conserve)
+                    conserve=conserve, force_method=method, grad_mu=_grad_mu(g))
     phi = O.spinodal(s, seed, phi0, amp)
     f = O.equilibrium(s, rho_uniform=1.0)
     return s, nsteps, lb, fe, phi, f
@@ -105,7 +141,9 @@ def test_gradient27_mu_force(golden, case, method):
 
 
 @pytest.mark.parametrize("case", ["binary16_o1_s20", "binary16_o3_s10", "binary16_o2_s5",
-                                  "binary16_g27_s10", "binary16_g27_o3_s10"])
+                                  "binary16_g27_s10", "binary16_g27_o3_s10",
+                                  "binary16_o4_s10", "binary16_o5_s10",
+                                  "binary16_gmu_s10", "binary16_gmu_stress_s10"])
 def test_binary_steps_full_fields(golden, case):
     g = golden(case)
     s, nsteps, lb, fe, phi, f = _binary(g)
@@ -122,7 +160,7 @@ def test_binary_steps_full_fields(golden, case):
 
 
 @pytest.mark.parametrize("case", ["binary16_c1_s20", "binary16_c1_o3_s10", "binary16_c2_s20",
-                                  "binary16_c2_o2_s10"])
+                                  "binary16_c2_o2_s10", "binary16_o4_g27_c1_s10"])
 def test_binary_steps_conserve(golden, case):
     """cahn_hilliard_options_conserve 1 (per-site Kahan update with the
     compensation field, phi_cahn_hilliard.c:1181-1218) and 2 (subtraction of

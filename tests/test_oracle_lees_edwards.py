@@ -52,7 +52,7 @@ def test_le_force_golden(golden, case, stencil):
     assert_exact(O.le_force((n, n, n), fe, _le(g), t, g["phi"]), g["force"], "force")
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
 def test_le_cahn_hilliard_golden(golden, order):
     g = golden(f"le_ch16_o{order}")
     n = int(g["n"])
@@ -72,7 +72,8 @@ def test_le_lb_bc_golden(golden):
 
 @pytest.mark.parametrize("case", ["le_binary16_o1_s20", "le_binary16_o3_g27_s10",
                                   "le_binary24_p3_o2_s12", "le_binary16_c1_s10",
-                                  "le_binary16_t0_s12"])
+                                  "le_binary16_t0_s12", "le_binary16_o4_s10",
+                                  "le_binary16_gmu_s10"])
 def test_le_binary_steps_golden(golden, case):
     g = golden(case)
     args = [str(a) for a in g["args"]]
@@ -80,8 +81,9 @@ def test_le_binary_steps_golden(golden, case):
     a, b, k, m, tau, phi0, amp = map(float, args[5:12])
     s = (n, n, n)
     lb = O.lb_param(tau=tau)
+    gm = tuple(float(v) for v in g["grad_mu"]) if "grad_mu" in g else (0.0, 0.0, 0.0)
     fe = O.fe_param(a=a, b=b, kappa=k, mobility=m, order=order, stencil=int(g["stencil"]),
-                    conserve=int(g["conserve"]))
+                    conserve=int(g["conserve"]), grad_mu=gm)
     le = _le(g)
     phi = O.spinodal(s, seed, phi0, amp)
     if int(g["le_profile"]):
