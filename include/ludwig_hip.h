@@ -432,12 +432,17 @@ int phi_cahn_hilliard(phi_ch_t * pch, fe_t * fe, field_t * phi,
  *   0  the forward step;
  *   1  the update as per-site compensated sums, the compensation kept
  *      between steps (phi_cahn_hilliard.c:1059-1097, 1181-1218);
- *   2  the forward step, then phi -= (S - field_init_sum)/N with S the
- *      plain sum of phi in site order, as the reference forms it on one
- *      thread (phi_cahn_hilliard.c:1102-1175, 1225-1327).  Needs
- *      cahn_hilliard_stats_time0() first.  The sum is one serial chain of
- *      additions, so this option costs time in proportion to N.
- * Either option runs the staged kernels, not the single-pass step. */
+ *   2  the forward step, then phi -= (S - field_init_sum)/N with S the sum
+ *      of phi (phi_cahn_hilliard.c:1102-1175, 1225-1327).  Needs
+ *      cahn_hilliard_stats_time0() first.  The reference's S is a plain
+ *      sum whose order it does not fix (atomics on the GPU, per-thread
+ *      partials under OpenMP); S here is, by default, a compensated tree
+ *      reduction (deterministic, within a few ulp of the exact sum).
+ *      ldw_conserve2_site_order(1) (or LDW_CONSERVE2_SITE_ORDER=1 in the
+ *      environment) takes the plain sum in site order on one thread
+ *      instead, the reference's one-thread order, bit-identical with it;
+ *      that sum is one serial chain of additions (~10 ms at 128^3). */
+int ldw_conserve2_site_order(int on);
 /* field_init_sum of phi, as the reference computes it on one thread
  * (cahn_hilliard_stats.c:59-76) */
 int cahn_hilliard_stats_time0(phi_ch_t * pch, field_t * phi, map_t * map);

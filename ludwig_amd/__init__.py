@@ -161,6 +161,7 @@ def lib():
         "phi_ch_create": [vp, vp, vp, ctypes.POINTER(PhiChInfo), pvp], "phi_ch_free": [vp],
         "phi_cahn_hilliard": [vp, vp, vp, vp, vp, vp],
         "advection_order_set": [ctypes.c_int],
+        "ldw_conserve2_site_order": [ctypes.c_int],
         "cahn_hilliard_stats_time0": [vp, vp, vp],
         "field_phi_init_spinodal": [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double],
         "ldw_lb_init_random_velocity": [vp, ctypes.c_int, ctypes.c_double],

@@ -1645,8 +1645,35 @@ struct phi_ch_s {
   lees_edw_t * le;
   int prec;
   void * d_csum;           // conserve 1: per-site compensation, kept between steps
-  double * d_sum;          // conserve 2: the site-order sum of phi
+  double * d_sum;          // conserve 2: the sum of phi (3 doubles: rounded sum, pair)
+  double * d_work;         // conserve 2: partial sums of the tree reduction
 };
+
+/* Conserve 2 subtracts the drift of the plain sum of phi from the initial
+ * sum (phi_cahn_hilliard.c:1102-1175).  That sum's rounding, not the true
+ * drift, dominates the difference, so the correction depends on the order
+ * of the additions, which the reference itself does not fix (atomics on
+ * the GPU, per-thread partial sums under OpenMP).  By default the sum here
+ * is the compensated tree reduction (Neumaier pairs merged with TwoSum,
+ * deterministic, within a few ulp of the exact sum, a few microseconds).
+ * ldw_conserve2_site_order(1), or LDW_CONSERVE2_SITE_ORDER=1 in the
+ * environment, selects the plain sum in site order instead: one thread
+ * adds every value in turn, as the reference's one-thread build does, so
+ * results are bit-identical with it (about 10 ms per step at 128^3). */
+static int g_conserve2_site_order = -1;
+
+static int conserve2_site_order(void) {
+  if (g_conserve2_site_order < 0) {
+    const char * e = getenv("LDW_CONSERVE2_SITE_ORDER");
+    g_conserve2_site_order = (e && atoi(e) != 0) ? 1 : 0;
+  }
+  return g_conserve2_site_order;
+}
+
+extern "C" int ldw_conserve2_site_order(int on) {
+  g_conserve2_site_order = on ? 1 : 0;
+  return 0;
+}
 
 /* phi_cahn_hilliard.c:128-163: conserve 1 keeps a compensation field
  * ("compensated sum"), zero at the start */
@@ -1669,7 +1696,8 @@ extern "C" int phi_ch_create(pe_t * pe, cs_t * cs, lees_edw_t * le, phi_ch_info_
     HIP_CHECK(hipStreamSynchronize(g_stream));
   }
   if (p->info.conserve == 2) {
-    if (dmalloc((void **) &p->d_sum, sizeof(double))) return -1;
+    if (dmalloc((void **) &p->d_sum, 3*sizeof(double))) return -1;
+    if (dmalloc((void **) &p->d_work, 2*sizeof(double)*(size_t) ldw::sum_blocks(geom_of(cs).nsite))) return -1;
   }
   *pch = p;
   return 0;
@@ -1680,6 +1708,7 @@ extern "C" int phi_ch_free(phi_ch_t * pch) {
   if (g_ready) (void) hipStreamSynchronize(g_stream);
   if (pch->d_csum) (void) hipFree(pch->d_csum);
   if (pch->d_sum) (void) hipFree(pch->d_sum);
+  if (pch->d_work) (void) hipFree(pch->d_work);
   free(pch);
   return 0;
 }
@@ -1721,13 +1750,18 @@ extern "C" int cahn_hilliard_stats_time0(phi_ch_t * pch, field_t * phi, map_t *)
 static int ch_subtract_drift(phi_ch_t * pch, field_t * phi) {
   if (pch->info.conserve != 2) return 0;
   const Geom g = phi->geo;
+  const bool site_order = conserve2_site_order();
   if (phi->prec == 64) {
-    ldw::launch_site_order_sum_f64((const double *) phi->d_data[phi->cur], g.nsite, pch->d_sum, g_stream);
+    const double * a = (const double *) phi->d_data[phi->cur];
+    if (site_order) ldw::launch_site_order_sum_f64(a, g.nsite, pch->d_sum, g_stream);
+    else ldw::launch_klein_sum_f64(a, g.nsite, pch->d_work, pch->d_sum, g_stream);
     ldw::launch_subtract_drift_f64((double *) phi->d_data[phi->cur], g.nsite, pch->d_sum,
 				   phi->init_sum, g_stream);
   }
   else {
-    ldw::launch_site_order_sum_f32((const float *) phi->d_data[phi->cur], g.nsite, pch->d_sum, g_stream);
+    const float * a = (const float *) phi->d_data[phi->cur];
+    if (site_order) ldw::launch_site_order_sum_f32(a, g.nsite, pch->d_sum, g_stream);
+    else ldw::launch_klein_sum_f32(a, g.nsite, pch->d_work, pch->d_sum, g_stream);
     ldw::launch_subtract_drift_f32((float *) phi->d_data[phi->cur], g.nsite, pch->d_sum,
 				   phi->init_sum, g_stream);
   }

@@ -24,7 +24,7 @@ struct Scratch {
 
 Scratch g_wgrad, g_bias, g_err, g_sum, g_sumsq, g_restore;
 
-int scratch(Scratch & s, size_t bytes, void ** out) {
+int scratch_buffer(Scratch & s, size_t bytes, void ** out) {
   if (s.bytes < bytes) {
     hipStream_t st = ldw::api::stream();
     LDW_HIP_CHECK(hipStreamSynchronize(st));
@@ -90,7 +90,7 @@ extern "C" int ldw_conv3d_bwd_weight(const void * x, const void * dy, float * dw
   int64_t chunk;
   const int64_t n = ldw::conv3_wgrad_workspace(batch, nx, ny, nz, cin, cout, &nchunk, &chunk);
   void * work = nullptr;
-  if (scratch(g_wgrad, (size_t) n*sizeof(float), &work)) return -1;
+  if (scratch_buffer(g_wgrad, (size_t) n*sizeof(float), &work)) return -1;
   KCHECK(ldw::launch_conv3_wgrad((const uint16_t *) x, (const uint16_t *) dy, dw, (float *) work, batch,
 				 nx, ny, nz, cin, cout, accumulate, ldw::api::stream()));
   return 0;
@@ -118,7 +118,7 @@ extern "C" int ldw_nn_silu_bwd(const float * z, const float * da, float * dz, vo
 extern "C" int ldw_nn_bias_grad(const float * dz, float * db, long long npos, int c, int accumulate) {
   READY();
   void * work = nullptr;
-  if (scratch(g_bias, (size_t) ldw::nn_bias_grad_workspace(npos, c)*sizeof(float), &work)) return -1;
+  if (scratch_buffer(g_bias, (size_t) ldw::nn_bias_grad_workspace(npos, c)*sizeof(float), &work)) return -1;
   KCHECK(ldw::nn_bias_grad(dz, db, (float *) work, npos, c, accumulate, ldw::api::stream()));
   return 0;
 }
@@ -214,12 +214,12 @@ extern "C" int ldw_nn_mse(const float * y, int cy, const float * tgt, long long 
   for (int c = 0; c < 4; c++) cw.w[c] = weights ? weights[c] : 1.0f;
   hipStream_t st = ldw::api::stream();
   void * err = nullptr;
-  if (scratch(g_err, (size_t) 4*npos*sizeof(float), &err)) return -1;
+  if (scratch_buffer(g_err, (size_t) 4*npos*sizeof(float), &err)) return -1;
   KCHECK(ldw::nn_mse(y, cy, tgt, npos, (float *) err, (uint16_t *) dy, dyf, cw, st));
   if (loss) {
     const int nb = ldw::sum_blocks(4*npos);
     void * work = nullptr;
-    if (scratch(g_sum, (size_t) (3*nb + 3)*sizeof(double), &work)) return -1;
+    if (scratch_buffer(g_sum, (size_t) (3*nb + 3)*sizeof(double), &work)) return -1;
     double * w = (double *) work;
     ldw::launch_klein_sum_f32((const float *) err, 4*npos, w + 3, w, st);
     LDW_HIP_CHECK(hipGetLastError());
@@ -251,7 +251,7 @@ extern "C" int ldw_nn_sum(const float * a, long long n, double * out) {
   if (out == nullptr) LDW_FAIL("ldw_nn_sum: null output");
   hipStream_t st = ldw::api::stream();
   void * work = nullptr;
-  if (scratch(g_sumsq, 513*sizeof(double), &work)) return -1;
+  if (scratch_buffer(g_sumsq, 513*sizeof(double), &work)) return -1;
   KCHECK(ldw::nn_sum(a, n, (double *) work, st));
   LDW_HIP_CHECK(hipMemcpyAsync(out, work, sizeof(double), hipMemcpyDeviceToHost, st));
   LDW_HIP_CHECK(hipStreamSynchronize(st));
@@ -263,7 +263,7 @@ extern "C" int ldw_nn_restore_sums(const float * now, float * out, long long ns,
   if (now == nullptr || out == nullptr) LDW_FAIL("ldw_nn_restore_sums: null buffer");
   if (ns < 1 || nplane < 1 || nplane > 64) LDW_FAIL("ldw_nn_restore_sums: ns %lld nplane %d", ns, nplane);
   void * work = nullptr;
-  if (scratch(g_restore, sizeof(double)*ldw::nn_restore_sums_scratch(nplane), &work)) return -1;
+  if (scratch_buffer(g_restore, sizeof(double)*ldw::nn_restore_sums_scratch(nplane), &work)) return -1;
   KCHECK(ldw::nn_restore_sums(now, out, ns, nplane, (double *) work, ldw::api::stream()));
   return 0;
 }
@@ -279,7 +279,7 @@ extern "C" int ldw_nn_sumsq(const float * a, long long n, double * out) {
   if (out == nullptr) LDW_FAIL("ldw_nn_sumsq: null output");
   hipStream_t st = ldw::api::stream();
   void * work = nullptr;
-  if (scratch(g_sumsq, 513*sizeof(double), &work)) return -1;
+  if (scratch_buffer(g_sumsq, 513*sizeof(double), &work)) return -1;
   KCHECK(ldw::nn_sumsq(a, n, (double *) work, st));
   LDW_HIP_CHECK(hipMemcpyAsync(out, work, sizeof(double), hipMemcpyDeviceToHost, st));
   LDW_HIP_CHECK(hipStreamSynchronize(st));

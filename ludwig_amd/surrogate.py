@@ -358,24 +358,35 @@ def loss_and_grad(y, tgt, npos, dyb, dyf, weights=(1.0, 1.0, 1.0, 1.0)):
 
 class Dataset:
     """(phi, u) snapshots of every replica of an ensemble, kept on the
-    device: snaps[r][k] is replica r at step k*every, 4 SoA planes of floats."""
+    device: snaps[r][k] is replica r at step k*every, 4 SoA planes of floats.
+    With offset c > 0 also osnaps[r][k], replica r at step k*every + c: the
+    times a jump pipeline with c correction steps reaches."""
 
-    def __init__(self, ensemble, nsnap, every, replicas=None):
+    def __init__(self, ensemble, nsnap, every, replicas=None, offset=0):
         self.specs = ensemble.specs
         self.shapes = ensemble.shapes
         self.every = every
+        self.offset = int(offset)
+        if not 0 <= self.offset < every:
+            raise ValueError(f"offset {offset} outside [0, {every})")
         self.replicas = list(range(len(self.specs))) if replicas is None else list(replicas)
         self.snaps = {r: [] for r in self.replicas}
-        self._take(ensemble)
+        self.osnaps = {r: [] for r in self.replicas} if self.offset else None
+        self._take(ensemble, self.snaps)
         for k in range(nsnap):
-            ensemble.step(every)
-            self._take(ensemble)
+            if self.offset:
+                ensemble.step(self.offset)
+                self._take(ensemble, self.osnaps)
+                ensemble.step(every - self.offset)
+            else:
+                ensemble.step(every)
+            self._take(ensemble, self.snaps)
 
-    def _take(self, e):
+    def _take(self, e, into):
         for r in self.replicas:
             d = DeviceArray(4 * int(np.prod(self.shapes[r])), np.float32)
             e.snapshot_device(r, d.ptr)
-            self.snaps[r].append(d)
+            into[r].append(d)
 
     def pairs(self, lag=1, kmin=0):
         """(replica, snapshot) of every pair k -> k + lag with k >= kmin."""
@@ -398,10 +409,11 @@ class Dataset:
         return Normaliser(1.0 / rms(us), 1.0 / rms(dps), 1.0 / rms(dus))
 
     def free(self):
-        for r in self.snaps:
-            for d in self.snaps[r]:
-                d.free()
-        self.snaps = {}
+        for series in (self.snaps, self.osnaps or {}):
+            for r in series:
+                for d in series[r]:
+                    d.free()
+        self.snaps, self.osnaps = {}, None
 
 
 class Trainer:
@@ -433,12 +445,17 @@ class Trainer:
         """One optimiser step (or, train=False, only the loss) on `items`,
         a list of (replica, snapshot) of one lattice size.  zero=True scores
         the zero-change prediction instead of the network."""
-        shape = tuple(data.shapes[items[0][0]])
+        return self.step_arrays([(data.snaps[r][k], data.snaps[r][k + lag], data.specs[r])
+                                 for r, k in items], tuple(data.shapes[items[0][0]]), lr, train, zero)
+
+    def step_arrays(self, items, shape, lr, train=True, zero=False):
+        """The same on explicit (state now, state one jump later, replica
+        spec) device arrays of one lattice shape."""
         ns = int(np.prod(shape))
         io = self._bufs(shape, len(items))
-        for j, (r, k) in enumerate(items):
-            encode(data.snaps[r][k], ns, data.specs[r], self.norm, io["x"], j * ns)
-            target(data.snaps[r][k], data.snaps[r][k + lag], ns, self.norm, io["t"], j * ns)
+        for j, (now, nxt, spec) in enumerate(items):
+            encode(now, ns, spec, self.norm, io["x"], j * ns)
+            target(now, nxt, ns, self.norm, io["t"], j * ns)
         if zero:
             if "zero" not in io:
                 io["zero"] = DeviceArray.from_host(np.zeros(len(items) * ns * COUT, np.float32), np.float32)
@@ -482,6 +499,121 @@ class Trainer:
 
     def evaluate(self, data, pairs, lag=1, zero=False):
         return float(np.mean([self.step(data, [p], 0.0, lag, train=False, zero=zero) for p in pairs]))
+
+    def fit_pushforward(self, data, pf, pairs, iters, lr, lr_final=None, p_pf=0.5, refresh_every=400,
+                        seed=1, log=None, max_seconds=None):
+        """Adam steps on a mix of the plain pairs (snapshot k -> k + 1) and,
+        with probability p_pf, pushforward pairs: the state the jump
+        pipeline reaches from snapshot k (one network jump, then the
+        correction steps, pf.inputs[r][k]) -> the true state one jump later
+        (data.osnaps[r][k + 2]).  The pushforward inputs are rebuilt with
+        the current network every `refresh_every` iterations."""
+        import time
+        t_start = time.perf_counter()
+        rng = np.random.default_rng(seed)
+        plain = [g for g in self.groups(data, pairs).values()]
+        pfp = [g for g in self.groups(data, pf.pairs()).values()]
+        wp = np.array([len(g) for g in plain], float); wp /= wp.sum()
+        wf = np.array([len(g) for g in pfp], float); wf /= wf.sum()
+        losses, losses_pf = [], []
+        for it in range(iters):
+            if it % refresh_every == 0:
+                t0 = time.perf_counter()
+                pf.refresh(self.net, self.norm)
+                if log:
+                    log(f"iter {it}: pushforward inputs rebuilt in {time.perf_counter() - t0:.1f} s")
+            frac = it / max(iters - 1, 1)
+            if max_seconds is not None:
+                frac = max(frac, (time.perf_counter() - t_start) / max_seconds)
+            lr_it = lr if lr_final is None else lr * (lr_final / lr) ** min(frac, 1.0)
+            if rng.random() < p_pf:
+                g = pfp[rng.choice(len(pfp), p=wf)]
+                sel = [g[i] for i in rng.choice(len(g), min(self.batch, len(g)), replace=False)]
+                items = [(pf.inputs[r][k], data.osnaps[r][k + 2], data.specs[r]) for r, k in sel]
+                losses_pf.append(self.step_arrays(items, tuple(data.shapes[sel[0][0]]), lr_it))
+            else:
+                g = plain[rng.choice(len(plain), p=wp)]
+                sel = [g[i] for i in rng.choice(len(g), min(self.batch, len(g)), replace=False)]
+                losses.append(self.step(data, sel, lr_it))
+            if log and it % 100 == 0:
+                lp = np.mean(losses[-50:]) if losses else float("nan")
+                lf = np.mean(losses_pf[-50:]) if losses_pf else float("nan")
+                log(f"iter {it} loss {lp:.4e} pushforward loss {lf:.4e} "
+                    f"({time.perf_counter() - t_start:.1f} s)")
+            if max_seconds is not None and time.perf_counter() - t_start > max_seconds:
+                if log:
+                    log(f"stopped after {it + 1} iterations: time budget {max_seconds} s")
+                break
+        return losses, losses_pf
+
+
+class Pushforward:
+    """The inputs the jump pipeline feeds the network after its first jump:
+    for each replica r and snapshot k, one network jump from data.snaps[r][k]
+    (to step (k + 1) every), then data.offset plain lattice Boltzmann
+    correction steps, giving the state at step (k + 1) every + offset.  The
+    true state one jump later is data.osnaps[r][k + 2].  The replicas are
+    advanced by an ensemble of their own (FP32-shifted storage)."""
+
+    def __init__(self, data, replicas, kmin, conserve=True, chunk=4):
+        from .ensemble import Ensemble
+        if not data.offset:
+            raise ValueError("pushforward pairs need a dataset with offset snapshots")
+        self.data = data
+        self.replicas = list(replicas)
+        self.kmin = kmin
+        self.kmax = len(data.osnaps[self.replicas[0]]) - 3    # osnaps[k + 2] must exist
+        if self.kmax < kmin:
+            raise ValueError("too few snapshots for a pushforward pair")
+        self.conserve, self.chunk = conserve, chunk
+        self.ens = Ensemble([data.specs[r] for r in self.replicas], precision=32, fatal=True)
+        self.ens.tune(reps=1)
+        self.slot = {r: j for j, r in enumerate(self.replicas)}
+        self.inputs = {r: {k: DeviceArray(4 * int(np.prod(data.shapes[r])), np.float32)
+                           for k in range(kmin, self.kmax + 1)} for r in self.replicas}
+        self._tmp = {}
+
+    def pairs(self):
+        return [(r, k) for r in self.replicas for k in range(self.kmin, self.kmax + 1)]
+
+    def refresh(self, net, norm):
+        L = lib()
+        data = self.data
+        by = {}
+        for r in self.replicas:
+            by.setdefault(tuple(data.shapes[r]), []).append(r)
+        for k in range(self.kmin, self.kmax + 1):
+            for shape, rs in by.items():
+                ns = int(np.prod(shape))
+                for c0 in range(0, len(rs), self.chunk):
+                    part = rs[c0:c0 + self.chunk]
+                    key = (len(part),) + shape
+                    if key not in self._tmp:
+                        self._tmp[key] = (DeviceArray(len(part) * ns * CIN, np.uint16),
+                                          DeviceArray(4 * ns, np.float32))
+                    x, out = self._tmp[key]
+                    for j, r in enumerate(part):
+                        encode(data.snaps[r][k], ns, data.specs[r], norm, x, j * ns)
+                    y = net.forward(x, len(part), shape, keep=False)
+                    for j, r in enumerate(part):
+                        decode(data.snaps[r][k], y, ns, norm, out, j * ns)
+                        if self.conserve:
+                            _ck(L.ldw_nn_restore_sums(_v(data.snaps[r][k]), _v(out), ns, 4),
+                                "ldw_nn_restore_sums")
+                        self.ens.load_device(self.slot[r], out.ptr)
+            self.ens.step(data.offset)
+            for r in self.replicas:
+                self.ens.snapshot_device(self.slot[r], self.inputs[r][k].ptr)
+
+    def free(self):
+        for r in self.inputs:
+            for d in self.inputs[r].values():
+                d.free()
+        for x, out in self._tmp.values():
+            x.free()
+            out.free()
+        self.inputs, self._tmp = {}, {}
+        self.ens.close()
 
 
 class Jumper:

@@ -36,7 +36,7 @@ import numpy as np
 from . import lib
 from .analysis import domain_length
 from .ensemble import Ensemble, draw_replicas
-from .surrogate import Dataset, Jumper, Trainer, UNet
+from .surrogate import Dataset, Jumper, Pushforward, Trainer, UNet
 
 
 def split_by_seed(specs, frac=0.2, seed=0):
@@ -176,13 +176,18 @@ def gain(weights=WEIGHTS, size=64, corrs=(20, 30), every=250, jumps=8, t_start=5
 
 def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=0, iters=2000,
         batch=4, lr=5e-4, jumps=8, corr=30, seed=2026, max_eval=None, train_seconds=None,
-        t_start=500, replica_kw=None, init_from=None, save_to=None, log=print):
+        t_start=500, replica_kw=None, init_from=None, save_to=None, log=print, pf_seconds=0.0,
+        pf_lr=None, p_pf=0.5, refresh_every=400, eval_corrs=None):
+    """Data, training and the held-out evaluation.  pf_seconds > 0 adds a
+    second training stage on pushforward pairs (Trainer.fit_pushforward):
+    the inputs the jump pipeline with `corr` correction steps reaches after
+    one network jump, against the true state one jump later."""
     specs = draw_replicas(nrep, seed=seed, sizes=sizes, **(replica_kw or {}))
     train_idx, val_idx = split_by_seed(specs, 0.2, seed)
     t0 = time.perf_counter()
     ens = Ensemble(specs, precision=32, fatal=True)
     ens.tune(reps=2)
-    data = Dataset(ens, steps // every, every)
+    data = Dataset(ens, steps // every, every, offset=corr if pf_seconds else 0)
     ens.close()
     _sync()
     t_data = time.perf_counter() - t0
@@ -199,14 +204,32 @@ def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=0, it
     tpairs = [(r, k) for (r, k) in data.pairs(kmin=kmin) if r in set(train_idx)]
     vpairs = [(r, k) for (r, k) in data.pairs(kmin=kmin) if r in set(val_idx)]
     t0 = time.perf_counter()
-    losses = tr.fit(data, tpairs, iters, lr=lr, lr_final=lr * 0.1, log=log, max_seconds=train_seconds)
+    losses = []
+    if iters > 0 and (train_seconds is None or train_seconds > 0):
+        losses = tr.fit(data, tpairs, iters, lr=lr, lr_final=lr * 0.1, log=log, max_seconds=train_seconds)
     _sync()
     t_train = time.perf_counter() - t0
+    losses_pf = []
+    t_pf = 0.0
+    if pf_seconds:
+        if save_to:
+            net.save(save_to[:-4] + "_stage1.npz" if save_to.endswith(".npz") else save_to + "_stage1", norm)
+        t0 = time.perf_counter()
+        pf = Pushforward(data, train_idx, kmin, conserve=True)
+        plr = pf_lr if pf_lr is not None else lr * 0.3
+        l1, losses_pf = tr.fit_pushforward(data, pf, tpairs, 10 ** 9, lr=plr, lr_final=plr * 0.1, p_pf=p_pf,
+                                           refresh_every=refresh_every, log=log, max_seconds=pf_seconds)
+        losses += l1
+        pf.free()
+        _sync()
+        t_pf = time.perf_counter() - t0
     vsub = vpairs if len(vpairs) <= 64 else [vpairs[i] for i in
                                               np.random.default_rng(1).choice(len(vpairs), 64, replace=False)]
     val_loss = tr.evaluate(data, vsub)
     zero_loss = tr.evaluate(data, vsub, zero=True)
-    log(f"training: {len(losses)} iterations in {t_train:.1f} s, last loss {np.mean(losses[-20:]):.4e}, "
+    last = float(np.mean(losses[-20:])) if losses else float("nan")
+    log(f"training: {len(losses)} plain + {len(losses_pf)} pushforward iterations in "
+        f"{t_train + t_pf:.1f} s, last loss {last:.4e}, "
         f"held-out loss {val_loss:.4e} (zero-change prediction {zero_loss:.4e})")
     data.free()
     if save_to:
@@ -219,12 +242,15 @@ def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=0, it
     if max_eval:
         ev_specs = ev_specs[:max_eval]
     result = {"replicas": nrep, "steps": steps, "every": every, "data_s": t_data,
-              "train_iters": len(losses), "train_s": t_train, "train_loss_last": float(np.mean(losses[-20:])),
+              "train_iters": len(losses), "train_s": t_train, "train_loss_last": last,
+              "pushforward_iters": len(losses_pf), "pushforward_s": t_pf,
+              "pushforward_loss_last": float(np.mean(losses_pf[-20:])) if losses_pf else None,
               "heldout_loss": val_loss, "heldout_zero_change_loss": zero_loss,
               "normaliser": norm.to_dict()}
     if ev_specs:
-        result["evaluation"] = evaluate(net, norm, ev_specs, every, jumps, corr, t_start=kmin * every,
-                                        log=log)
+        for c in (eval_corrs or (corr,)):
+            result[f"evaluation_corr{c}"] = evaluate(net, norm, ev_specs, every, jumps, c,
+                                                     t_start=kmin * every, log=log)
     return result
 
 
@@ -245,11 +271,19 @@ def main():
     ap.add_argument("--init-from", default=None, help="weights (.npz) to start from")
     ap.add_argument("--save-to", default=None, help="write the trained weights (.npz) here")
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--pf-seconds", type=float, default=0.0,
+                    help="length of the pushforward training stage (0: none)")
+    ap.add_argument("--pf-lr", type=float, default=None)
+    ap.add_argument("--p-pf", type=float, default=0.5, help="share of pushforward batches in that stage")
+    ap.add_argument("--refresh-every", type=int, default=400)
+    ap.add_argument("--eval-corr", type=int, nargs="*", default=None,
+                    help="correction steps of the evaluations (default: --corr)")
     a = ap.parse_args()
     res = run(a.replicas, a.steps, a.every, eval_size=a.eval_size, iters=a.iters, batch=a.batch, lr=a.lr,
               jumps=a.jumps, corr=a.corr, seed=a.seed, max_eval=a.max_eval or None,
-              train_seconds=a.train_seconds or None, t_start=a.t_start, init_from=a.init_from,
-              save_to=a.save_to,
+              train_seconds=a.train_seconds, t_start=a.t_start, init_from=a.init_from,
+              save_to=a.save_to, pf_seconds=a.pf_seconds, pf_lr=a.pf_lr, p_pf=a.p_pf,
+              refresh_every=a.refresh_every, eval_corrs=a.eval_corr,
               log=lambda m: print(m, flush=True))
     print(json.dumps(res))
 

@@ -66,12 +66,12 @@ def compare_logs(got, want, tol=1e-12):
             assert abs(x - y) <= tol, f"\n got  {a}\n want {b}"
 
 
-def run_driver(name, tmp_path):
+def run_driver(name, tmp_path, env=None):
     if not os.path.exists(DRIVER):
         pytest.fail("ludwig_amd/ludwig_hip is not built")
     inp = os.path.join(LOGS, name + ".inp")
     out = subprocess.run([DRIVER, inp], capture_output=True, text=True, timeout=300,
-                         cwd=tmp_path)
+                         cwd=tmp_path, env=None if env is None else dict(os.environ, **env))
     assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
     return out.stdout
 
@@ -87,6 +87,15 @@ def test_driver_matches_reference_log(name, tmp_path):
     want = open(os.path.join(LOGS, name + ".ref")).read()
     print(name, "\n".join(stat_lines(got)[-4:]), flush=True)
     compare_logs(got, want)
+
+
+@pytest.mark.gpu
+def test_driver_conserve2_tree_sum_matches_log(tmp_path):
+    """cahn_hilliard_options_conserve 2 with the library's default sum (the
+    compensated tree reduction, not the one-thread site order): the
+    statistics still match Ludwig.exe's log under the 1e-12 rule."""
+    got = run_driver("spin7-c2", tmp_path, env={"LDW_CONSERVE2_SITE_ORDER": "0"})
+    compare_logs(got, open(os.path.join(LOGS, "spin7-c2.ref")).read())
 
 
 def test_log_comparison_catches_a_change():

@@ -228,6 +228,61 @@ def test_binary_conserve_golden(golden, case):
     b.close()
 
 
+@pytest.mark.parametrize("case", ["binary16_c2_s20", "binary16_c2_o2_s10"])
+def test_conserve2_tree_sum_matches_golden(golden, case):
+    """The default conserve-2 sum (compensated tree reduction) against the
+    reference's one-thread run: the correction differs from the site-order
+    one only by that sum's rounding, so every field stays within 1e-12
+    relative (1e-12 absolute for u, whose values sit near zero)."""
+    g = golden(case)
+    n, nsteps, seed, order, a, b_, k, m, tau, phi0, amp = _binary_args(g)
+    print(f"case {case} tree sum", flush=True)
+    lib = L.lib()
+    lib.ldw_conserve2_site_order(0)
+    try:
+        b = L.BinaryFluid((n, n, n), a=a, b=b_, kappa=k, mobility=m, order=order, tau=tau, conserve=2)
+        b.init_spinodal(seed, phi0, amp)
+        b.step(nsteps)
+        phi = b.phi_field()
+        rho, u, _ = b.hydro_fields()
+        b.close()
+    finally:
+        lib.ldw_conserve2_site_order(1)
+    assert np.max(np.abs(phi - g["phi"]) / np.maximum(np.abs(g["phi"]), 1e-3)) <= 1e-12
+    assert np.max(np.abs(rho - g["rho"])) <= 1e-12
+    assert np.max(np.abs(u - g["u"])) <= 1e-12
+
+
+def test_conserve2_tree_sum_holds_the_sum_at_128():
+    """At 128^3 the tree-sum mode keeps sum(phi) at its initial value to
+    round-off over 50 steps, and it is far cheaper than the serial
+    site-order sum (timed with HIP events on the library stream)."""
+    import ctypes
+    lib = L.lib()
+    res = {}
+    for mode in (0, 1):
+        lib.ldw_conserve2_site_order(mode)
+        try:
+            b = L.BinaryFluid((128, 128, 128), tau=1.0, mobility=0.25, conserve=2)
+            b.init_spinodal(8361235, 0.1, 0.1)
+            _, p0 = b.conserved_sums()
+            b.step(2)
+            lib.ldw_stream_mark(0)
+            b.step(20)
+            lib.ldw_stream_mark(1)
+            ms = ctypes.c_double()
+            L._ck(lib.ldw_stream_elapsed(0, 1, ctypes.byref(ms)), "ldw_stream_elapsed")
+            b.step(28)
+            _, p1 = b.conserved_sums()
+            b.close()
+        finally:
+            lib.ldw_conserve2_site_order(1)
+        res[mode] = ms.value / 20
+        assert abs(p1 - p0) <= 1e-10 * abs(p0), (mode, p0, p1)
+    print(f"conserve 2 at 128^3: {res[0]:.3f} ms/step tree sum, {res[1]:.3f} ms/step site order")
+    assert res[0] < 0.25 * res[1]
+
+
 def test_conserve2_vs_oracle_nonuniform_box():
     """Conserve 2 on a box of odd sizes larger than one sum tile (2048
     sites), several steps, against the oracle bit for bit."""

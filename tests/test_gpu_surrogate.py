@@ -1,6 +1,7 @@
 """The surrogate pipeline end to end, scaled down from the full study: data
 from the replica ensemble, training on the GPU with the library's kernels,
-and jumps with lattice Boltzmann correction on held-out replicas.  The
+a short pushforward stage (inputs the jump pipeline itself produces), and
+jumps with lattice Boltzmann correction on held-out replicas.  The
 trained network must beat the zero-change prediction on held-out pairs by
 2x, and its jumps must leave a smaller L(t) error than identity jumps (the
 method without the network)."""
@@ -19,9 +20,10 @@ def test_small_surrogate_study():
     res = run(nrep=24, steps=1500, every=250, sizes=(32,), eval_size=32, iters=800, batch=4,
               lr=5e-4, jumps=2, corr=20, seed=5, t_start=500,
               replica_kw=dict(phi0=(-0.05, 0.05), kappa=(0.035, 0.045), mobility=(0.15, 0.2)),
-              log=lambda m: print(m, flush=True))
+              pf_seconds=6.0, refresh_every=100, log=lambda m: print(m, flush=True))
     assert np.isfinite(res["train_loss_last"])
+    assert res["pushforward_iters"] > 0 and np.isfinite(res["pushforward_loss_last"])
     assert res["heldout_loss"] < 0.5 * res["heldout_zero_change_loss"], res
-    ev = res["evaluation"]
+    ev = res["evaluation_corr20"]
     assert ev["T"] == 2 * 270 and ev["t_start"] == 500
     assert ev["L_err_surrogate_mean"] < ev["L_err_identity_mean"], ev
