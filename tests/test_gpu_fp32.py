@@ -68,3 +68,31 @@ def test_fp32_single_fluid_c1_against_fp64():
     assert rel_rms(res[32][2], res[64][2]) < 1e-5
     assert np.max(np.abs(res[32][1] - res[64][1])) < 1e-6
     assert np.max(np.abs(res[32][0] - res[64][0])) < 1e-7
+
+
+def test_fp32_256_against_fp64():
+    """The 256^3 lattice (the FP32 speed configuration) in both precisions
+    for 40 steps: per-field error and conservation at full size."""
+    n, steps = 256, 40
+    out = {}
+    for prec in (64, 32):
+        print(f"256^3 fp{prec}, {steps} steps", flush=True)
+        b = L.BinaryFluid((n, n, n), tau=1.0, mobility=0.25, precision=prec, fatal=True)
+        b.init_spinodal(8361235, 0.0, 0.1)
+        s0 = b.conserved_sums()
+        b.step(steps)
+        rho, u, _ = b.hydro_fields()
+        out[prec] = (b.phi_field(), rho, u, s0, b.conserved_sums())
+        b.close()
+    phi64, rho64, u64 = out[64][:3]
+    phi32, rho32, u32 = out[32][:3]
+    ep, eu = rel_rms(phi32, phi64), rel_rms(u32, u64)
+    print(f"256^3 {steps} steps: phi rel-rms {ep:.2e}, max {np.max(np.abs(phi32 - phi64)):.2e}; "
+          f"u rel-rms {eu:.2e}", flush=True)
+    assert ep < 1e-6
+    assert np.max(np.abs(phi32 - phi64)) < 1e-5
+    assert eu < 2e-5
+    assert np.max(np.abs(rho32 - rho64)) < 1e-6
+    (r0, p0), (r1, p1) = out[32][3], out[32][4]
+    assert abs(r1 - r0) / r0 < 1e-9
+    assert abs(p1 - p0) / n ** 3 < 1e-8
