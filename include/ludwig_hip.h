@@ -666,6 +666,14 @@ int ldw_nn_shift(float * a, long long n, float v);
  * out[p])/ns, sums in FP64 in a fixed order, all on the stream (no host
  * read-back) */
 int ldw_nn_restore_sums(const float * now, float * out, long long ns, int nplane);
+/* ldw_nn_restore_sums without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops that forbid hipMalloc).  The partial
+ * sums live in a fixed device array of 64 KiB per GPU (nplane <= 64; more
+ * is refused with an error) that every call on that GPU shares: calls that
+ * overlap on one GPU race.  Results equal ldw_nn_restore_sums's bit for
+ * bit. */
+int ldw_nn_restore_sums_noalloc(const float * now, float * out, long long ns, int nplane);
 
 #ifdef __cplusplus
 }

@@ -16,6 +16,9 @@ int fail(const char * file, int line, const char * fmt, ...);
 // select the device and create the library's stream (once)
 int device_init(void);
 hipStream_t stream(void);
+// device address of the fixed scratch of ldw_nn_restore_sums_noalloc on
+// the current device (surrogate_api.cpp); once per device at set-up
+int noalloc_scratch_init(void);
 
 // Collision parameters for fixed shear and bulk viscosities
 // (collision.c:1163-1246); fglobal may be null (zero body force)

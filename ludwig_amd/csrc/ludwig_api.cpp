@@ -114,6 +114,7 @@ static int device_init(void) {
   HIP_CHECK(hipStreamCreateWithFlags(&g_stream2, hipStreamNonBlocking));
   HIP_CHECK(hipEventCreateWithFlags(&g_ev_fork, hipEventDisableTiming));
   HIP_CHECK(hipEventCreateWithFlags(&g_ev_join, hipEventDisableTiming));
+  if (ldw::api::noalloc_scratch_init()) return -1;
   g_ready = 1;
   return 0;
 }

@@ -45,7 +45,37 @@ int check_conv(int batch, int nx, int ny, int nz, int cin, int cout) {
   return 0;
 }
 
+// The scratch of ldw_nn_restore_sums_noalloc: the block partial sums of
+// up to NOALLOC_MAX_PLANES planes in a fixed device array (64 KiB), which
+// each GPU the library runs on holds until the process ends.  Host code
+// never passes the array's name to a kernel: it is a host-side stand-in,
+// not a device address.  The device address is looked up once per device
+// at library set-up (noalloc_scratch_init) and kept in g_noalloc_ptr.
+constexpr int NOALLOC_MAX_PLANES = 64;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[2*NOALLOC_MAX_PLANES*64];
+double * g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+int current_device(int * dev) {
+  LDW_HIP_CHECK(hipGetDevice(dev));
+  if (*dev < 0 || *dev >= NOALLOC_MAX_DEVICES) LDW_FAIL("device %d: at most %d devices", *dev, NOALLOC_MAX_DEVICES);
+  return 0;
+}
+
 }  // namespace
+
+// Called by the library's init (device_init) with its device current.
+int ldw::api::noalloc_scratch_init(void) {
+  int dev;
+  if (current_device(&dev)) return -1;
+  if (g_noalloc_ptr[dev] == nullptr) {
+    void * p = nullptr;
+    LDW_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (p == nullptr) LDW_FAIL("hipGetSymbolAddress: no device address for the restore-sums scratch");
+    g_noalloc_ptr[dev] = (double *) p;
+  }
+  return 0;
+}
 
 #define READY() do { if (ldw::api::device_init()) return -1; } while (0)
 #define KCHECK(call) LDW_HIP_CHECK(call)
@@ -265,6 +295,22 @@ extern "C" int ldw_nn_restore_sums(const float * now, float * out, long long ns,
   void * work = nullptr;
   if (scratch_buffer(g_restore, sizeof(double)*ldw::nn_restore_sums_scratch(nplane), &work)) return -1;
   KCHECK(ldw::nn_restore_sums(now, out, ns, nplane, (double *) work, ldw::api::stream()));
+  return 0;
+}
+
+extern "C" int ldw_nn_restore_sums_noalloc(const float * now, float * out, long long ns, int nplane) {
+  READY();
+  if (now == nullptr || out == nullptr) LDW_FAIL("ldw_nn_restore_sums_noalloc: null buffer");
+  if (ns < 1 || nplane < 1) LDW_FAIL("ldw_nn_restore_sums_noalloc: ns %lld nplane %d", ns, nplane);
+  if (nplane > NOALLOC_MAX_PLANES || ldw::nn_restore_sums_scratch(nplane) > (int) (sizeof(scratch)/sizeof(double)))
+    LDW_FAIL("ldw_nn_restore_sums_noalloc: %d planes, at most %d (fixed scratch of %zu bytes)", nplane,
+	     NOALLOC_MAX_PLANES, sizeof(scratch));
+  int dev;
+  if (current_device(&dev)) return -1;
+  if (g_noalloc_ptr[dev] == nullptr && ldw::api::noalloc_scratch_init()) return -1;
+  // every partial the shift kernel reads is written first by the sums
+  // kernel of this call, so the scratch needs no clearing
+  KCHECK(ldw::nn_restore_sums(now, out, ns, nplane, g_noalloc_ptr[dev], ldw::api::stream()));
   return 0;
 }
 
