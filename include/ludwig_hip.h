@@ -662,6 +662,12 @@ int ldw_nn_scale(float * a, long long n, float f);
 int ldw_nn_sum(const float * a, long long n, double * out);
 /* a[i] += v */
 int ldw_nn_shift(float * a, long long n, float v);
+/* out = an element of the cubic group applied to the state in (4 planes
+ * of n^3 floats: phi, ux, uy, uz, sites z fastest): output axis a is input
+ * axis perm[a], reflected (t -> n-1-t) where flip[a] != 0, and the velocity
+ * turns with it.  The lattice dynamics commute with these maps, so they
+ * make new training pairs from old ones. */
+int ldw_nn_cubic(const float * in, float * out, int n, const int perm[3], const int flip[3]);
 /* For each of nplane planes of ns floats: out[p] += (sum now[p] - sum
  * out[p])/ns, sums in FP64 in a fixed order, all on the stream (no host
  * read-back) */

@@ -208,6 +208,7 @@ def lib():
         "ldw_nn_shift": [vp, ctypes.c_longlong, ctypes.c_float],
         "ldw_nn_restore_sums": [vp, vp, ctypes.c_longlong, ctypes.c_int],
         "ldw_nn_restore_sums_noalloc": [vp, vp, ctypes.c_longlong, ctypes.c_int],
+        "ldw_nn_cubic": [vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     }
     for name, args in sig.items():
         f = getattr(L, name)

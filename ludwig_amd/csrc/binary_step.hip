@@ -27,7 +27,7 @@ namespace {
 
 using namespace tile;
 
-template <typename T, class S, int ORDER, int METHOD, bool GZ, bool CONS>
+template <typename T, class S, int ORDER, int METHOD, bool GZ, bool CONS, bool GMU>
 __global__ void __launch_bounds__(S::NT)
 binary_step_kernel(Geom g, FeParams fe, LbParams prm, int xchunk, int nbz, int nby, int nbx, int remap,
 		   const T * __restrict__ phi, T * __restrict__ phi_out,
@@ -41,7 +41,7 @@ binary_step_kernel(Geom g, FeParams fe, LbParams prm, int xchunk, int nbz, int n
   if (!tile_of_block(nbz, nby, nbx, remap, bzi, byi, bxi)) return;
   const int xb = bxi*xchunk;
   const int xe = min(xb + xchunk, g.nx);
-  step_tile<T, S, ORDER, METHOD, GZ, CONS>(s, g, fe, prm, byi*S::TY, bzi*S::TZ, xb, xe, phi, phi_out,
+  step_tile<T, S, ORDER, METHOD, GZ, CONS, GMU>(s, g, fe, prm, byi*S::TY, bzi*S::TZ, xb, xe, phi, phi_out,
 					   u_prev, u_next, rho, g_in, g_out, fz0, fz1, fz2, csum);
 }
 
@@ -54,7 +54,7 @@ int resident_slots() {
     int per_cu = 0, dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess
-	|| hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, binary_step_kernel<T, S, 1, 0, true, false>,
+	|| hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, binary_step_kernel<T, S, 1, 0, true, false, false>,
 							 S::NT, 0) != hipSuccess || per_cu < 1) {
       (void) hipGetLastError();
       return 256;
@@ -82,7 +82,7 @@ inline int step_xchunk(int nx, int64_t ncol, int slots) {
   return best;
 }
 
-template <typename T, class S, bool CONS>
+template <typename T, class S, bool CONS, bool GMU = false>
 void launch_shape(const Geom & g, const FeParams & fe, const LbParams & prm, const T * phi,
 		  T * phi_out, const T * u_prev, T * u_next, T * rho, const T * g_in,
 		  T * g_out, const double fzero[3], T * csum, hipStream_t st) {
@@ -102,7 +102,7 @@ void launch_shape(const Geom & g, const FeParams & fe, const LbParams & prm, con
   dim3 grid(((ntile + NXCD - 1)/NXCD)*NXCD), block(S::TZ, S::TY, 1);
   const T f0 = T(fzero[0]), f1 = T(fzero[1]), f2 = T(fzero[2]);
   const bool gz = (prm.relax == 0);
-#define L_(O, M, G) hipLaunchKernelGGL((binary_step_kernel<T, S, O, M, G, CONS>), grid, block, 0, st, \
+#define L_(O, M, G) hipLaunchKernelGGL((binary_step_kernel<T, S, O, M, G, CONS, GMU>), grid, block, 0, st, \
 				       g, fe, prm, xchunk, bz, by, bx, tune_remap(), phi, phi_out, u_prev, u_next, rho, \
 				       g_in, g_out, f0, f1, f2, csum)
 #define L_G(O, M) do { if (gz) L_(O, M, true); else L_(O, M, false); } while (0)
@@ -123,13 +123,22 @@ void launch_shape(const Geom & g, const FeParams & fe, const LbParams & prm, con
 // 64 x 8 / 128 x 4 tiles, which halve the share of pulled rows that cross a
 // tile edge in y or z; chosen by precision and lattice size
 // (tune_step_tile).  The tile changes only which block computes a site,
-// not the arithmetic.  With a compensation array (conserve 1) or advection
-// order 4 the 64 x 8 tile is used, so those kernels are built for one tile
-// only.
+// not the arithmetic.  With a compensation array (conserve 1), advection
+// order 4 or an external chemical potential gradient the 64 x 8 tile is
+// used, so those kernels are built for one tile only.  The kernels without
+// the external gradient leave its code out, which keeps the FP32 kernels
+// at 3 waves per SIMD.
 template <typename T>
 void launch(const Geom & g, const FeParams & fe, const LbParams & prm, const T * phi,
 	    T * phi_out, const T * u_prev, T * u_next, T * rho, const T * g_in,
 	    T * g_out, const double fzero[3], T * csum, hipStream_t st) {
+  if (fe.gmu_on) {
+    if (csum) launch_shape<T, Shape<64, 8>, true, true>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in,
+							 g_out, fzero, csum, st);
+    else launch_shape<T, Shape<64, 8>, false, true>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in,
+						    g_out, fzero, nullptr, st);
+    return;
+  }
   if (csum) {
     launch_shape<T, Shape<64, 8>, true>(g, fe, prm, phi, phi_out, u_prev, u_next, rho, g_in, g_out, fzero,
 					csum, st);

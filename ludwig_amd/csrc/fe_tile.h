@@ -201,7 +201,10 @@ __device__ __forceinline__ T adv1(T uf, T phi_self, T phi_other, bool west) {
 // second addition under phi grad mu with an external gradient, and the
 // updated phi; with csum_site (the site's compensation) the update is the
 // compensated one.  Advection orders 1-4 (order 4 reads phi to x +- 2).
-template <typename T, class S, int ORDER, int METHOD, bool HAVE_U>
+// GMU: the external gradient terms are compiled in (they act where
+// fe.gmu_on); without them the kernels that never see one keep their
+// register count.
+template <typename T, class S, int ORDER, int METHOD, bool HAVE_U, bool GMU = true>
 __device__ __forceinline__ void force_ch_site(const Smem<T, S> & s, const FeParams & fe,
 					      int x, int ty, int tz, const T (&uc)[9], const T (&fz)[3],
 					      T (&F)[3], T & pnew, T * csum_site = nullptr) {
@@ -234,7 +237,7 @@ __device__ __forceinline__ void force_ch_site(const Smem<T, S> & s, const FePara
 	F[0] = fz[0] + (T(0.0) + -phi0*T(0.5)*(qp[4][c] - qm[4][c]));
 	F[1] = fz[1] + (T(0.0) + -phi0*T(0.5)*(q0[4][c + QZ] - q0[4][c - QZ]));
 	F[2] = fz[2] + (T(0.0) + -phi0*T(0.5)*(q0[4][c + 1] - q0[4][c - 1]));
-	if (fe.gmu_on) {
+	if (GMU && fe.gmu_on) {
 	  F[0] += -phi0*T(fe.gmu[0]);
 	  F[1] += -phi0*T(fe.gmu[1]);
 	  F[2] += -phi0*T(fe.gmu[2]);
@@ -317,7 +320,7 @@ __device__ __forceinline__ void force_ch_site(const Smem<T, S> & s, const FePara
 	fzp -= mob*(q0[4][c + 1] - m0);
 	fzm -= mob*(m0 - q0[4][c - 1]);
   }
-  if (fe.gmu_on) adv::mu_ext(mob, fe.gmu, fw, fe_, fyp, fym, fzp, fzm);
+  if (GMU && fe.gmu_on) adv::mu_ext(mob, fe.gmu, fw, fe_, fyp, fym, fzp, fzm);
   pnew = p0;
   if (csum_site != nullptr) {
     T cs = *csum_site;

@@ -235,6 +235,10 @@ hipError_t nn_scale(float * a, int64_t n, float f, hipStream_t s);
 hipError_t nn_sumsq(const float * a, int64_t n, double * work, hipStream_t s);
 hipError_t nn_sum(const float * a, int64_t n, double * work, hipStream_t s);
 hipError_t nn_shift(float * a, int64_t n, float v, hipStream_t s);
+// out = the state `in` (4 planes of n^3 floats: phi, ux, uy, uz) under the
+// axis permutation perm and the reflections flip (cubic group)
+hipError_t nn_cubic(const float * in, float * out, int n, const int perm[3], const int flip[3],
+		    hipStream_t s);
 // out plane p += (sum now plane p - sum out plane p)/ns for nplane planes of
 // ns floats; work holds nn_restore_sums_scratch(nplane) doubles
 int nn_restore_sums_scratch(int nplane);

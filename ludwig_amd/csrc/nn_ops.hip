@@ -288,6 +288,30 @@ __global__ void shift_kernel(float * __restrict__ a, int64_t n, float s) {
   GRID_STRIDE(i, n) a[i] += s;
 }
 
+// An element of the cubic group on a (phi, ux, uy, uz) state of an n^3
+// periodic lattice (4 SoA planes): output axis a takes input axis perm[a],
+// reflected about the site n-1-t where flip[a] (a map of sites to sites);
+// the velocity is a vector, u'_a = (flip[a] ? -1 : 1) u_perm[a].  One
+// thread per output site, gathering.
+struct CubicOp { int perm[3], flip[3]; };
+
+__global__ void cubic_kernel(const float * __restrict__ in, float * __restrict__ out, int n, CubicOp op) {
+  const int64_t ns = (int64_t) n*n*n;
+  GRID_STRIDE(s, ns) {
+    const int o[3] = {(int) (s/((int64_t) n*n)), (int) ((s/n) % n), (int) (s % n)};
+    int r[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) r[op.perm[a]] = op.flip[a] ? n - 1 - o[a] : o[a];
+    const int64_t src = ((int64_t) r[0]*n + r[1])*n + r[2];
+    out[s] = in[src];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const float v = in[(int64_t) (1 + op.perm[a])*ns + src];
+      out[(int64_t) (1 + a)*ns + s] = op.flip[a] ? -v : v;
+    }
+  }
+}
+
 // per-block sums (SQUARE: of squares) in FP64 (fixed reduction order)
 template <bool SQUARE>
 __global__ void sumsq_kernel(const float * __restrict__ a, int64_t n, double * __restrict__ part) {
@@ -488,6 +512,14 @@ hipError_t nn_restore_sums(const float * now, float * out, int64_t ns, int nplan
 		     now, out, ns, nplane, work);
   hipLaunchKernelGGL(plane_shift_kernel, dim3(256, nplane), dim3(256), 0, s,
 		     out, ns, nplane, RESTORE_BLOCKS, work);
+  return hipGetLastError();
+}
+
+hipError_t nn_cubic(const float * in, float * out, int n, const int perm[3], const int flip[3],
+		    hipStream_t s) {
+  CubicOp op;
+  for (int a = 0; a < 3; a++) { op.perm[a] = perm[a]; op.flip[a] = flip[a] ? 1 : 0; }
+  LAUNCH(cubic_kernel, (int64_t) n*n*n, in, out, n, op);
   return hipGetLastError();
 }
 

@@ -43,9 +43,10 @@ __device__ __forceinline__ void load_f(T (&f)[NVEL], const T * __restrict__ g_in
 // g_in -> g_out with pull indexing, rho written.  fz: the uniform force
 // recorded by hydro_f_zero, added to the phi force.  CONS: the compensated
 // update of cahn_hilliard_options_conserve 1, with the per-site
-// compensation csum read and written.  Ends with a block barrier, so the
-// caller may reuse the LDS tiles at once.
-template <typename T, class S, int ORDER, int METHOD, bool GZ, bool CONS = false>
+// compensation csum read and written.  GMU: the external chemical
+// potential gradient is compiled in (fe.gmu_on then decides).  Ends with a
+// block barrier, so the caller may reuse the LDS tiles at once.
+template <typename T, class S, int ORDER, int METHOD, bool GZ, bool CONS = false, bool GMU = false>
 __device__ __forceinline__ void step_tile(Smem<T, S> & s, const Geom & g, const FeParams & fe,
 					  const LbParams & prm, int y0, int z0, int xb, int xe,
 					  const T * __restrict__ phi, T * __restrict__ phi_out,
@@ -81,7 +82,7 @@ __device__ __forceinline__ void step_tile(Smem<T, S> & s, const Geom & g, const 
       const int64_t site = ((int64_t) x*g.ny + y)*g.nz + z;
       T F[3], pnew;
       const T fzv[3] = {fz0, fz1, fz2};
-      force_ch_site<T, S, ORDER, METHOD, true>(s, fe, x, ty, tz, uc, fzv, F, pnew,
+      force_ch_site<T, S, ORDER, METHOD, true, GMU>(s, fe, x, ty, tz, uc, fzv, F, pnew,
 					       CONS ? csum + site : nullptr);
       phi_out[site] = pnew;
       // site force: force_global + (hydro_f_zero value + phi force)

@@ -314,6 +314,21 @@ extern "C" int ldw_nn_restore_sums_noalloc(const float * now, float * out, long 
   return 0;
 }
 
+extern "C" int ldw_nn_cubic(const float * in, float * out, int n, const int perm[3], const int flip[3]) {
+  READY();
+  if (in == nullptr || out == nullptr || in == out) LDW_FAIL("ldw_nn_cubic: null or aliased buffers");
+  if (n < 1) LDW_FAIL("ldw_nn_cubic: lattice %d", n);
+  if (perm == nullptr || flip == nullptr) LDW_FAIL("ldw_nn_cubic: null permutation or reflections");
+  int seen = 0;
+  for (int a = 0; a < 3; a++) {
+    if (perm[a] < 0 || perm[a] > 2) LDW_FAIL("ldw_nn_cubic: perm[%d] = %d", a, perm[a]);
+    seen |= 1 << perm[a];
+  }
+  if (seen != 7) LDW_FAIL("ldw_nn_cubic: (%d, %d, %d) is not a permutation", perm[0], perm[1], perm[2]);
+  KCHECK(ldw::nn_cubic(in, out, n, perm, flip, ldw::api::stream()));
+  return 0;
+}
+
 extern "C" int ldw_nn_shift(float * a, long long n, float v) {
   READY();
   KCHECK(ldw::nn_shift(a, n, v, ldw::api::stream()));

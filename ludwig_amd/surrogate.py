@@ -345,6 +345,20 @@ def decode(now, y, nsite, normaliser, out, offset_sites=0):
                             normaliser.dphi_scale, normaliser.du_scale, PHI_MAX, U_MAX), "ldw_nn_decode")
 
 
+# The 48 elements of the cubic group (axis permutations x reflections).
+# The lattice dynamics commute with each of them (the D3Q19 velocity set,
+# the centred stencils and the upwind fluxes are all invariant), so a
+# training pair mapped by one is again a pair of the dynamics.
+CUBIC_GROUP = [(p, (f >> 2 & 1, f >> 1 & 1, f & 1))
+               for p in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)) for f in range(8)]
+
+
+def cubic(src, dst, n, perm, flip):
+    """dst = the state src (4 planes of n^3 floats) under one cubic map."""
+    _ck(lib().ldw_nn_cubic(_v(src), _v(dst), n, (ctypes.c_int * 3)(*perm), (ctypes.c_int * 3)(*flip)),
+        "ldw_nn_cubic")
+
+
 def loss_and_grad(y, tgt, npos, dyb, dyf, weights=(1.0, 1.0, 1.0, 1.0)):
     loss = ctypes.c_double()
     w = (ctypes.c_float * 4)(*weights)
@@ -419,13 +433,18 @@ class Dataset:
 class Trainer:
     """Adam on the GPU over batches of same-size pairs (snapshot k -> k + lag)."""
 
-    def __init__(self, net, normaliser, batch=4, weights=(1.0, 0.1, 0.1, 0.1), clip=1.0):
+    def __init__(self, net, normaliser, batch=4, weights=(1.0, 0.1, 0.1, 0.1), clip=1.0, augment=False,
+                 seed=0):
         """weights: loss weights of (phi, ux, uy, uz) changes -- the velocity
         over a whole jump is nearly decorrelated, so phi leads; clip: global
-        gradient-norm bound."""
+        gradient-norm bound; augment: map every training pair of a cubic
+        lattice by a random element of the cubic group (CUBIC_GROUP)."""
         self.net, self.norm, self.batch = net, normaliser, batch
         self.weights, self.clip = tuple(weights), clip
+        self.augment = augment
+        self._rng = np.random.default_rng(seed + 7919)
         self._io = {}
+        self._aug = {}
 
     def _bufs(self, shape, batch):
         key = (batch,) + tuple(shape)
@@ -453,6 +472,19 @@ class Trainer:
         spec) device arrays of one lattice shape."""
         ns = int(np.prod(shape))
         io = self._bufs(shape, len(items))
+        if train and self.augment and shape[0] == shape[1] == shape[2]:
+            key = (len(items),) + tuple(shape)
+            if key not in self._aug:
+                self._aug[key] = [(DeviceArray(4 * ns, np.float32), DeviceArray(4 * ns, np.float32))
+                                  for _ in range(len(items))]
+            mapped = []
+            for j, (now, nxt, spec) in enumerate(items):
+                perm, flip = CUBIC_GROUP[self._rng.integers(len(CUBIC_GROUP))]
+                a, b = self._aug[key][j]
+                cubic(now, a, shape[0], perm, flip)
+                cubic(nxt, b, shape[0], perm, flip)
+                mapped.append((a, b, spec))
+            items = mapped
         for j, (now, nxt, spec) in enumerate(items):
             encode(now, ns, spec, self.norm, io["x"], j * ns)
             target(now, nxt, ns, self.norm, io["t"], j * ns)

@@ -147,28 +147,48 @@ def heldout_specs(nrep=256, seed=2026, size=None):
     return [specs[i] for i in val_idx if size is None or specs[i].nx == size]
 
 
+FRESH_SEED = 424242
+
+
+def fresh_specs(nrep, size=64, train_nrep=256, train_seed=2026):
+    """nrep replicas of one lattice size drawn from the training ensemble's
+    parameter distribution with a generator of their own: seeds and
+    parameters that no training run has seen (checked against the
+    training ensemble's seeds)."""
+    specs = draw_replicas(nrep, seed=FRESH_SEED, sizes=(size,))
+    seen = {s.seed for s in draw_replicas(train_nrep, seed=train_seed)}
+    if any(s.seed in seen for s in specs):
+        raise ValueError("a fresh replica's seed is one of the training ensemble's")
+    return specs
+
+
 def gain(weights=WEIGHTS, size=64, corrs=(20, 30), every=250, jumps=8, t_start=500,
-         conserve=True, log=None):
+         conserve=True, log=None, fresh=0):
     """Surrogate + LB against plain LB on the held-out replicas (all sizes
-    with size=None): for each number of correction steps, the time-to-T
-    speedup and the L(T) error per replica (mean, max) and of the ensemble
-    mean, with T = jumps * (every + corr)."""
+    with size=None), plus `fresh` never-seen replicas of that size: for each
+    number of correction steps, the time-to-T speedup and the L(T) error
+    per replica (mean, max, signed mean) and of the ensemble mean, with
+    T = jumps * (every + corr)."""
     net = UNet(seed=0)
     norm = net.load(weights)
     if norm is None:
         raise ValueError(f"{weights}: no normaliser stored with the weights")
     specs = heldout_specs(size=size)
-    out = {"weights": os.path.basename(weights), "replicas": len(specs),
+    if fresh:
+        specs = specs + fresh_specs(fresh, size=size or 64)
+    out = {"weights": os.path.basename(weights), "replicas": len(specs), "fresh": fresh,
            "sizes": sorted({s.nx for s in specs}), "jumps": jumps, "every": every, "t_start": t_start,
            "sums_restored": conserve}
     for c in corrs:
         ev = evaluate(net, norm, specs, every, jumps, c, t_start=t_start,
                       log=log or (lambda m: None), variants={"surrogate": ("pred", conserve)},
                       control=False, identity=False)
+        signed = [(r["L_surrogate"] - r["L_plain"]) / r["L_plain"] for r in ev["per_replica"]]
         out[f"corr{c}"] = {"T": ev["T"], "speedup": ev["speedup"], "plain_s": ev["plain_s"],
                            "surrogate_s": ev["surrogate_s"],
                            "L_err_per_replica_mean": ev["L_err_surrogate_mean"],
                            "L_err_per_replica_max": ev["L_err_surrogate_max"],
+                           "L_err_per_replica_signed_mean": float(np.mean(signed)),
                            "L_err_ensemble_mean": ev["meanL_err_surrogate"],
                            "phi_rms_err_mean": ev["phi_rms_err_surrogate_mean"]}
     return out
@@ -177,7 +197,7 @@ def gain(weights=WEIGHTS, size=64, corrs=(20, 30), every=250, jumps=8, t_start=5
 def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=0, iters=2000,
         batch=4, lr=5e-4, jumps=8, corr=30, seed=2026, max_eval=None, train_seconds=None,
         t_start=500, replica_kw=None, init_from=None, save_to=None, log=print, pf_seconds=0.0,
-        pf_lr=None, p_pf=0.5, refresh_every=400, eval_corrs=None):
+        pf_lr=None, p_pf=0.5, refresh_every=400, eval_corrs=None, augment=False):
     """Data, training and the held-out evaluation.  pf_seconds > 0 adds a
     second training stage on pushforward pairs (Trainer.fit_pushforward):
     the inputs the jump pipeline with `corr` correction steps reaches after
@@ -200,7 +220,7 @@ def run(nrep=256, steps=4000, every=250, sizes=(48, 64, 80, 96), eval_size=0, it
         if stored is not None:
             norm = stored
         log(f"weights from {init_from} (after {net.step_count} steps)")
-    tr = Trainer(net, norm, batch=batch)
+    tr = Trainer(net, norm, batch=batch, augment=augment, seed=seed)
     tpairs = [(r, k) for (r, k) in data.pairs(kmin=kmin) if r in set(train_idx)]
     vpairs = [(r, k) for (r, k) in data.pairs(kmin=kmin) if r in set(val_idx)]
     t0 = time.perf_counter()
@@ -278,12 +298,14 @@ def main():
     ap.add_argument("--refresh-every", type=int, default=400)
     ap.add_argument("--eval-corr", type=int, nargs="*", default=None,
                     help="correction steps of the evaluations (default: --corr)")
+    ap.add_argument("--augment", action="store_true",
+                    help="map training pairs by random elements of the cubic group")
     a = ap.parse_args()
     res = run(a.replicas, a.steps, a.every, eval_size=a.eval_size, iters=a.iters, batch=a.batch, lr=a.lr,
               jumps=a.jumps, corr=a.corr, seed=a.seed, max_eval=a.max_eval or None,
               train_seconds=a.train_seconds, t_start=a.t_start, init_from=a.init_from,
               save_to=a.save_to, pf_seconds=a.pf_seconds, pf_lr=a.pf_lr, p_pf=a.p_pf,
-              refresh_every=a.refresh_every, eval_corrs=a.eval_corr,
+              refresh_every=a.refresh_every, eval_corrs=a.eval_corr, augment=a.augment,
               log=lambda m: print(m, flush=True))
     print(json.dumps(res))
 
