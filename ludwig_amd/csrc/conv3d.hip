@@ -79,7 +79,7 @@ conv3_fwd_kernel(const uint16_t * __restrict__ x, const uint16_t * __restrict__ 
 		 int64_t npos, int nx, int ny, int nz, int cin, int cout, int accumulate) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 15, kq = lane >> 4;
-  const int64_t m0 = (int64_t) blockIdx.x*128 + wave*32;
+  const int64_t m0 = (int64_t) blockIdx.x*(blockDim.x/2) + wave*32;   // 32 positions per wave
   const int n0 = blockIdx.y*16*NT16;
 
   Pos q[2];
@@ -261,6 +261,205 @@ conv3_fwd_lds_kernel(const uint16_t * __restrict__ x, const uint16_t * __restric
   }
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined variant of the LDS-tiled forward pass (the default up to 64
+// output channels per block).  The tiling and the MFMA fragments are those of
+// conv3_fwd_lds_kernel; what changes is where the memory latency goes.
+// rocprof counters of the tiled kernel (MFMA busy 25 % of cycles, waves
+// waiting half their cycles, LDS active a quarter of the time, no bank
+// conflicts) show the halo fill of each input-channel chunk as the binding
+// cost: every block stops for it with nothing to compute.  Here
+//   - the halo of the NEXT input-channel chunk is loaded into registers
+//     while the current chunk is computed, and written to LDS when the
+//     current chunk is done;
+//   - the weights run one stage ahead across chunk boundaries;
+//   - a chunk of 16 channels (cin = 16 mod 32) runs on the K = 16 MFMA
+//     (v_mfma_f32_16x16x16_bf16) instead of a K = 32 one half of zeros.
+// The products and the FP32 accumulation bound are those of the tiled
+// kernel; a 16-channel chunk's MFMA sums 16 products where the tiled
+// kernel's sums them with 16 zero ones, so the last bits may differ.
+
+constexpr int HPT = (LH_N*4 + 255)/256;       // halo uint4 per thread
+
+// One stage (taps dx, dy, dz = 0..2) of the pipelined kernel on the LDS
+// tiles: K = 32 MFMAs on 8-channel fragments, or for a 16-channel chunk
+// K = 16 MFMAs, lane (r, q) holding channels 4q .. 4q+3 (the (q & 1) half
+// of 16-byte part q >> 1).
+template <int NT, bool K16>
+__device__ __forceinline__ void conv_stage(const uint4 * sa, const uint4 (*sb)[16*NT][LK/8], f4 (&acc)[4][NT],
+					   int wave, int r, int q, int dx, int dy, int swz) {
+#pragma unroll
+  for (int dz = 0; dz < 3; dz++) {
+    if constexpr (!K16) {
+      s8 bv[NT];
+#pragma unroll
+      for (int n = 0; n < NT; n++) {
+	const int row = dz*16*NT + 16*n + r;
+	bv[n] = *reinterpret_cast<const s8 *>(&sb[dz][16*n + r][lds_part(row, q, swz)]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+	const int h = ((wave + dx)*LH_Y + (j + dy))*LH_Z + (r + dz);
+	const s8 a = *reinterpret_cast<const s8 *>(&sa[h*4 + lds_part(h, q, swz)]);
+#pragma unroll
+	for (int n = 0; n < NT; n++)
+	  acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a),
+							       __builtin_bit_cast(b8, bv[n]), acc[j][n], 0, 0, 0);
+      }
+    }
+    else {
+      s4 bv[NT];
+#pragma unroll
+      for (int n = 0; n < NT; n++) {
+	const int row = dz*16*NT + 16*n + r;
+	bv[n] = *reinterpret_cast<const s4 *>(reinterpret_cast<const char *>(&sb[dz][16*n + r][lds_part(row, q >> 1, swz)])
+					      + 8*(q & 1));
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+	const int h = ((wave + dx)*LH_Y + (j + dy))*LH_Z + (r + dz);
+	const s4 a = *reinterpret_cast<const s4 *>(reinterpret_cast<const char *>(&sa[h*4 + lds_part(h, q >> 1, swz)])
+						   + 8*(q & 1));
+#pragma unroll
+	for (int n = 0; n < NT; n++)
+	  acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv[n], acc[j][n], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// With 16 or 32 output channels per block the kernel is held to 168
+// registers, i.e. 3 blocks (12 waves) per CU as the LDS allows; 48 and 64
+// channels do not fit 168 without spilling and run at 2 blocks per CU;
+// wider blocks take conv3_fwd_lds_kernel.
+template <int NT>
+__global__ void __launch_bounds__(256, (NT <= 2 ? 3 : 2))
+conv3_fwd_pipe_kernel(const uint16_t * __restrict__ x, const uint16_t * __restrict__ w,
+		      const float * __restrict__ bias, float * __restrict__ y, uint16_t * __restrict__ act,
+		      int nx, int ny, int nz, int cin, int cout, int accumulate,
+		      int ntx, int nty, int ntz, int swz) {
+  __shared__ uint4 sa[LH_N*(LK/8)];
+  __shared__ uint4 sb[3][16*NT][LK/8];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, q = lane >> 4;
+  int t = blockIdx.x;
+  const int tz = t % ntz; t /= ntz;
+  const int ty = t % nty; t /= nty;
+  const int tx = t % ntx;
+  const int b = t / ntx;
+  const int x0 = tx*LT_X, y0 = ty*LT_Y, z0 = tz*LT_Z;
+  const int n0 = blockIdx.y*16*NT;
+  constexpr int NB = 3*16*NT*(LK/8);
+  constexpr int NBT = (NB + 255)/256;
+  const uint16_t * xb = x + (int64_t) b*nx*ny*nz*cin;
+
+  // halo of chunk c0 into registers
+  uint4 hpre[HPT];
+  auto fetch_halo = [&](int c0) {
+    const int kc = (cin - c0 < LK) ? cin - c0 : LK;
+#pragma unroll
+    for (int k = 0; k < HPT; k++) {
+      const int i = threadIdx.x + 256*k;
+      const int h = i >> 2, part = i & 3;
+      uint4 v = {0u, 0u, 0u, 0u};
+      if (i < LH_N*4 && 8*part < kc) {
+	const int hz = h % LH_Z, hy = (h / LH_Z) % LH_Y, hx = h / (LH_Z*LH_Y);
+	const int gx = wrapi(x0 + hx - 1, nx), gy = wrapi(y0 + hy - 1, ny), gz = wrapi(z0 + hz - 1, nz);
+	v = *reinterpret_cast<const uint4 *>(xb + (((int64_t) gx*ny + gy)*nz + gz)*cin + c0 + 8*part);
+      }
+      hpre[k] = v;
+    }
+  };
+  auto store_halo = [&]() {
+#pragma unroll
+    for (int k = 0; k < HPT; k++) {
+      const int i = threadIdx.x + 256*k;
+      if (i < LH_N*4) sa[4*(i >> 2) + lds_part(i >> 2, i & 3, swz)] = hpre[k];
+    }
+  };
+  // weights of taps 3st .. 3st+2 of chunk c0: [tap][n][part]
+  uint4 wpre[NBT];
+  auto fetch_w = [&](int c0, int st) {
+    const int kc = (cin - c0 < LK) ? cin - c0 : LK;
+#pragma unroll
+    for (int k = 0; k < NBT; k++) {
+      const int i = threadIdx.x + 256*k;
+      uint4 v = {0u, 0u, 0u, 0u};
+      if (i < NB) {
+	const int part = i & 3, n = (i >> 2) % (16*NT), tp = (i >> 2)/(16*NT);
+	if (8*part < kc)
+	  v = *reinterpret_cast<const uint4 *>(w + ((int64_t) (3*st + tp)*cout + n0 + n)*cin + c0 + 8*part);
+      }
+      wpre[k] = v;
+    }
+  };
+  auto store_w = [&]() {
+#pragma unroll
+    for (int k = 0; k < NBT; k++) {
+      const int i = threadIdx.x + 256*k;
+      if (i < NB) (&sb[0][0][0])[4*(i >> 2) + lds_part(i >> 2, i & 3, swz)] = wpre[k];
+    }
+  };
+
+  f4 acc[4][NT];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int n = 0; n < NT; n++) acc[j][n] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  // full 32-channel chunks; the halo of the next chunk (full or not) is
+  // loaded while this one is computed, the weights one stage ahead
+  const int cfull = cin - cin % LK;
+  fetch_halo(0);
+  fetch_w(0, 0);
+#pragma unroll 1
+  for (int c0 = 0; c0 < cfull; c0 += LK) {
+    // every wave is done with the previous chunk's halo (barrier at the
+    // end of its last stage)
+    store_halo();
+    if (c0 + LK < cin) fetch_halo(c0 + LK);
+#pragma unroll 1
+    for (int st = 0; st < 9; st++) {
+      store_w();
+      __syncthreads();                        // halo (first stage) and weights in place
+      if (st + 1 < 9) fetch_w(c0, st + 1);
+      else if (c0 + LK < cin) fetch_w(c0 + LK, 0);
+      conv_stage<NT, false>(sa, sb, acc, wave, r, q, st/3, st%3, swz);
+      __syncthreads();                        // everyone done with this stage's weights
+    }
+  }
+  // a last chunk of 16 channels
+  if (cfull < cin) {
+    store_halo();
+#pragma unroll 1
+    for (int st = 0; st < 9; st++) {
+      store_w();
+      __syncthreads();
+      if (st + 1 < 9) fetch_w(cfull, st + 1);
+      conv_stage<NT, true>(sa, sb, acc, wave, r, q, st/3, st%3, swz);
+      __syncthreads();
+    }
+  }
+  const int gx = x0 + wave;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int gy = y0 + j;
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+      const int ch = n0 + 16*n + r;
+      const float bv = bias ? bias[ch] : 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+	const int gz = z0 + 4*q + i;
+	if (gx < nx && gy < ny && gz < nz) {
+	  const int64_t p = (((int64_t) b*nx + gx)*ny + gy)*nz + gz;
+	  store_out(y, act, p*cout + ch, acc[j][n][i] + bv, accumulate);
+	}
+      }
+    }
+  }
+}
+
 constexpr int WG_P = 64;          // positions per LDS slice
 constexpr int WG_CMAX = 160;      // channel capacity of the LDS rows (host checks)
 
@@ -385,12 +584,14 @@ int conv_swz() {
   return v;
 }
 
-// LDW_CONV_LDS=0 selects the fragment-direct kernel everywhere (for A/B timing)
+// LDW_CONV_LDS=0 selects the fragment-direct kernel everywhere, =1 the
+// tiled kernel without the pipeline (A/B timing); by default the
+// pipelined persistent kernel
 int conv_lds_enabled() {
   static int on = -1;
   if (on < 0) {
     const char * e = getenv("LDW_CONV_LDS");
-    on = (e && atoi(e) == 0) ? 0 : 1;
+    on = e ? atoi(e) : 2;
   }
   return on;
 }
@@ -412,6 +613,18 @@ hipError_t launch_conv3_fwd(const uint16_t * x, const uint16_t * w, const float 
     for (int c = LNT_MAX; c >= 1; c--) if (nb16 % c == 0) { nt = c; break; }
     const int ntx = (nx + LT_X - 1)/LT_X, nty = (ny + LT_Y - 1)/LT_Y, ntz = (nz + LT_Z - 1)/LT_Z;
     dim3 grid((unsigned) ((int64_t) batch*ntx*nty*ntz), nb16/nt), block(256);
+    if (conv_lds_enabled() == 2 && nt <= 4) {
+#define PIPE_(NT) hipLaunchKernelGGL(conv3_fwd_pipe_kernel<NT>, grid, block, 0, s, x, w, bias, y, act, nx, ny, nz, \
+				     cin, cout, accumulate, ntx, nty, ntz, conv_swz())
+      switch (nt) {
+      case 4: PIPE_(4); break;
+      case 3: PIPE_(3); break;
+      case 2: PIPE_(2); break;
+      default: PIPE_(1); break;
+      }
+#undef PIPE_
+      return hipGetLastError();
+    }
 #define LDS_(NT) hipLaunchKernelGGL(conv3_fwd_lds_kernel<NT>, grid, block, 0, s, x, w, bias, y, act, nx, ny, nz, \
 				    cin, cout, accumulate, ntx, nty, ntz, conv_swz())
     switch (nt) {
@@ -425,8 +638,20 @@ hipError_t launch_conv3_fwd(const uint16_t * x, const uint16_t * w, const float 
 #undef LDS_
     return hipGetLastError();
   }
-  const int nt = (nb16 % 4 == 0) ? 4 : (nb16 % 3 == 0) ? 3 : (nb16 % 2 == 0) ? 2 : 1;
-  dim3 grid((unsigned) ((npos + 127)/128), nb16/nt), block(256);
+  // small lattices (the network's coarsest level): fewer output blocks
+  // per wave and fewer waves per block, so that the grid still spreads
+  // over the CUs (at 8 x 8^3 positions, 64 -> 96 channels: 768 one-wave
+  // blocks instead of 64 four-wave ones)
+  int nt = (nb16 % 4 == 0) ? 4 : (nb16 % 3 == 0) ? 3 : (nb16 % 2 == 0) ? 2 : 1;
+  int wpb = 4;
+  const int64_t want = 1024;
+  while (wpb > 1 && ((npos + 32*wpb - 1)/(32*wpb))*(nb16/nt) < want) wpb /= 2;
+  while (nt > 1 && ((npos + 32*wpb - 1)/(32*wpb))*(nb16/nt) < want) {
+    int c = nt - 1;
+    while (c > 1 && nb16 % c) c--;
+    nt = c;
+  }
+  dim3 grid((unsigned) ((npos + 32*wpb - 1)/(32*wpb)), nb16/nt), block(64*wpb);
   switch (nt) {
   case 4: hipLaunchKernelGGL(conv3_fwd_kernel<4>, grid, block, 0, s, x, w, bias, y, act, npos, nx, ny, nz, cin, cout, accumulate); break;
   case 3: hipLaunchKernelGGL(conv3_fwd_kernel<3>, grid, block, 0, s, x, w, bias, y, act, npos, nx, ny, nz, cin, cout, accumulate); break;

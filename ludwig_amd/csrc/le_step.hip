@@ -54,7 +54,7 @@ __device__ __forceinline__ LeSlab slab_of(const Geom & g, const LeGeom & le, con
 }
 
 // launch 1: the slab interiors, tiles z-fastest, then y, then (slab, chunk)
-template <class S, int ORDER, bool GZ>
+template <class S, int ORDER, bool GZ, bool GMU>
 __global__ void __launch_bounds__(S::NT)
 le_step_interior_kernel(Geom g, FeParams fe, LbParams prm, LeGeom le, LeStepArgs a, int xchunk,
 			int nchunk, int nbz, int nby, const double * __restrict__ phi,
@@ -73,12 +73,12 @@ le_step_interior_kernel(Geom g, FeParams fe, LbParams prm, LeGeom le, LeStepArgs
   const int xb = L.slab_lo + 1 + k*xchunk;
   const int xe = min(xb + xchunk, L.slab_hi);
   if (xb >= xe) return;
-  le_step_tile<S, ORDER, GZ>(s, g, fe, prm, L, byi*S::TY, bzi*S::TZ, xb, xe, phi, phi_out, u_prev,
+  le_step_tile<S, ORDER, GZ, GMU>(s, g, fe, prm, L, byi*S::TY, bzi*S::TZ, xb, xe, phi, phi_out, u_prev,
 			     u_next, rho, g_in, g_out, force, csum, 0.0, 0.0, 0.0);
 }
 
 // launch 2: the edge slices, (slab, edge) after the tiles
-template <class S, int ORDER, bool GZ>
+template <class S, int ORDER, bool GZ, bool GMU>
 __global__ void __launch_bounds__(S::NT)
 le_step_edges_kernel(Geom g, FeParams fe, LbParams prm, LeGeom le, LeStepArgs a, int nbz, int nby,
 		     const double * __restrict__ phi, double * __restrict__ phi_out,
@@ -93,7 +93,7 @@ le_step_edges_kernel(Geom g, FeParams fe, LbParams prm, LeGeom le, LeStepArgs a,
   if (e >= 2*le.nplane) return;
   const LeSlab L = slab_of(g, le, a, e/2, true);
   const int x = (e % 2) ? L.slab_hi : L.slab_lo;
-  le_step_tile<S, ORDER, GZ>(s, g, fe, prm, L, byi*S::TY, bzi*S::TZ, x, x + 1, phi, phi_out, u_prev,
+  le_step_tile<S, ORDER, GZ, GMU>(s, g, fe, prm, L, byi*S::TY, bzi*S::TZ, x, x + 1, phi, phi_out, u_prev,
 			     u_next, rho, g_in, g_out, force, csum, 0.0, 0.0, 0.0);
 }
 
@@ -104,7 +104,7 @@ int resident_slots_le() {
     int per_cu = 0, dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess
-	|| hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, le_step_interior_kernel<S, 1, true>,
+	|| hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, le_step_interior_kernel<S, 1, true, false>,
 							 S::NT, 0) != hipSuccess || per_cu < 1) {
       (void) hipGetLastError();
       return 256;
@@ -128,7 +128,7 @@ inline int le_xchunk(int m, int64_t ncol, int nplane, int slots) {
   return best;
 }
 
-template <class S, int ORDER, bool GZ>
+template <class S, int ORDER, bool GZ, bool GMU>
 void launch_both(const Geom & g, const FeParams & fe, const LbParams & prm, const LeGeom & le,
 		 const LeStepArgs & a, const double * phi, double * phi_out, const double * u_prev,
 		 double * u_next, double * rho, const double * g_in, double * g_out, double * force,
@@ -140,11 +140,11 @@ void launch_both(const Geom & g, const FeParams & fe, const LbParams & prm, cons
   const int nchunk = (m + xchunk - 1)/xchunk;
   dim3 block(S::TZ, S::TY, 1);
   const unsigned n1 = (unsigned) (bz*by*le.nplane*nchunk);
-  hipLaunchKernelGGL((le_step_interior_kernel<S, ORDER, GZ>), dim3(n1), block, 0, st, g, fe, prm, le, a,
+  hipLaunchKernelGGL((le_step_interior_kernel<S, ORDER, GZ, GMU>), dim3(n1), block, 0, st, g, fe, prm, le, a,
 		     xchunk, nchunk, bz, by, phi, phi_out, u_prev, u_next, rho, g_in, g_out, force, csum);
   (void) hipStreamWaitEvent(st, sums_done, 0);
   const unsigned n2 = (unsigned) (bz*by*2*le.nplane);
-  hipLaunchKernelGGL((le_step_edges_kernel<S, ORDER, GZ>), dim3(n2), block, 0, st, g, fe, prm, le, a,
+  hipLaunchKernelGGL((le_step_edges_kernel<S, ORDER, GZ, GMU>), dim3(n2), block, 0, st, g, fe, prm, le, a,
 		     bz, by, phi, phi_out, u_prev, u_next, rho, g_in, g_out, force, csum);
 }
 
@@ -161,11 +161,18 @@ void launch_le_step(const Geom & g, const FeParams & fe, const LbParams & prm, c
   a.ra = ra; a.jw = jw; a.je = je; a.frw = frw; a.fre = fre;
   using S = Shape<64, 8>;
   const bool gz = (prm.relax == 0);
-#define LB_(O, G) launch_both<S, O, G>(g, fe, prm, le, a, phi, phi_out, u_prev, u_next, rho, g_in, g_out, force, csum, st, sums_done)
-  if (fe.order == 1) { if (gz) LB_(1, true); else LB_(1, false); }
-  else if (fe.order == 2) { if (gz) LB_(2, true); else LB_(2, false); }
-  else if (fe.order == 3) { if (gz) LB_(3, true); else LB_(3, false); }
-  else { if (gz) LB_(4, true); else LB_(4, false); }
+  // the external-gradient term is compiled only into the kernels that see
+  // one: without it the interior kernel keeps 230 VGPRs, so two of its
+  // waves and a plane-sum wave (launched just before it on the second
+  // stream) still fit in a SIMD's 512 registers together
+#define LB_(O, G, M) launch_both<S, O, G, M>(g, fe, prm, le, a, phi, phi_out, u_prev, u_next, rho, g_in, g_out, force, csum, st, sums_done)
+#define LB_G(O, M) do { if (gz) LB_(O, true, M); else LB_(O, false, M); } while (0)
+#define LB_O(M) do { if (fe.order == 1) LB_G(1, M); else if (fe.order == 2) LB_G(2, M); \
+		     else if (fe.order == 3) LB_G(3, M); else LB_G(4, M); } while (0)
+  if (fe.gmu_on) LB_O(true);
+  else LB_O(false);
+#undef LB_O
+#undef LB_G
 #undef LB_
 }
 

@@ -104,8 +104,9 @@ __device__ __forceinline__ double pab_str(const double (&q)[5][QPLANE], int i, d
 
 // Flux-form force and the Cahn-Hilliard update at site (x, y, z) (x
 // unwrapped; y, z lattice coordinates for the face buffers), from the
-// rings as force_ch_site() reads them.
-template <class S, int ORDER>
+// rings as force_ch_site() reads them.  GMU: the external chemical
+// potential gradient's flux term is compiled in (fe.gmu_on then decides).
+template <class S, int ORDER, bool GMU>
 __device__ __forceinline__ void le_force_ch_site(const Smem<double, S> & s, const FeParams & fe,
 						 const LeSlab & L, const Geom & g, int x, int y, int z,
 						 int ty, int tz, const double (&uc)[9], double (&F)[3],
@@ -215,7 +216,7 @@ __device__ __forceinline__ void le_force_ch_site(const Smem<double, S> & s, cons
     fzp -= mob*(q0[4][c + 1] - m0);
     fzm -= mob*(m0 - q0[4][c - 1]);
   }
-  if (fe.gmu_on) adv::mu_ext(mob, fe.gmu, fw, fe_, fyp, fym, fzp, fzm);
+  if (GMU && fe.gmu_on) adv::mu_ext(mob, fe.gmu, fw, fe_, fyp, fym, fzp, fzm);
   // phi_ch_le_fix_fluxes (phi_cahn_hilliard.c:655-720) at the slab's edges
   if (L.chface != nullptr) {
     const int N = g.ny;
@@ -246,7 +247,7 @@ __device__ __forceinline__ void le_force_ch_site(const Smem<double, S> & s, cons
 
 // step_tile() over the x-planes [xb, xe) of one slab (unwrapped x); the
 // force is also stored (force_mode 0: fzero + F)
-template <class S, int ORDER, bool GZ>
+template <class S, int ORDER, bool GZ, bool GMU>
 __device__ __forceinline__ void le_step_tile(Smem<double, S> & s, const Geom & g, const FeParams & fe,
 					     const LbParams & prm, const LeSlab & L, int y0, int z0,
 					     int xb, int xe, const double * __restrict__ phi,
@@ -283,7 +284,7 @@ __device__ __forceinline__ void le_step_tile(Smem<double, S> & s, const Geom & g
     if (active) {
       const int64_t site = ((int64_t) xw*g.ny + y)*g.nz + z;
       double F[3], pnew;
-      le_force_ch_site<S, ORDER>(s, fe, L, g, x, y, z, ty, tz, uc, F, pnew,
+      le_force_ch_site<S, ORDER, GMU>(s, fe, L, g, x, y, z, ty, tz, uc, F, pnew,
 				 csum ? csum + site : nullptr);
       phi_out[site] = pnew;
       double fs[3];

@@ -54,6 +54,38 @@ def _off(d, nbytes):
     return ctypes.c_void_p(d.ptr + int(nbytes))
 
 
+def _weight_mask():
+    """Over the flat parameter vector: True for convolution weights, False
+    for biases (each layer is 27 cout cin weights, then cout biases)."""
+    keep = []
+    for name, ci, co in LAYERS:
+        keep += [np.ones(27 * co * ci, dtype=bool), np.zeros(co, dtype=bool)]
+    return np.concatenate(keep)
+
+
+def write_inference(path, w, step, norm_dict):
+    """The inference file of a flat FP32 parameter vector w (host code,
+    no device needed): BF16 weights, FP32 biases, the normaliser."""
+    keep = _weight_mask()
+    w = np.asarray(w, dtype=np.float32)
+    if w.size != keep.size:
+        raise ValueError(f"{w.size} parameters, the network has {keep.size}")
+    u = w[keep].view(np.uint32).astype(np.uint64)
+    bits = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    np.savez_compressed(path, w_bf16=bits, bias=w[~keep], step=np.array(step),
+                        **{"norm_" + k: np.array(v) for k, v in norm_dict.items()})
+
+
+def inference_from_training_file(src, dst, **gains):
+    """The inference file of a save() file (weights + optimiser state),
+    with the normaliser stored in it and, if given, the jump gains
+    (jump_gain_phi, jump_gain_u) replaced."""
+    d = np.load(src)
+    norm = {k[5:]: float(d[k]) for k in d.files if k.startswith("norm_")}
+    norm.update({k: float(v) for k, v in gains.items()})
+    write_inference(dst, d["w"], int(d["step"]), norm)
+
+
 class UNet:
     """Parameters, optimiser state and the forward/backward passes."""
 
@@ -126,15 +158,7 @@ class UNet:
         (round to nearest even, the rounding the MFMA copies get anyway),
         biases in FP32, and the normaliser.  Loading it reproduces the
         network's outputs exactly."""
-        w = self.w.download()
-        keep = np.ones(self.nparam, dtype=bool)
-        for name, ci, co in LAYERS:
-            o = self.layout[name][0]
-            keep[o + 27 * co * ci:o + 27 * co * ci + co] = False
-        u = w[keep].view(np.uint32).astype(np.uint64)
-        bits = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
-        np.savez_compressed(path, w_bf16=bits, bias=w[~keep], step=np.array(self.step_count),
-                            **{"norm_" + k: np.array(v) for k, v in norm.to_dict().items()})
+        write_inference(path, self.w.download(), self.step_count, norm.to_dict())
 
     def load(self, path):
         """Weights from save() or save_inference(); returns the normaliser
@@ -306,11 +330,21 @@ class UNet:
 
 
 class Normaliser:
-    """Scales of the network's inputs and outputs, from training data."""
+    """Scales of the network's inputs and outputs, from training data.
 
-    def __init__(self, u_scale, dphi_scale, du_scale, phi_shift=0.0, phi_scale=1.0):
+    jump_gain_phi, jump_gain_u: factors on the predicted changes of phi
+    and u when a jump is applied (decode), 1 by default.  A network trained
+    on the mean-square error predicts the conditional mean of the change,
+    which shrinks the part of a domain wall's displacement it cannot be
+    sure of, so its jumps advance the domain structure a little less than
+    the 250 steps they stand for; the gain is a scalar calibrated on
+    training replicas (surrogate_study.calibrate), never on held-out ones."""
+
+    def __init__(self, u_scale, dphi_scale, du_scale, phi_shift=0.0, phi_scale=1.0, jump_gain_phi=1.0,
+                 jump_gain_u=1.0):
         self.u_scale, self.dphi_scale, self.du_scale = float(u_scale), float(dphi_scale), float(du_scale)
         self.phi_shift, self.phi_scale = float(phi_shift), float(phi_scale)
+        self.jump_gain_phi, self.jump_gain_u = float(jump_gain_phi), float(jump_gain_u)
 
     @staticmethod
     def params(spec):
@@ -322,7 +356,11 @@ class Normaliser:
 
     def to_dict(self):
         return dict(u_scale=self.u_scale, dphi_scale=self.dphi_scale, du_scale=self.du_scale,
-                    phi_shift=self.phi_shift, phi_scale=self.phi_scale)
+                    phi_shift=self.phi_shift, phi_scale=self.phi_scale, jump_gain_phi=self.jump_gain_phi,
+                    jump_gain_u=self.jump_gain_u)
+
+    def with_gains(self, phi=1.0, u=1.0):
+        return Normaliser(self.u_scale, self.dphi_scale, self.du_scale, self.phi_shift, self.phi_scale, phi, u)
 
 
 def encode(state, nsite, spec, normaliser, out, offset_sites=0):
@@ -341,8 +379,11 @@ PHI_MAX, U_MAX = 1.5, 0.1      # clamps on a jump's result (|phi| ~ 1, |u| << 0.
 
 
 def decode(now, y, nsite, normaliser, out, offset_sites=0):
+    """The state after a jump: now + predicted change (times the
+    normaliser's jump gains), clamped."""
     _ck(lib().ldw_nn_decode(_v(now), _off(y, 4 * COUT * offset_sites), COUT, nsite, _v(out),
-                            normaliser.dphi_scale, normaliser.du_scale, PHI_MAX, U_MAX), "ldw_nn_decode")
+                            normaliser.dphi_scale / normaliser.jump_gain_phi,
+                            normaliser.du_scale / normaliser.jump_gain_u, PHI_MAX, U_MAX), "ldw_nn_decode")
 
 
 # The 48 elements of the cubic group (axis permutations x reflections).

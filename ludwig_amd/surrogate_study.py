@@ -162,6 +162,44 @@ def fresh_specs(nrep, size=64, train_nrep=256, train_seed=2026):
     return specs
 
 
+def train_specs(nrep=256, seed=2026, size=64):
+    """The training 80 % of the training ensemble's replicas, optionally
+    only those of one lattice size."""
+    specs = draw_replicas(nrep, seed=seed)
+    tr_idx, _ = split_by_seed(specs, 0.2, seed)
+    return [specs[i] for i in tr_idx if size is None or specs[i].nx == size]
+
+
+def calibrate(weights, gains=(1.0, 1.05, 1.1, 1.15, 1.2), size=64, corr=20, every=250, jumps=8,
+              t_start=500, max_replicas=32, conserve=True, u_too=False, log=print):
+    """The jump gain (Normaliser.jump_gain_phi; with u_too also the
+    velocity's) that removes the mean L(T) bias of the jump pipeline on
+    TRAINING replicas of one size: each gain's signed mean L error, mean
+    |L error| and ensemble-mean L error; returns (best gain, rows)."""
+    net = UNet(seed=0)
+    norm = net.load(weights)
+    specs = train_specs(size=size)[:max_replicas]
+    rows = []
+    for g in gains:
+        nm = norm.with_gains(g, g if u_too else 1.0)
+        ev = evaluate(net, nm, specs, every, jumps, corr, t_start=t_start, log=lambda m: None,
+                      variants={"surrogate": ("pred", conserve)}, control=False, identity=False)
+        signed = float(np.mean([(r["L_surrogate"] - r["L_plain"]) / r["L_plain"] for r in ev["per_replica"]]))
+        rows.append({"gain": g, "signed_mean": signed, "abs_mean": ev["L_err_surrogate_mean"],
+                     "ensemble_mean": ev["meanL_err_surrogate"], "speedup": ev["speedup"]})
+        log(f"calibration gain {g}: {rows[-1]}")
+    best = min(rows, key=lambda r: abs(r["signed_mean"]))["gain"]
+    return best, rows
+
+
+def write_gains(src, dst, phi=1.0, u=1.0):
+    """A copy of the weights file src with the jump gains set."""
+    d = dict(np.load(src))
+    d["norm_jump_gain_phi"] = np.array(float(phi))
+    d["norm_jump_gain_u"] = np.array(float(u))
+    np.savez_compressed(dst, **d)
+
+
 def gain(weights=WEIGHTS, size=64, corrs=(20, 30), every=250, jumps=8, t_start=500,
          conserve=True, log=None, fresh=0):
     """Surrogate + LB against plain LB on the held-out replicas (all sizes
