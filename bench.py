@@ -173,6 +173,32 @@ def run_ensemble(nrep, steps, warmup, precision=32, seed=2026):
             "mass_first_last": mass}
 
 
+def run_conserve2(n=128, steps=100, warmup=5):
+    """cahn_hilliard_options_conserve 2 at n^3 FP64: ms per step with the
+    default compensated tree sum and with the one-thread site-order sum
+    (the reference's one-thread order, bit-identical with it)."""
+    import ludwig_amd as L
+    lib = L.lib()
+    out = {"n": n, "steps": steps}
+    for mode, key in ((0, "tree_sum"), (1, "site_order_sum")):
+        lib.ldw_conserve2_site_order(mode)
+        b = L.BinaryFluid((n, n, n), precision=64, fatal=True, conserve=2, **C2)
+        b.init_spinodal(SEED, 0.0, 0.1)
+        _, p0 = b.conserved_sums()
+        b.step(warmup)
+        b.synchronize()
+        nst = steps if mode == 0 else max(10, steps // 10)
+        t0 = time.perf_counter()
+        b.step(nst)
+        b.synchronize()
+        out[key + "_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / nst
+        _, p1 = b.conserved_sums()
+        out[key + "_phi_sum_drift"] = abs(p1 - p0)
+        b.close()
+    lib.ldw_conserve2_site_order(0)
+    return out
+
+
 # Lees-Edwards path (FP64).  The single-pass step (two launches of the
 # tiled kernel, timer "le_step") reads phi, u and f and writes phi, rho, u,
 # force and f: 400 bytes per site, the plane-free step's 376 plus the stored
@@ -461,6 +487,9 @@ def main():
             "wall": e64["ms_per_step"] / e32["ms_per_step"],
             "events": e64["event_ms_per_step"] / e32["event_ms_per_step"]}
 
+        r = run_conserve2(steps=xsteps)
+        extra["conserve2_128^3_fp64"] = r
+        print(f"# conserve 2: {r}", file=sys.stderr, flush=True)
         r = run_le(steps=xsteps)
         extra["lees_edwards_128^3_2planes_fp64"] = r
         print(f"# lees-edwards: {r}", file=sys.stderr, flush=True)
