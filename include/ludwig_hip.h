@@ -364,6 +364,11 @@ int hydro_f_local_set(hydro_t * obj, int index, const double force[3]);
  * u.iodata (hydro.c:941-967). */
 int hydro_io_write(hydro_t * obj, int timestep, io_event_t * event); /* hydro.c:941 */
 int hydro_io_read(hydro_t * obj, int timestep, io_event_t * event);  /* hydro.c:959 */
+/* rho alone and velocity alone, the individual requests between
+ * configuration steps: the reference calls field_io_write() on hydro->rho
+ * and hydro->u (ludwig.c:951-960); hydro_t is opaque here */
+int hydro_io_write_rho(hydro_t * obj, int timestep, io_event_t * event);
+int hydro_io_write_vel(hydro_t * obj, int timestep, io_event_t * event);
 
 /* ------------------------------------------------------------------------ */
 /* Symmetric free energy (symmetric.h)                                      */

@@ -2558,6 +2558,22 @@ extern "C" int hydro_io_write(hydro_t * h, int timestep, io_event_t *) {
 		  h->u, h->nsite, 3, 3);
 }
 
+// The individual rho and velocity outputs between configuration steps
+// (ludwig.c:951-960, field_io_write() of hydro->rho and hydro->u)
+extern "C" int hydro_io_write_rho(hydro_t * h, int timestep, io_event_t *) {
+  if (!h) FAIL("hydro_io_write_rho: null hydro");
+  if (hydro_memcpy(h, tdpMemcpyDeviceToHost)) return -1;
+  return io_write("hydro_io_write_rho", "rho", timestep, h->rho_io.output, h->rho_meta_written, h->cs,
+		  h->rho, h->nsite, 1, 1);
+}
+
+extern "C" int hydro_io_write_vel(hydro_t * h, int timestep, io_event_t *) {
+  if (!h) FAIL("hydro_io_write_vel: null hydro");
+  if (hydro_memcpy(h, tdpMemcpyDeviceToHost)) return -1;
+  return io_write("hydro_io_write_vel", "vel", timestep, h->u_io.output, h->u_meta_written, h->cs,
+		  h->u, h->nsite, 3, 3);
+}
+
 // rho and u from the host copy to the device; the force is left alone
 extern "C" int hydro_io_read(hydro_t * h, int timestep, io_event_t *) {
   if (!h) FAIL("hydro_io_read: null hydro");

@@ -461,10 +461,6 @@ int main(int argc, char ** argv) {
   hydro_options_t ho = hydro_options_default();
   ho.rho.iodata = io_args_keys("rho");
   ho.u.iodata = io_args_keys("vel");
-  if (ho.rho.iodata.iofreq > 0 || ho.u.iodata.iofreq > 0) {
-    fatal("rho and vel output between configuration steps (rho_io_freq, vel_io_freq, "
-	  "default_io_freq) is not supported by this build", NULL);
-  }
   hydro_create(pe, cs, le, &ho, &hydro);
   map_options_t mo = map_options_default();
   map_create(pe, cs, &mo, &map);
@@ -628,6 +624,19 @@ int main(int argc, char ** argv) {
       io_event_t event = {0};
       printf("Writing rho/velocity output at step %d!\n", step);
       hydro_io_write(hydro, step, &event);
+    }
+    else {
+      /* individual requests (ludwig.c:951-960) */
+      if (util_mod(step, ho.rho.iodata.iofreq) == 0) {
+	io_event_t event = {0};
+	printf("Writing rho output at step %d!\n", step);
+	hydro_io_write_rho(hydro, step, &event);
+      }
+      if (util_mod(step, ho.u.iodata.iofreq) == 0) {
+	io_event_t event = {0};
+	printf("Writing velocity output at step %d!\n", step);
+	hydro_io_write_vel(hydro, step, &event);
+      }
     }
 
     if (freq_stats > 0 && step % freq_stats == 0) {

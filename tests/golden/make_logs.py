@@ -38,6 +38,9 @@ CHAINS = {
     # two Lees-Edwards planes (serial-le3d-st1 with advection order 3): the
     # metadata records the planes, and the restarted time counter
     "le-restart": ["le-io1", "le-io2", "le-io3"],
+    # rho and velocity files between configuration steps (rho_io_freq,
+    # vel_io_freq), the velocity in ascii records
+    "hydro-io": ["hydro-io1"],
 }
 
 
