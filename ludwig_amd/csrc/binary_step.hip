@@ -103,7 +103,7 @@ void launch_shape(const Geom & g, const FeParams & fe, const LbParams & prm, con
   const T f0 = T(fzero[0]), f1 = T(fzero[1]), f2 = T(fzero[2]);
   const bool gz = (prm.relax == 0);
 #define L_(O, M, G) hipLaunchKernelGGL((binary_step_kernel<T, S, O, M, G, CONS, GMU>), grid, block, 0, st, \
-				       g, fe, prm, xchunk, bz, by, bx, tune_remap(), phi, phi_out, u_prev, u_next, rho, \
+				       g, fe, prm, xchunk, bz, by, bx, tune_remap_step(bz*by, ntile), phi, phi_out, u_prev, u_next, rho, \
 				       g_in, g_out, f0, f1, f2, csum)
 #define L_G(O, M) do { if (gz) L_(O, M, true); else L_(O, M, false); } while (0)
 #define L_M(O) do { if (fe.force_method == 0) L_G(O, 0); else L_G(O, 1); } while (0)

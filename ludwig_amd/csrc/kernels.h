@@ -41,6 +41,9 @@ struct FeParams {
 // LDW_XCD_REMAP (1: XCD-contiguous tile ranges, 0: linear order, default) and
 // LDW_MIN_BLOCKS (x-chunks are halved until the grid has this many blocks).
 int tune_remap(void);
+// the tile order of the single-lattice step for ncol column tiles and
+// ntile tiles in all (LDW_XCD_REMAP overrides)
+int tune_remap_step(int ncol, int ntile);
 int tune_min_blocks(void);
 // tile of the single-pass step kernel: 0 = 64 x 4, 1 = 64 x 8, 2 = 128 x 4
 // (z x y); LDW_STEP_TILE forces one, else chosen by precision and size

@@ -115,6 +115,21 @@ int tune_remap(void) {
   return v;
 }
 
+int tune_remap_step(int ncol, int ntile) {
+  static int v = -2;
+  if (v == -2) {
+    const char * e = getenv("LDW_XCD_REMAP");
+    v = e ? (atoi(e) ? 1 : 0) : -1;
+  }
+  if (v >= 0) return v;
+  // measured on MI355X (binary_step_kernel, FP64, 64 x 8 tiles): the
+  // XCD-contiguous order is 2.5 % faster at 128^3, where each XCD gets
+  // whole x-chunks (32 tiles = every column tile of one chunk), and 7 %
+  // slower at 256^3, where it gets a quarter of a chunk's columns
+  const int per = (ntile + 7)/8;
+  return (ntile % 8 == 0 && per % ncol == 0) ? 1 : 0;
+}
+
 int tune_min_blocks(void) {
   static int v = -1;
   if (v < 0) v = env_int("LDW_MIN_BLOCKS", 2048);

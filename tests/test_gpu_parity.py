@@ -336,7 +336,8 @@ def test_single_fluid_c1_golden(golden):
     ((21, 34, 70), 4, 5, 0, (0.0, 0.0, 0.0)),           # order 4, single-pass step, tile edges
     ((19, 23, 11), 5, 5, 1, (0.0, 0.0, 0.0)),           # order 5 wraps three sites on 11
     ((25, 12, 66), 4, 4, 1, (2e-4, -1e-4, 3e-4)),      # grad_mu with phi grad mu
-    ((14, 30, 20), 2, 4, 0, (-3e-4, 0.0, 1e-4))])      # grad_mu, stress divergence
+    ((14, 30, 20), 2, 4, 0, (-3e-4, 0.0, 1e-4)),       # grad_mu, stress divergence
+    ((128, 16, 64), 1, 4, 0, (0.0, 0.0, 0.0))])        # XCD-contiguous tile order (whole x-chunks per XCD)
 def test_binary_nonuniform_box_vs_oracle(shape, order, nsteps, method, gm):
     """Non-cubic, odd sizes: the periodic wrap in every kernel, for every
     advection order and with an external chemical potential gradient."""
