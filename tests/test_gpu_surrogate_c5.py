@@ -13,9 +13,9 @@ deterministic (fixed-order reductions, tile shapes that change only which
 block computes a site), so the errors below are reproducible to the last
 digit; only the speedup varies with the machine.
 
-Measured on MI355X: speedup 5.6x, ensemble-mean L(T) error 0.02 %, mean
-per-replica |L(T) error| 3.6 % (max 11 %).  The per-replica 2 % target is
-not met; the bounds assert what is."""
+Measured on MI355X: speedup 5.8x, ensemble-mean L(T) error 0.39 %, mean
+per-replica |L(T) error| 2.9 % (max 10.7 %).  The per-replica 2 % target
+is not met; the bounds assert what is."""
 
 import pytest
 
@@ -33,5 +33,5 @@ def test_c5_surrogate_gain_on_held_out_and_fresh_64():
     assert c["T"] == 2160
     assert c["speedup"] >= 5.0, c
     assert c["L_err_ensemble_mean"] <= 0.02, c
-    assert c["L_err_per_replica_mean"] <= 0.04, c
+    assert c["L_err_per_replica_mean"] <= 0.035, c
     assert abs(c["L_err_per_replica_signed_mean"]) <= 0.02, c
