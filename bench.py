@@ -55,9 +55,9 @@ KERNEL_DESC = {
 }
 # HBM bytes per site of binary_step_kernel (FP64, default 64 x 8 tiles) from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's
-# reading for the plain copy kernel (fetch x 1.9884, write x 0.9706);
+# reading for the plain copy kernel (fetch x 1.9885, write x 0.9701);
 # profiles/binary_step_hbm_traffic_pmc.md
-TRAFFIC_B_SITE = {("binary_step", 64, 128): 425.5, ("binary_step", 64, 256): 421.1}
+TRAFFIC_B_SITE = {("binary_step", 64, 128): 388.4, ("binary_step", 64, 256): 421.0}
 C2 = dict(a=-0.0625, b=0.0625, kappa=0.04, mobility=0.25, tau=1.0, order=1)
 SEED = 8361235
 
