@@ -3,8 +3,8 @@
 // The reference performs, per step (ludwig.c:528-860): the thermodynamic
 // force from phi_t, the Cahn-Hilliard update phi_t -> phi_t+1 with the
 // velocity of the previous collision, the collision of f_t with that force
-// (giving rho_t, u_t), and the propagation.  Here one block owns a 4 x 64
-// tile of (y, z) columns and marches along x through a chunk of planes
+// (giving rho_t, u_t), and the propagation.  Here one block owns a tile of
+// (y, z) columns (8 x 64 by default) and marches along x through a chunk of planes
 // (step_tile.h): for each site it forms F and the new phi from the LDS
 // tiles, then collides the site's 19 pulled distributions with that F.  The
 // force never goes to memory.

@@ -20,8 +20,10 @@ namespace ldw {
 namespace tile {
 
 // Tile shape: TZ x TY sites of a plane, one thread per site (block
-// dimensions (TZ, TY)).  The single-lattice kernels use 64 x 4; the replica
-// ensemble chooses among several shapes at run time (ensemble.hip).
+// dimensions (TZ, TY)).  The single-lattice step chooses 64 x 4, 64 x 8 or
+// 128 x 4 by precision and lattice size (tune_step_tile); the staged
+// force/Cahn-Hilliard kernel uses 64 x 4; the replica ensemble chooses among
+// several shapes at run time (ensemble.hip).
 template <int TZ_, int TY_>
 struct Shape {
   static constexpr int TZ = TZ_, TY = TY_;

@@ -53,21 +53,19 @@ __device__ __forceinline__ LeSlab slab_of(const Geom & g, const LeGeom & le, con
   return L;
 }
 
-// launch 1: the slab interiors, tiles z-fastest, then y, then (slab, chunk)
+// launch 1: the slab interiors, tiles z-fastest, then y, then (slab, chunk),
+// in the step kernel's block-to-tile order (tile_of_block)
 template <class S, int ORDER, bool GZ, bool GMU>
 __global__ void __launch_bounds__(S::NT)
 le_step_interior_kernel(Geom g, FeParams fe, LbParams prm, LeGeom le, LeStepArgs a, int xchunk,
-			int nchunk, int nbz, int nby, const double * __restrict__ phi,
+			int nchunk, int nbz, int nby, int remap, const double * __restrict__ phi,
 			double * __restrict__ phi_out, const double * __restrict__ u_prev,
 			double * __restrict__ u_next, double * __restrict__ rho,
 			const double * __restrict__ g_in, double * __restrict__ g_out,
 			double * __restrict__ force, double * __restrict__ csum) {
   __shared__ Smem<double, S> s;
-  const int t = blockIdx.x;
-  const int bzi = t % nbz;
-  const int byi = (t/nbz) % nby;
-  const int ck = t/(nbz*nby);
-  if (ck >= le.nplane*nchunk) return;
+  int bzi, byi, ck;
+  if (!tile_of_block(nbz, nby, le.nplane*nchunk, remap, bzi, byi, ck)) return;
   const int sl = ck/nchunk, k = ck % nchunk;
   const LeSlab L = slab_of(g, le, a, sl, false);
   const int xb = L.slab_lo + 1 + k*xchunk;
@@ -139,9 +137,13 @@ void launch_both(const Geom & g, const FeParams & fe, const LbParams & prm, cons
   const int xchunk = le_xchunk(m, (int64_t) bz*by, le.nplane, resident_slots_le<S>());
   const int nchunk = (m + xchunk - 1)/xchunk;
   dim3 block(S::TZ, S::TY, 1);
-  const unsigned n1 = (unsigned) (bz*by*le.nplane*nchunk);
+  // the tile order of the plane-free step: XCD-contiguous where every
+  // XCD's range is whole (slab, chunk) x-ranges
+  const int ntile = bz*by*le.nplane*nchunk;
+  const int remap = tune_remap_step(bz*by, ntile);
+  const unsigned n1 = (unsigned) (remap ? ((ntile + NXCD - 1)/NXCD)*NXCD : ntile);
   hipLaunchKernelGGL((le_step_interior_kernel<S, ORDER, GZ, GMU>), dim3(n1), block, 0, st, g, fe, prm, le, a,
-		     xchunk, nchunk, bz, by, phi, phi_out, u_prev, u_next, rho, g_in, g_out, force, csum);
+		     xchunk, nchunk, bz, by, remap, phi, phi_out, u_prev, u_next, rho, g_in, g_out, force, csum);
   (void) hipStreamWaitEvent(st, sums_done, 0);
   const unsigned n2 = (unsigned) (bz*by*2*le.nplane);
   hipLaunchKernelGGL((le_step_edges_kernel<S, ORDER, GZ, GMU>), dim3(n2), block, 0, st, g, fe, prm, le, a,
