@@ -151,6 +151,37 @@ def test_c4_sizes_match_single_lattice():
     e.close()
 
 
+def test_c4_full_256_replicas():
+    # C4 at its stated size, the bench's workload: 256 replicas of sizes
+    # {48, 64, 80, 96} (110.5 M sites) in one persistent launch per step,
+    # autotuned, through the graph.  One replica of each size and the last
+    # one are checked bit for bit against single-lattice runs; every
+    # replica keeps its mass and its phi sum.
+    specs = draw_replicas(256, seed=2026)
+    e = Ensemble(specs, precision=32)
+    print(f"256 replicas, {e.nsite} sites", flush=True)
+    assert e.nsite == 110460928
+    sums0 = [e.conserved_sums(r) for r in range(len(specs))]
+    e.tune(reps=1)
+    print(f"tuned {e.config()}", flush=True)
+    e.step(10)
+    check = {}
+    for r, s in enumerate(specs):
+        check.setdefault(s.nx, r)
+    for r in sorted(set(check.values()) | {len(specs) - 1}):
+        phi, rho, u, f = single_run(specs[r], 10, 32)
+        ephi, erho, eu = e.snapshot(r)
+        assert_exact(ephi, phi, f"replica {r} (n={specs[r].nx}) phi")
+        assert_exact(erho, rho, f"replica {r} (n={specs[r].nx}) rho")
+        assert_exact(eu, u, f"replica {r} (n={specs[r].nx}) u")
+    for r, s in enumerate(specs):
+        n3 = s.nx * s.ny * s.nz
+        m, p = e.conserved_sums(r)
+        assert abs(m - n3) <= 1e-6 * n3, f"replica {r}: mass {m} against {n3}"
+        assert abs(p - sums0[r][1]) <= 1e-6 * n3, f"replica {r}: phi sum {p} against {sums0[r][1]}"
+    e.close()
+
+
 def test_bad_arguments_fail_loudly():
     with pytest.raises(RuntimeError, match="tile"):
         e = Ensemble(small_specs(), precision=32)
