@@ -282,9 +282,11 @@ conv3_fwd_lds_kernel(const uint16_t * __restrict__ x, const uint16_t * __restric
 constexpr int HPT = (LH_N*4 + 255)/256;       // halo uint4 per thread
 
 // One stage (taps dx, dy, dz = 0..2) of the pipelined kernel on the LDS
-// tiles: K = 32 MFMAs on 8-channel fragments, or for a 16-channel chunk
-// K = 16 MFMAs, lane (r, q) holding channels 4q .. 4q+3 (the (q & 1) half
-// of 16-byte part q >> 1).
+// tiles: K = 32 MFMAs on 8-channel fragments.  For a 16-channel chunk taps
+// dz = 0 and 1 share one K = 32 MFMA and tap dz = 2 takes a K = 16 MFMA,
+// lane (r, q) holding channels 4q .. 4q+3 (the (q & 1) half of 16-byte
+// part q >> 1): on gfx950 a K = 16 MFMA occupies the unit as long as a
+// K = 32 one, so the pair saves a third of the chunk's MFMA time.
 template <int NT, bool K16>
 __device__ __forceinline__ void conv_stage(const uint4 * sa, const uint4 (*sb)[16*NT][LK/8], f4 (&acc)[4][NT],
 					   int wave, int r, int q, int dx, int dy, int swz) {
@@ -307,7 +309,27 @@ __device__ __forceinline__ void conv_stage(const uint4 * sa, const uint4 (*sb)[1
 							       __builtin_bit_cast(b8, bv[n]), acc[j][n], 0, 0, 0);
       }
     }
-    else {
+    else if (dz == 0) {
+      // taps dz = 0 and 1 in one K = 32 MFMA: lanes q = 0, 1 hold tap 0's
+      // 16 channels, lanes q = 2, 3 tap 1's (8 channels each, part q & 1)
+      const int dq = q >> 1;
+      s8 bv[NT];
+#pragma unroll
+      for (int n = 0; n < NT; n++) {
+	const int row = dq*16*NT + 16*n + r;
+	bv[n] = *reinterpret_cast<const s8 *>(&sb[dq][16*n + r][lds_part(row, q & 1, swz)]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+	const int h = ((wave + dx)*LH_Y + (j + dy))*LH_Z + (r + dq);
+	const s8 a = *reinterpret_cast<const s8 *>(&sa[h*4 + lds_part(h, q & 1, swz)]);
+#pragma unroll
+	for (int n = 0; n < NT; n++)
+	  acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a),
+							       __builtin_bit_cast(b8, bv[n]), acc[j][n], 0, 0, 0);
+      }
+    }
+    else if (dz == 2) {
       s4 bv[NT];
 #pragma unroll
       for (int n = 0; n < NT; n++) {
