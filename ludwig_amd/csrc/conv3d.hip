@@ -608,7 +608,7 @@ int conv_swz() {
 
 // LDW_CONV_LDS=0 selects the fragment-direct kernel everywhere, =1 the
 // tiled kernel without the pipeline (A/B timing); by default the
-// pipelined persistent kernel
+// pipelined kernel (up to 64 output channels per block)
 int conv_lds_enabled() {
   static int on = -1;
   if (on < 0) {
