@@ -50,7 +50,7 @@ force_ch_kernel(Geom g, FeParams fe, int xchunk, int nbz, int nby, int nbx, int 
 
   // prologue: phi planes xb-2 .. xb+1, Q planes xb-1, xb; registers hold
   // phi plane xb+2 and the velocities of plane xb
-  for (int x = xb - 2; x <= xb + 1; x++) load_phi_plane(s, phi, g, x, y0, z0);
+  load_phi_planes<T, S, 4>(s, phi, g, xb - 2, y0, z0);
   T pre[S::NPRE];
   prefetch_phi_plane<T, S>(pre, phi, g, xb + 2, y0, z0);
   T uc[9];

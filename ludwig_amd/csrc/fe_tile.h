@@ -72,19 +72,6 @@ struct Smem {
   T q[3][5][S::QPLANE];       // g0, g1, g2, p0, mu
 };
 
-template <typename T, class S>
-__device__ __forceinline__ void load_phi_plane(Smem<T, S> & s, const T * __restrict__ phi,
-					       const Geom & g, int x, int y0, int z0) {
-  const int slot = modn(x, 5);
-  const int xw = modn(x, g.nx);
-  for (int i = threadIdx.x + threadIdx.y*S::TZ; i < S::PPLANE; i += S::NT) {
-    const int yy = i / S::PZ, zz = i - yy*S::PZ;
-    const int y = modn(y0 - 2 + yy, g.ny);
-    const int z = modn(z0 - 2 + zz, g.nz);
-    s.phi[slot][i] = phi[((int64_t) xw*g.ny + y)*g.nz + z];
-  }
-}
-
 // Software pipelining: the phi plane needed by the next iteration is
 // fetched into registers one iteration ahead, so its memory latency overlaps
 // the LDS compute of the current plane.
@@ -112,6 +99,19 @@ __device__ __forceinline__ void store_phi_plane(Smem<T, S> & s, const T (&pre)[S
     const int i = threadIdx.x + threadIdx.y*S::TZ + k*S::NT;
     if (i < S::PPLANE) s.phi[slot][i] = pre[k];
   }
+}
+
+// The prologue's phi planes x0 .. x0+N-1 into the ring.  Every load is
+// issued before the first LDS store, so the N planes cost one memory
+// latency (a load-then-store loop per plane waits for each load in turn).
+template <typename T, class S, int N>
+__device__ __forceinline__ void load_phi_planes(Smem<T, S> & s, const T * __restrict__ phi,
+						const Geom & g, int x0, int y0, int z0) {
+  T p[N][S::NPRE];
+#pragma unroll
+  for (int k = 0; k < N; k++) prefetch_phi_plane<T, S>(p[k], phi, g, x0 + k, y0, z0);
+#pragma unroll
+  for (int k = 0; k < N; k++) store_phi_plane(s, p[k], x0 + k);
 }
 
 // velocity components the fluxes of site (x, y, z) need:

@@ -58,18 +58,6 @@ __device__ __forceinline__ const double * le_plane_src(const LeSlab & L, const d
 }
 
 template <class S>
-__device__ __forceinline__ void le_load_phi_plane(Smem<double, S> & s, const double * __restrict__ pl,
-						  const Geom & g, int x, int y0, int z0) {
-  const int slot = modn(x, 5);
-  for (int i = threadIdx.x + threadIdx.y*S::TZ; i < S::PPLANE; i += S::NT) {
-    const int yy = i / S::PZ, zz = i - yy*S::PZ;
-    const int y = modn(y0 - 2 + yy, g.ny);
-    const int z = modn(z0 - 2 + zz, g.nz);
-    s.phi[slot][i] = pl[(int64_t) y*g.nz + z];
-  }
-}
-
-template <class S>
 __device__ __forceinline__ void le_prefetch_phi_plane(double (&pre)[S::NPRE], const double * __restrict__ pl,
 						      const Geom & g, int y0, int z0) {
 #pragma unroll
@@ -82,6 +70,19 @@ __device__ __forceinline__ void le_prefetch_phi_plane(double (&pre)[S::NPRE], co
       pre[k] = pl[(int64_t) y*g.nz + z];
     }
   }
+}
+
+// the prologue's four phi planes x0 .. x0+3 (real or buffer planes), every
+// load issued before the first LDS store (load_phi_planes)
+template <class S>
+__device__ __forceinline__ void le_load_phi_planes4(Smem<double, S> & s, const LeSlab & L,
+						    const double * __restrict__ phi, const Geom & g,
+						    int x0, int y0, int z0) {
+  double p[4][S::NPRE];
+#pragma unroll
+  for (int k = 0; k < 4; k++) le_prefetch_phi_plane<S>(p[k], le_plane_src(L, phi, g, x0 + k), g, y0, z0);
+#pragma unroll
+  for (int k = 0; k < 4; k++) store_phi_plane(s, p[k], x0 + k);
 }
 
 // velocities of load_u(); at the slab's edges u_x of the neighbour across
@@ -260,7 +261,7 @@ __device__ __forceinline__ void le_step_tile(Smem<double, S> & s, const Geom & g
   const int y = y0 + ty, z = z0 + tz;
   const bool active = (y < g.ny) && (z < g.nz);
 
-  for (int x = xb - 2; x <= xb + 1; x++) le_load_phi_plane(s, le_plane_src(L, phi, g, x), g, x, y0, z0);
+  le_load_phi_planes4<S>(s, L, phi, g, xb - 2, y0, z0);
   double pre[S::NPRE];
   le_prefetch_phi_plane<S>(pre, le_plane_src(L, phi, g, xb + 2), g, y0, z0);
   __syncthreads();

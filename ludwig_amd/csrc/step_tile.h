@@ -58,7 +58,7 @@ __device__ __forceinline__ void step_tile(Smem<T, S> & s, const Geom & g, const 
   const int y = y0 + ty, z = z0 + tz;
   const bool active = (y < g.ny) && (z < g.nz);
 
-  for (int x = xb - 2; x <= xb + 1; x++) load_phi_plane(s, phi, g, x, y0, z0);
+  load_phi_planes<T, S, 4>(s, phi, g, xb - 2, y0, z0);
   T pre[S::NPRE];
   prefetch_phi_plane<T, S>(pre, phi, g, xb + 2, y0, z0);
   __syncthreads();
