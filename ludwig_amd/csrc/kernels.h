@@ -354,9 +354,13 @@ void launch_le_plane_sums(const Geom & g, const LeGeom & le, const double * xfac
 // the whole step in two launches of the tiled single-pass kernel
 // (le_step.hip): slab interiors, then (after sums_done) the slab edges with
 // the sums fcor; force stored; csum for conserve 1 (or null)
+// The edge launch also forms the LB plane reprojection of its sites into
+// sbuf (duy: the velocity jump per side), so that launch_le_lb_interp
+// completes the plane condition.
 void launch_le_step(const Geom & g, const FeParams & fe, const LbParams & prm, const LeGeom & le,
 		    const double * phib, const double * ub, const double * fcor,
 		    const double * chface, double ra, int jw, double frw, int je, double fre,
+		    const double duy[2], double * sbuf,
 		    const double * phi, double * phi_out, const double * u_prev, double * u_next,
 		    double * rho, const double * g_in, double * g_out, double * force, double * csum,
 		    hipStream_t st, hipEvent_t sums_done);
@@ -365,6 +369,10 @@ void launch_le_step(const Geom & g, const FeParams & fe, const LbParams & prm, c
 // sites, side 1 cx = -1 above; dj/fr per side, duy the velocity jump
 void launch_le_lb_bc(const Geom & g, const LeGeom & le, const int dj[2], const double fr[2],
 		     const double duy[2], double * gdist, double * sbuf, hipStream_t s);
+// the second half of it alone: the reprojected values in sbuf displaced
+// and interpolated into the plane sites of g
+void launch_le_lb_interp(const Geom & g, const LeGeom & le, const int dj[2], const double fr[2],
+			 double * gdist, const double * sbuf, hipStream_t s);
 
 }  // namespace ldw
 

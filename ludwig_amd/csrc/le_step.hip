@@ -32,6 +32,8 @@ struct LeStepArgs {
   double ra;
   int jw, je;
   double frw, fre;
+  double * sbuf;               // LB plane reprojections (launch 2)
+  double duy0, duy1;
 };
 
 __device__ __forceinline__ LeSlab slab_of(const Geom & g, const LeGeom & le, const LeStepArgs & a, int sl,
@@ -50,6 +52,8 @@ __device__ __forceinline__ LeSlab slab_of(const Geom & g, const LeGeom & le, con
   L.chface = edges ? a.chface : nullptr;
   L.jw = a.jw; L.je = a.je;
   L.frw = a.frw; L.fre = a.fre;
+  L.sbuf = edges ? a.sbuf : nullptr;
+  L.duy0 = a.duy0; L.duy1 = a.duy1;
   return L;
 }
 
@@ -91,8 +95,8 @@ le_step_edges_kernel(Geom g, FeParams fe, LbParams prm, LeGeom le, LeStepArgs a,
   if (e >= 2*le.nplane) return;
   const LeSlab L = slab_of(g, le, a, e/2, true);
   const int x = (e % 2) ? L.slab_hi : L.slab_lo;
-  le_step_tile<S, ORDER, GZ, GMU>(s, g, fe, prm, L, byi*S::TY, bzi*S::TZ, x, x + 1, phi, phi_out, u_prev,
-			     u_next, rho, g_in, g_out, force, csum, 0.0, 0.0, 0.0);
+  le_step_tile<S, ORDER, GZ, GMU, true>(s, g, fe, prm, L, byi*S::TY, bzi*S::TZ, x, x + 1, phi, phi_out, u_prev,
+				   u_next, rho, g_in, g_out, force, csum, 0.0, 0.0, 0.0);
 }
 
 template <class S>
@@ -155,12 +159,14 @@ void launch_both(const Geom & g, const FeParams & fe, const LbParams & prm, cons
 void launch_le_step(const Geom & g, const FeParams & fe, const LbParams & prm, const LeGeom & le,
 		    const double * phib, const double * ub, const double * fcor,
 		    const double * chface, double ra, int jw, double frw, int je, double fre,
+		    const double duy[2], double * sbuf,
 		    const double * phi, double * phi_out, const double * u_prev, double * u_next,
 		    double * rho, const double * g_in, double * g_out, double * force, double * csum,
 		    hipStream_t st, hipEvent_t sums_done) {
   LeStepArgs a;
   a.phib = phib; a.ub = ub; a.fcor = fcor; a.chface = chface;
   a.ra = ra; a.jw = jw; a.je = je; a.frw = frw; a.fre = fre;
+  a.sbuf = sbuf; a.duy0 = duy[0]; a.duy1 = duy[1];
   using S = Shape<64, 8>;
   const bool gz = (prm.relax == 0);
   // the external-gradient term is compiled only into the kernels that see

@@ -25,6 +25,7 @@
 #ifndef LDW_LE_STEP_TILE_H
 #define LDW_LE_STEP_TILE_H
 
+#include "le_lb.h"
 #include "step_tile.h"
 
 namespace ldw {
@@ -46,6 +47,8 @@ struct LeSlab {
   const double * chface;      // raw Cahn-Hilliard x faces [plane][side][nyz]
   int jw, je;                 // integer and fractional split of +dy and -dy
   double frw, fre;
+  double * sbuf;              // LB plane reprojections [plane][side][5][nyz]
+  double duy0, duy1;          // their velocity jump, side 0 and side 1
 };
 
 // where phi plane x (unwrapped) of the slab's ring comes from
@@ -247,8 +250,11 @@ __device__ __forceinline__ void le_force_ch_site(const Smem<double, S> & s, cons
 }
 
 // step_tile() over the x-planes [xb, xe) of one slab (unwrapped x); the
-// force is also stored (force_mode 0: fzero + F)
-template <class S, int ORDER, bool GZ, bool GMU>
+// force is also stored (force_mode 0: fzero + F).  EDGE (the edge launch):
+// every site is a slab edge, and the LB plane reprojection of its
+// post-collision distributions goes to L.sbuf (x = slab_lo: side 1 of
+// plane lo; x = slab_hi: side 0 of plane hi).
+template <class S, int ORDER, bool GZ, bool GMU, bool EDGE = false>
 __device__ __forceinline__ void le_step_tile(Smem<double, S> & s, const Geom & g, const FeParams & fe,
 					     const LbParams & prm, const LeSlab & L, int y0, int z0,
 					     int xb, int xe, const double * __restrict__ phi,
@@ -297,6 +303,16 @@ __device__ __forceinline__ void le_step_tile(Smem<double, S> & s, const Geom & g
       force[2*g.nsite + site] = fz2 + F[2];
       double r, uu[3];
       site::collide_site<double, GZ>(prm, fs, fc, r, uu);
+      if constexpr (EDGE) {
+	const int side = (x == L.slab_lo) ? 1 : 0;
+	const int p = side ? L.lo : L.hi;
+	double out[5];
+	le_reproject_site(fc, side, side ? L.duy1 : L.duy0, out);
+	const int64_t nyz = (int64_t) g.ny*g.nz;
+	const int64_t rem = (int64_t) y*g.nz + z;
+#pragma unroll
+	for (int ip = 0; ip < 5; ip++) L.sbuf[((int64_t) (2*p + side)*5 + ip)*nyz + rem] = out[ip];
+      }
       site::store_planes<double>(g_out, g.nsite, site, fc);
       rho[site] = r;
       u_next[site] = uu[0];

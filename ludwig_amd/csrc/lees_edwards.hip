@@ -44,6 +44,7 @@
 #include "d3q19.h"
 #include "grad27.h"
 #include "kernels.h"
+#include "le_lb.h"
 #include "le_plane_sum.h"
 
 namespace ldw {
@@ -574,25 +575,7 @@ le_ch_update_kernel(Geom g, LeGeom le, FeParams fe, ChFields C, const double * _
 /* Lattice Boltzmann plane conditions                                     */
 /* ---------------------------------------------------------------------- */
 
-struct Props {
-  int p[2][5];            // cx = +1 (side 0) and cx = -1 (side 1), in p order
-};
-
-constexpr Props make_props() {
-  Props t{};
-  int n0 = 0, n1 = 0;
-  for (int p = 1; p < NVEL; p++) {
-    if (CV.v[p][0] == +1) t.p[0][n0++] = p;
-    if (CV.v[p][0] == -1) t.p[1][n1++] = p;
-  }
-  return t;
-}
-
-constexpr Props PROPS = make_props();
-
-// le_reproject (model_le.c:245-330): moments of the post-collision f at the
-// plane site, then f_p + w_p (rho u.c / cs2 + ds:Q / 2 cs2^2) for the
-// crossing p, with the jump du = (0, duy, 0)
+// le_reproject (model_le.c:245-330) at every plane site (le_lb.h)
 __global__ void __launch_bounds__(LB)
 le_lb_reproject_kernel(Geom g, LeGeom le, double duy0, double duy1,
 		       const double * __restrict__ gd, double * __restrict__ sbuf) {
@@ -606,37 +589,10 @@ le_lb_reproject_kernel(Geom g, LeGeom le, double duy0, double duy1,
   double f[NVEL];
 #pragma unroll
   for (int q = 0; q < NVEL; q++) f[q] = gd[q*g.nsite + site];
-  double rho = 0.0;
+  double out[5];
+  le_reproject_site(f, side, side ? duy1 : duy0, out);
 #pragma unroll
-  for (int q = 0; q < NVEL; q++) rho += f[q];
-  double gm[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-  for (int q = 0; q < NVEL; q++)
-#pragma unroll
-    for (int n = 0; n < 3; n++) gm[n] += CV.v[q][n]*f[q];
-  const double du[3] = {0.0, side ? duy1 : duy0, 0.0};
-  double ds[3][3];
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) ds[a][b] = (gm[a]*du[b] + du[a]*gm[b] + rho*du[a]*du[b]);
-  const double cs2 = (1.0/3.0);
-  const double rcs2 = 1.0/cs2;
-#pragma unroll
-  for (int ip = 0; ip < 5; ip++) {
-    const int q = side ? PROPS.p[1][ip] : PROPS.p[0][ip];
-    const double udotc = du[1]*CV.v[q][1];
-    double sdotq = 0.0;
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-      for (int b = 0; b < 3; b++) {
-	const double dab = cs2*(a == b);
-	const double qq = (CV.v[q][a]*CV.v[q][b] - dab);
-	sdotq += ds[a][b]*qq;
-      }
-    double fq = side ? f[PROPS.p[1][ip]] : f[PROPS.p[0][ip]];
-    fq += weight(q)*(rho*udotc*rcs2 + 0.5*sdotq*rcs2*rcs2);
-    sbuf[((int64_t) ps*5 + ip)*nyz + rem] = fq;
-  }
+  for (int ip = 0; ip < 5; ip++) sbuf[((int64_t) ps*5 + ip)*nyz + rem] = out[ip];
 }
 
 // lb_data_displace_kernel + lb_data_interpolate_kernel (model_le.c:
@@ -662,7 +618,7 @@ le_lb_interp_kernel(Geom g, LeGeom le, int dj0, int dj1, double fr0, double fr1,
   for (int ip = 0; ip < 5; ip++) {
     const double * s = sbuf + ((int64_t) ps*5 + ip)*nyz;
     const double fq = (1.0 - fr)*s[(int64_t) js0*g.nz + z] + fr*s[(int64_t) js1*g.nz + z];
-    const int q = side ? PROPS.p[1][ip] : PROPS.p[0][ip];
+    const int q = side ? LE_PROPS.p[1][ip] : LE_PROPS.p[0][ip];
     gd[q*g.nsite + site] = fq;
   }
 }
@@ -775,6 +731,13 @@ void launch_le_lb_bc(const Geom & g, const LeGeom & le, const int dj[2], const d
   const int64_t nf = (int64_t) le.nplane*2*g.ny*g.nz;
   hipLaunchKernelGGL(le_lb_reproject_kernel, dim3(nblk(nf)), dim3(LB), 0, s, g, le, duy[0], duy[1],
 		     gdist, sbuf);
+  hipLaunchKernelGGL(le_lb_interp_kernel, dim3(nblk(nf)), dim3(LB), 0, s, g, le, dj[0], dj[1],
+		     fr[0], fr[1], sbuf, gdist);
+}
+
+void launch_le_lb_interp(const Geom & g, const LeGeom & le, const int dj[2], const double fr[2],
+			 double * gdist, const double * sbuf, hipStream_t s) {
+  const int64_t nf = (int64_t) le.nplane*2*g.ny*g.nz;
   hipLaunchKernelGGL(le_lb_interp_kernel, dim3(nblk(nf)), dim3(LB), 0, s, g, le, dj[0], dj[1],
 		     fr[0], fr[1], sbuf, gdist);
 }

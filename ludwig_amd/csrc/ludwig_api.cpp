@@ -2237,15 +2237,16 @@ static int step_le_fused(lb_t * lb, hydro_t * hydro, field_t * phi, fe_symm_t * 
   {
     KTimer t("le_step");
     ldw::launch_le_step(g, p, q, le->geo, phi->d_phib, hydro->d_ub, le->d_fcor, le->d_chface,
-			ra, jw, dyw - jw, je, dye - je, phi0, (double *) phi->d_data[1 - phi->cur],
+			ra, jw, dyw - jw, je, dye - je, duy, le->d_sbuf, phi0, (double *) phi->d_data[1 - phi->cur],
 			(const double *) hydro->d_u, (double *) hydro->d_u2, (double *) hydro->d_rho,
 			(const double *) lb->d_g[lb->cur], (double *) lb->d_g[1 - lb->cur],
 			(double *) hydro->d_force, (double *) pch->d_csum, g_stream, g_ev_join);
   }
   CHECK_LAUNCH();
   {
+    // the edge launch formed the reprojections; displace and interpolate
     KTimer t("le_lb_bc");
-    ldw::launch_le_lb_bc(g, le->geo, dj, fr, duy, (double *) lb->d_g[1 - lb->cur], le->d_sbuf, g_stream);
+    ldw::launch_le_lb_interp(g, le->geo, dj, fr, (double *) lb->d_g[1 - lb->cur], le->d_sbuf, g_stream);
   }
   CHECK_LAUNCH();
   phi->cur = 1 - phi->cur;
