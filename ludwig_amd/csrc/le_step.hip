@@ -8,10 +8,11 @@
 // force correction sums (a serial chain each, le_plane_sum.h) run beside
 // it on a second stream in blocks small enough to share its CUs.  Launch 2
 // forms the edge slices, whose force and Cahn-Hilliard x-faces take the
-// plane values.  Before them the
-// small plane kernels of lees_edwards.hip fill the buffer planes (phi,
-// gradient next to the planes, u) and the raw x-faces at the planes; after
-// them the LB plane conditions act on the post-collision distributions.
+// plane values, and the LB plane reprojection of their post-collision
+// distributions (le_lb.h).  Before them the small plane kernels of
+// lees_edwards.hip fill the buffer planes (phi, gradient next to the
+// planes, u) and the raw x-faces at the planes; after them one more
+// displaces and interpolates the reprojections into the plane sites.
 
 #include <hip/hip_runtime.h>
 
