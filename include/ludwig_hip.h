@@ -589,7 +589,9 @@ long long ldw_ensemble_time(ldw_ensemble_t * ens);
 int ldw_ensemble_snapshot(ldw_ensemble_t * ens, int r, double * phi, double * rho,
 			  double * u);
 int ldw_ensemble_f_read(ldw_ensemble_t * ens, int r, double * f);
-/* phi, ux, uy, uz of replica r as 4 nsite floats into device memory dst */
+/* phi, ux, uy, uz of replica r as 4 nsite floats into device memory dst,
+ * ordered on the library's stream (the library's later calls see it; the
+ * host or another stream waits with ldw_device_synchronize) */
 int ldw_ensemble_snapshot_device(ldw_ensemble_t * ens, int r, float * dst);
 /* Compensated FP64 sums of replica r: out[0] = total density, out[1] = sum phi */
 int ldw_ensemble_conserved_sums(ldw_ensemble_t * ens, int r, double out[2]);

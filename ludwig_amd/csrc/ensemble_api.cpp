@@ -478,7 +478,8 @@ extern "C" int ldw_ensemble_snapshot_device(ldw_ensemble_t * e, int r, float * d
     ldw::launch_to_f32(u, 64, dst + n, 3*n, st);
     LDW_HIP_CHECK(hipGetLastError());
   }
-  LDW_HIP_CHECK(hipStreamSynchronize(st));
+  // ordered on the library stream like every other call: no host wait
+  // (a jump snapshots every replica in turn)
   return 0;
 }
 
