@@ -219,6 +219,25 @@ __global__ void encode_kernel(const float * __restrict__ state, int64_t nsite, E
   }
 }
 
+// the same, one thread per site, for c a multiple of 8 and out 16-byte
+// aligned: the site's channels leave as 16-byte vectors (the element kernel
+// divides a 64-bit index per channel and stores 2 bytes at a time); the
+// values are the element kernel's
+__global__ void encode_site_kernel(const float * __restrict__ state, int64_t nsite, EncodeNorm e,
+				   uint4 * __restrict__ out, int c8) {
+  GRID_STRIDE(s, nsite) {
+    const uint32_t h0 = bf16_of((state[s] - e.phi_shift)*e.phi_scale);
+    const uint32_t h1 = bf16_of(state[nsite + s]*e.u_scale);
+    const uint32_t h2 = bf16_of(state[2*nsite + s]*e.u_scale);
+    const uint32_t h3 = bf16_of(state[3*nsite + s]*e.u_scale);
+    const uint32_t h4 = bf16_of(e.prm[0]), h5 = bf16_of(e.prm[1]), h6 = bf16_of(e.prm[2]);
+    const uint32_t h7 = bf16_of(0.0f);
+    uint4 * o = out + s*c8;
+    o[0] = make_uint4(h0 | (h1 << 16), h2 | (h3 << 16), h4 | (h5 << 16), h6 | (h7 << 16));
+    for (int k = 1; k < c8; k++) o[k] = make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+
 // training target: scaled change of (phi, u) over the jump, channels-last [nsite][4]
 __global__ void target_kernel(const float * __restrict__ now, const float * __restrict__ next,
 			      int64_t nsite, float * __restrict__ tgt, float dphi_scale, float du_scale) {
@@ -241,6 +260,26 @@ __global__ void decode_kernel(const float * __restrict__ now, const float * __re
     const float v = now[i] + y[s*cy + ch]/(ch == 0 ? dphi_scale : du_scale);
     const float m = (ch == 0) ? phi_max : u_max;
     out[i] = fminf(fmaxf(v, -m), m);
+  }
+}
+
+// the same, one thread per site, for cy a multiple of 4 and y 16-byte
+// aligned: the site's four outputs are one 16-byte load (the element
+// kernel reads them 4 bytes at a time, cy floats apart); the values are
+// the element kernel's
+__global__ void decode_site_kernel(const float * __restrict__ now, const float4 * __restrict__ y, int cy4,
+				   int64_t nsite, float * __restrict__ out, float dphi_scale, float du_scale,
+				   float phi_max, float u_max) {
+  GRID_STRIDE(s, nsite) {
+    const float4 yv = y[s*cy4];
+    const float d[4] = {yv.x, yv.y, yv.z, yv.w};
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) {
+      const int64_t i = (int64_t) ch*nsite + s;
+      const float v = now[i] + d[ch]/(ch == 0 ? dphi_scale : du_scale);
+      const float m = (ch == 0) ? phi_max : u_max;
+      out[i] = fminf(fmaxf(v, -m), m);
+    }
   }
 }
 
@@ -455,7 +494,12 @@ hipError_t nn_to_bf16(const float * a, uint16_t * b, int64_t n, hipStream_t s) {
 
 hipError_t nn_encode(const float * state, int64_t nsite, const EncodeNorm & e, uint16_t * out, int c,
 		     hipStream_t s) {
-  LAUNCH(encode_kernel, nsite*c, state, nsite, e, out, c);
+  if (c % 8 == 0 && ((uintptr_t) out % 16) == 0) {
+    LAUNCH(encode_site_kernel, nsite, state, nsite, e, reinterpret_cast<uint4 *>(out), c/8);
+  }
+  else {
+    LAUNCH(encode_kernel, nsite*c, state, nsite, e, out, c);
+  }
   return hipGetLastError();
 }
 
@@ -467,7 +511,13 @@ hipError_t nn_target(const float * now, const float * next, int64_t nsite, float
 
 hipError_t nn_decode(const float * now, const float * y, int cy, int64_t nsite, float * out,
 		     float dphi_scale, float du_scale, float phi_max, float u_max, hipStream_t s) {
-  LAUNCH(decode_kernel, nsite*4, now, y, cy, nsite, out, dphi_scale, du_scale, phi_max, u_max);
+  if (cy % 4 == 0 && ((uintptr_t) y % 16) == 0) {
+    LAUNCH(decode_site_kernel, nsite, now, reinterpret_cast<const float4 *>(y), cy/4, nsite, out, dphi_scale,
+	   du_scale, phi_max, u_max);
+  }
+  else {
+    LAUNCH(decode_kernel, nsite*4, now, y, cy, nsite, out, dphi_scale, du_scale, phi_max, u_max);
+  }
   return hipGetLastError();
 }
 

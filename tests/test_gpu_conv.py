@@ -293,3 +293,36 @@ def test_binary_fluid_commutes_with_cubic_maps(perm, flip):
     assert np.max(np.abs(got[1:] - want[1:])) < 1e-13
     # and the map is not the identity on this state
     assert np.max(np.abs(want[0] - s[0])) > 0.01 or perm == (0, 1, 2) and not any(flip)
+
+
+@pytest.mark.parametrize("c,cy", [(16, 16), (12, 6)])
+def test_encode_decode_against_numpy(c, cy):
+    """The network's input encoding and the jump's decoding, bit for bit
+    against float32 numpy: per-site kernels (channel counts that allow
+    16-byte vectors) and element kernels (those that do not)."""
+    rng = np.random.default_rng(c * 100 + cy)
+    ns = 1000
+    state = rng.standard_normal(4 * ns).astype(np.float32)
+    norm = np.array([0.1, 1.7, 25.0, 0.3, -0.2, 0.9], np.float32)
+    print(f"encode/decode c={c} cy={cy}", flush=True)
+    ds, dx = dev(state, np.float32), L.DeviceArray(ns * c, np.uint16)
+    lib = L.lib()
+    assert lib.ldw_nn_encode(vp(ds), ns, norm.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), vp(dx), c) == 0
+    want = np.zeros((ns, c), np.float32)
+    want[:, 0] = (state[:ns] - norm[0]) * norm[1]
+    for k in range(1, 4):
+        want[:, k] = state[k * ns:(k + 1) * ns] * norm[2]
+    want[:, 4:7] = norm[3:6]
+    assert np.array_equal(dx.download().reshape(ns, c), bf16_bits(want)), "encode"
+    y = rng.standard_normal(ns * cy).astype(np.float32)
+    dy, do = dev(y, np.float32), L.DeviceArray(4 * ns, np.float32)
+    sc = np.float32(0.05), np.float32(0.002)
+    assert lib.ldw_nn_decode(vp(ds), vp(dy), cy, ns, vp(do), sc[0], sc[1], 1.5, 0.1) == 0
+    got = do.download().reshape(4, ns)
+    yy = y.reshape(ns, cy)
+    for k in range(4):
+        m = np.float32(1.5 if k == 0 else 0.1)
+        v = state[k * ns:(k + 1) * ns] + yy[:, k] / (sc[0] if k == 0 else sc[1])
+        assert np.array_equal(got[k], np.minimum(np.maximum(v, -m), m)), f"decode channel {k}"
+    for d in (ds, dx, dy, do):
+        d.free()

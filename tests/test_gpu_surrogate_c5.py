@@ -13,7 +13,7 @@ deterministic (fixed-order reductions, tile shapes that change only which
 block computes a site), so the errors below are reproducible to the last
 digit; only the speedup varies with the machine.
 
-Measured on MI355X: speedup 5.8x, ensemble-mean L(T) error 0.39 %, mean
+Measured on MI355X: speedup 6.0-6.1x, ensemble-mean L(T) error 0.39 %, mean
 per-replica |L(T) error| 2.9 % (max 10.7 %).  The per-replica 2 % target
 is not met; the bounds assert what is."""
 
